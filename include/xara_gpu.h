@@ -1,0 +1,220 @@
+/*
+ * xara_gpu.h -- C-ABI of the MI355X state-determination library.
+ *
+ * One handle (xg_domain) holds one finite-element model resident on one GPU:
+ * nodes, fixities, materials, sections, elements, nodal reference loads, the
+ * equation numbering, the fixed-pattern CSR system and all trial/committed
+ * material state in structure-of-arrays form.  Every call returns an int
+ * status: 0 = success, negative = failure (the reference convention of
+ * Domain::update / LinearSOE::solve, Domain.cpp:1996-2003), and the last error
+ * message is available from xg_last_error().
+ *
+ * Arrays are plain host pointers with explicit sizes.  Node-indexed arrays are
+ * in ascending node-tag order (the order of the reference's node map and of
+ * PlainHandler's DOF groups, PlainHandler.cpp:60-120); element-indexed arrays
+ * are in ascending element-tag order (FE_Element order, PlainHandler.cpp:212).
+ *
+ * Each declaration names the reference interface it replaces.  The path is
+ * xara's nonlinear state determination: Element::update / getTangentStiff /
+ * getResistingForce of stdBrick, FourNodeQuad, DispBeamColumn2d/3d over
+ * J2Plasticity, Steel02, Concrete02 and fiber sections, assembled into a CSR
+ * LinearSOE and solved by preconditioned CG on the GPU.
+ */
+#ifndef XARA_GPU_H
+#define XARA_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct xg_domain xg_domain;
+
+/* ---- status codes (SolutionAlgorithm codes, NewtonRaphson.cpp:139-219) ---- */
+#define XG_OK                 0
+#define XG_ERR_ARG           -1   /* bad argument / unknown tag                */
+#define XG_ERR_STATE         -2   /* call out of order (e.g. before setup)     */
+#define XG_ERR_DEVICE        -3   /* HIP runtime failure                       */
+#define XG_ERR_MATERIAL      -4   /* return map failed (J2Plasticity.cpp:309)  */
+#define XG_BAD_LINEAR_SOLVE  -5   /* LinearSOE::solve < 0                      */
+#define XG_TEST_FAILED       -6   /* ConvergenceTest::Failure                  */
+#define XG_BAD_STEP          -7   /* integrator newStep/update failed          */
+
+/* ---- enumerations --------------------------------------------------------- */
+/* system (soe_table, SRC/runtime/commands/analysis/solver.hpp:113) */
+#define XG_SYSTEM_PCG_JACOBI       1  /* CSR, Jacobi-preconditioned CG           */
+#define XG_SYSTEM_PCG_BLOCKJACOBI  2  /* CSR, nodal block-Jacobi CG             */
+#define XG_SYSTEM_DENSE            3  /* dense LU (small models, testing)       */
+/* numberer (SRC/runtime/commands/analysis/numberer.cpp) */
+#define XG_NUMBERER_RCM    1          /* DOF_Numberer(RCM(false)), the default  */
+#define XG_NUMBERER_PLAIN  2          /* PlainNumberer                          */
+/* test (SRC/analysis/criteria) */
+#define XG_TEST_NORM_UNBALANCE  1     /* CTestNormUnbalance.cpp:81              */
+#define XG_TEST_NORM_DISP_INCR  2     /* CTestNormDispIncr.cpp:78               */
+#define XG_TEST_ENERGY_INCR     3     /* CTestEnergyIncr.cpp:85                 */
+/* per-Gauss-point / per-section outputs for xg_get_point_data */
+#define XG_OUT_STRESS      1          /* NDMaterial::getStress / section s      */
+#define XG_OUT_TANGENT     2          /* NDMaterial::getTangent / section ks    */
+#define XG_OUT_STRAIN      3          /* NDMaterial::getStrain / section e      */
+#define XG_OUT_PLASTIC     4          /* J2: eps_p(6) + xi trial                */
+/* operations for xg_time_op */
+#define XG_OP_UPDATE       1
+#define XG_OP_TANGENT      2
+#define XG_OP_UNBALANCE    3
+#define XG_OP_SPMV         4
+
+typedef struct xg_counters {
+  long long updates;          /* Domain::update calls                         */
+  long long material_points;  /* Gauss points + fibers updated, cumulative    */
+  long long newton_iters;     /* NewtonRaphson iterations                     */
+  long long solves;           /* LinearSOE::solve calls                        */
+  long long cg_iters;         /* cumulative PCG iterations                    */
+  double    last_cg_relres;   /* last PCG relative residual                   */
+  double    last_norm;        /* last convergence-test norm                   */
+  double    time_update_ms;   /* device time, cumulative                      */
+  double    time_tangent_ms;
+  double    time_unbalance_ms;
+  double    time_solve_ms;
+} xg_counters;
+
+/* ---- lifetime -------------------------------------------------------------- */
+/* `model basic -ndm ndm -ndf ndf` (BasicModelBuilder, SRC/runtime/runtime/BasicModelBuilder.h) */
+int  xg_create(int ndm, int ndf, int device, xg_domain** out);
+int  xg_destroy(xg_domain* d);
+const char* xg_last_error(void);
+int  xg_device_name(char* buf, int len);
+
+/* ---- model definition (Tcl commands; Domain::addNode/addElement) ---------- */
+/* `node tag x y <z>`  (SRC/domain/node/Node.cpp constructor)                  */
+int  xg_nodes(xg_domain* d, int n, const int* tags, const double* crds /* n*ndm */);
+/* `fix tag f1 .. fndf` -> SP_Constraints seen by PlainHandler.cpp:80-110     */
+int  xg_fix(xg_domain* d, int n, const int* tags, const int* flags /* n*ndf */);
+/* `mass tag m1 .. mndf` (Node::setMass)                                       */
+int  xg_mass(xg_domain* d, int n, const int* tags, const double* m /* n*ndf */);
+/* `nDMaterial J2Plasticity tag K G sig0 sigInf delta H <eta>`
+   (plastic.cpp:616 -> J2Plasticity.cpp:83-110)                                */
+int  xg_nd_j2plasticity(xg_domain* d, int tag, double K, double G, double sig0,
+                        double sigInf, double delta, double H, double eta);
+/* `uniaxialMaterial Steel02 tag fy E b R0 cR1 cR2 a1 a2 a3 a4 sigInit`
+   (fedeas.cpp:578 -> Steel02.cpp:44-54)                                      */
+int  xg_uniaxial_steel02(xg_domain* d, int tag, double fy, double E0, double b,
+                         double R0, double cR1, double cR2, double a1, double a2,
+                         double a3, double a4, double sigInit);
+/* `uniaxialMaterial Concrete02 tag fpc epsc0 fpcu epscu rat ft Ets`
+   (Concrete02.cpp:52-112)                                                    */
+int  xg_uniaxial_concrete02(xg_domain* d, int tag, double fc, double epsc0,
+                            double fcu, double epscu, double rat, double ft,
+                            double Ets);
+/* `uniaxialMaterial Elastic tag E` (ElasticMaterial), used for -GJ torsion    */
+int  xg_uniaxial_elastic(xg_domain* d, int tag, double E);
+/* `section Fiber tag <-GJ v> {fiber y z A mat ...}`
+   (section.cpp:443/497 -> FiberSection2d.cpp:99 / FiberSection3d.cpp:294).
+   For a 2D section z is ignored (may be NULL).  GJ < 0: no torsion.
+   compute_centroid = 0 is `-noCentroid`.                                     */
+int  xg_section_fiber(xg_domain* d, int tag, int nfib, const int* mat_tags,
+                      const double* y, const double* z, const double* A,
+                      double GJ, int compute_centroid);
+/* `geomTransf Linear tag <vecxz>` (geomTransf.cpp:350/468 ->
+   LinearCrdTransf2d/3d).  vecxz is ignored for ndm = 2.                      */
+int  xg_geom_transf_linear(xg_domain* d, int tag, const double* vecxz);
+/* `element stdBrick tag n1..n8 mat <b1 b2 b3>` (brick.cpp:25,126 -> Brick.cpp:77) */
+int  xg_elements_brick(xg_domain* d, int n, const int* tags, const int* conn /* n*8 */,
+                       int mat_tag, double b1, double b2, double b3);
+/* `element quad tag n1..n4 thk PlaneStrain mat <pressure rho b1 b2>`
+   (plane.cpp:99,249,395 -> FourNodeQuad.cpp:36)                              */
+int  xg_elements_quad(xg_domain* d, int n, const int* tags, const int* conn /* n*4 */,
+                      double thk, int mat_tag, double pressure, double rho,
+                      double b1, double b2);
+/* `element dispBeamColumn tag i j transf "Legendre" sec nip <-mass rho>`
+   (frames.cpp:449,906 -> DispBeamColumn2d.cpp:48 / DispBeamColumn3d.cpp) */
+int  xg_elements_dispbeam(xg_domain* d, int n, const int* tags, const int* conn /* n*2 */,
+                          int transf_tag, int sec_tag, int nip, double rho);
+/* `pattern Plain 1 Linear { load tag P1..Pndf }` (NodalLoad; LoadPattern
+   with a Linear series, applied as lambda*P by Domain::applyLoad,
+   Domain.cpp:1765)                                                           */
+int  xg_nodal_loads(xg_domain* d, int n, const int* tags, const double* P /* n*ndf */);
+/* `pattern UniformExcitation tag dir -accel {Path -dt dt -values ...}`
+   (UniformExcitation.cpp:132, PathSeries.cpp:334).  dir is 0-based.         */
+int  xg_uniform_excitation(xg_domain* d, int dir, int npts, double dt,
+                           const double* accel, double factor);
+/* `rayleigh alphaM betaK betaKinit betaKcomm` (only alphaM, betaKcomm used) */
+int  xg_rayleigh(xg_domain* d, double alphaM, double betaK, double betaKinit,
+                 double betaKcomm);
+
+/* ---- analysis objects (BasicAnalysisBuilder::set, BasicAnalysisBuilder.cpp:781-822) */
+/* `system ...` (solver.hpp:113 soe_table)                                    */
+int  xg_system(xg_domain* d, int kind, double rtol, int maxit);
+/* `numberer RCM|Plain`                                                       */
+int  xg_numberer(xg_domain* d, int kind);
+/* `test NormUnbalance tol maxIter` etc.                                      */
+int  xg_test(xg_domain* d, int kind, double tol, int max_iter);
+/* `integrator LoadControl dlambda` (LoadControl.cpp:70-120)                  */
+int  xg_integrator_load_control(xg_domain* d, double dlambda);
+/* `integrator DisplacementControl node dof incr` (DisplacementControl.cpp:121-283);
+   dof is 1-based as in the Tcl command.                                      */
+int  xg_integrator_disp_control(xg_domain* d, int node_tag, int dof, double incr);
+/* `integrator Newmark gamma beta` (Newmark.cpp:105-458)                      */
+int  xg_integrator_newmark(xg_domain* d, double gamma, double beta);
+/* `analyze n <dt>`: n steps of NewtonRaphson (NewtonRaphson.cpp:125-221)
+   under the chosen integrator.  Returns 0 or the failing status.            */
+int  xg_analyze(xg_domain* d, int nsteps, double dt);
+
+/* ---- domainChanged: numbering and CSR pattern (bit-exact) ----------------- */
+/* PlainHandler::handle + DOF_Numberer::numberDOF (RCM.cpp:67-259) +
+   AnalysisModel::getDOFGraph + SparseGenRowLinSOE::setSize
+   (SparseGenRowLinSOE.cpp:129-234).  Also builds the assembly scatter maps. */
+int  xg_setup(xg_domain* d);
+int  xg_num_nodes(xg_domain* d);
+int  xg_num_elements(xg_domain* d);
+int  xg_num_eqn(xg_domain* d);
+long long xg_csr_nnz(xg_domain* d);
+/* equation number of every node DOF (node-tag order, n*ndf), -1 for fixed
+   (DOF_Group::getID after numberDOF, DOF_Numberer.cpp:120-133)               */
+int  xg_node_eqns(xg_domain* d, int* eqn);
+/* CSR pattern: rowptr (neq+1), colind (nnz)                                  */
+int  xg_csr_pattern(xg_domain* d, int* rowptr, int* colind);
+
+/* ---- state determination: the hot path ------------------------------------ */
+/* Node::setTrialDisp for every node (n*ndf, node-tag order)                   */
+int  xg_set_trial_disp(xg_domain* d, const double* U);
+int  xg_get_trial_disp(xg_domain* d, double* U);
+/* AnalysisModel::incrDisp (AnalysisModel.cpp:452): U += dU (dU per equation) */
+int  xg_incr_disp(xg_domain* d, const double* dU);
+/* Domain::update (Domain.cpp:1988-2006): Element::update of every element,
+   every Gauss point / fiber trial state.                                     */
+int  xg_update(xg_domain* d);
+/* IncrementalIntegrator::formTangent (IncrementalIntegrator.cpp:74-98) with
+   SparseGenRowLinSOE::addA (SparseGenRowLinSOE.cpp:237-291): K into the CSR
+   values; copied to vals (nnz) when vals != NULL.                            */
+int  xg_form_tangent(xg_domain* d, double* vals);
+/* StaticIntegrator::formUnbalance (StaticIntegrator.cpp:51-70): B = -sum R_e
+   + lambda P; copied to B (neq) when B != NULL.                              */
+int  xg_form_unbalance(xg_domain* d, double* B);
+/* LinearSOE::solve (LinearSOE.h:46-89) of the assembled system with right-hand
+   side B (neq) into X (neq).                                                 */
+int  xg_solve(xg_domain* d, const double* B, double* X);
+/* y = A x with the assembled CSR values (for tests)                          */
+int  xg_spmv(xg_domain* d, const double* x, double* y);
+/* Domain::commit (Domain.cpp:1893-1923) / revertToLastCommit / revertToStart */
+int  xg_commit(xg_domain* d);
+int  xg_revert_to_last_commit(xg_domain* d);
+int  xg_revert_to_start(xg_domain* d);
+/* Load factor lambda (Domain::applyLoad) and pseudo-time.                    */
+int  xg_set_load_factor(xg_domain* d, double lambda);
+double xg_get_load_factor(xg_domain* d);
+
+/* ---- element / material point read-back (Element::getResponse, print) ----- */
+/* Per-point data in element-tag order, points in element quadrature order.
+   what = XG_OUT_*; returns the number of doubles written per point in *width. */
+int  xg_get_point_data(xg_domain* d, int what, double* out, int* width);
+/* Element::getTangentStiff / getResistingForce of one element (by tag).      */
+int  xg_element_tangent(xg_domain* d, int ele_tag, double* K, int* ndof);
+int  xg_element_resisting_force(xg_domain* d, int ele_tag, double* R, int* ndof);
+int  xg_get_counters(xg_domain* d, xg_counters* c);
+/* Device time of `reps` back-to-back launches of op (XG_OP_*), measured with
+   HIP events on the stream the kernels run on.  *ms = average per launch.    */
+int  xg_time_op(xg_domain* d, int op, int reps, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XARA_GPU_H */
