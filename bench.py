@@ -1,0 +1,141 @@
+"""Benchmark: J2 material-point update rate of the stdBrick state
+determination on one MI355X.
+
+Workload: the 64^3 stdBrick cube (274,625 nodes, 262,144 elements,
+2,097,152 Gauss points) of J2Plasticity (K=166.7e3, G=76.9e3, sigma_y=250,
+H=2000), FP64.  One step is one Domain::update over the whole mesh: every
+element gathers its nodal coordinates and trial displacements, every Gauss
+point forms its strain and runs the J2 return map, writing plastic state,
+stress and the consistent tangent.  The trial displacement is a fixed
+synthetic field that leaves part of the cube plastic; inputs are resident in
+HBM before the timed region.
+
+Prints one JSON line (value = material-point updates per second) with the
+roofline of the update kernel (algorithmic bytes / HIP-event time) and the
+CPU oracle timed on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def update_bytes(nnode, nele, npts):
+    # per point: read eps_p,n (6) + xi_n (1); write eps_p,n+1 (6) + xi (1) +
+    # stress (6) + tangent upper triangle (21) = 41 doubles = 328 B;
+    # per node: coordinates + trial displacement = 48 B; per element: 8 int32
+    return npts * 328 + nnode * 48 + nele * 32
+
+
+def synthetic_field(mesh):
+    x, y, z = mesh["crds"].T
+    eps0 = 2.0e-3
+    mod = 1.0 + 0.6 * np.sin(2 * np.pi * x) * np.cos(np.pi * y)
+    U = np.zeros((len(x), 3))
+    U[:, 2] = -eps0 * z * mod
+    U[:, 0] = 0.3 * eps0 * (x - 0.5) * z
+    U[:, 1] = 0.3 * eps0 * (y - 0.5) * z * mod
+    return U
+
+
+def cpu_baseline(n_sample, seconds):
+    from tests import _oracle
+    from xara_amd import meshes
+    mesh = meshes.brick_cube(n_sample)
+    o = _oracle.oracle_model(3, 3)
+    meshes.build_brick_cube(o, mesh, pressure=0.0)
+    o.setup()
+    o.set_trial_disp(synthetic_field(mesh))
+    o.update()
+    reps = 1
+    ms = o.time_op("update", reps)
+    reps = max(1, int(seconds * 1000.0 / max(ms, 1e-3)))
+    ms = o.time_op("update", reps)
+    npts = 8 * n_sample ** 3
+    return dict(value=npts / (ms * 1e-3), unit="points/s", cores=1, kind="port",
+                sample=f"{reps} serial updates of a {n_sample}^3 stdBrick J2 cube ({npts} points) "
+                       f"with the same synthetic field, oracle/liboracle.so (-O2), one thread")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=64, help="cube edge in elements")
+    ap.add_argument("--cpu-n", type=int, default=24, help="cube edge of the CPU sample")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    from xara_amd import meshes
+    from xara_amd.model import Model
+
+    n = args.n
+    print(f"bench: building {n}^3 stdBrick cube", file=sys.stderr, flush=True)
+    mesh = meshes.brick_cube(n)
+    m = Model(3, 3, device=0)
+    meshes.build_brick_cube(m, mesh, pressure=0.0)
+    t0 = time.time()
+    m.setup()
+    setup_s = time.time() - t0
+    nnode, nele = len(mesh["node_tags"]), len(mesh["ele_tags"])
+    npts = 8 * nele
+    m.set_trial_disp(synthetic_field(mesh))
+    print(f"bench: setup {setup_s:.1f}s neq={m.neq} nnz={m.nnz}; warmup", file=sys.stderr, flush=True)
+    m.time_op("update", args.warmup)
+    # timed region: exactly `steps` updates on the library stream, bracketed
+    # by device synchronisation (time_op synchronises before returning)
+    t0 = time.perf_counter()
+    ev_ms = m.time_op("update", args.steps)
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    value = npts * args.steps / wall
+    bytes_per = update_bytes(nnode, nele, npts)
+    achieved = bytes_per / (ev_ms * 1e-3) / 1e9
+    plastic = float(np.mean(m.point_data("plastic", npts)[:, 6] > 0))
+    print(f"bench: {ms_per_step:.3f} ms/step (event {ev_ms:.3f} ms), plastic fraction {plastic:.2f}",
+          file=sys.stderr, flush=True)
+    out = {
+        "metric": "material-point updates/s",
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"stdBrick J2 cube {n}^3 update (Domain::update)",
+                   "elements": nele, "nodes": nnode, "points": npts,
+                   "plastic_fraction": round(plastic, 3), "setup_s": round(setup_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_per, "event_ms": ev_ms},
+    }
+    if not args.no_cpu:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        cb = cpu_baseline(args.cpu_n, args.cpu_seconds)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

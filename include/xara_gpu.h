@@ -61,6 +61,7 @@ typedef struct xg_domain xg_domain;
 #define XG_OP_TANGENT      2
 #define XG_OP_UNBALANCE    3
 #define XG_OP_SPMV         4
+#define XG_OP_COPY         5          /* 256 MiB device copy: HBM counter calibration */
 
 typedef struct xg_counters {
   long long updates;          /* Domain::update calls                         */

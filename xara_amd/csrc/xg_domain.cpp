@@ -1,0 +1,615 @@
+// Host driver of the GPU state-determination path: model definition, setup
+// (numbering, CSR pattern, gather-assembly maps), the batched element
+// operations, the PCG LinearSOE and the Newton / LoadControl /
+// DisplacementControl analysis loop over device-resident vectors.
+#include "xg_domain.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "xg_frame.h"
+#include "xg_kernels.h"
+
+namespace xg {
+
+// ---------------------------------------------------------------------------
+// J2 continuum groups: stdBrick (3D) and FourNodeQuad (plane strain)
+// ---------------------------------------------------------------------------
+struct J2Group : Group {
+  bool brick = true;
+  std::vector<int> mat_h;        // index into the J2 table
+  std::vector<double> b_h;       // body force per element
+  std::vector<double> thk_h;     // quad thickness
+  bool has_b = false;
+  DVec<int> dconn, dmat;
+  DVec<J2Param> dprm;
+  DVec<double> db, dthk;
+  DVec<double> epn, xin, ept, xit, sig, D;
+  int ncomp() const { return brick ? 6 : 4; }
+  int nsig() const { return brick ? 6 : 3; }
+  int ntan() const { return brick ? 21 : 6; }
+  int npts() const override { return brick ? 8 : 4; }
+  long long np() const { return (long long)nele() * npts(); }
+
+  void setup(Domain& d) override {
+    hipStream_t s = d.stream;
+    dconn.upload(conn, s);
+    dmat.upload(mat_h, s);
+    std::vector<J2Param> tab;
+    for (auto& kv : d.j2defs) tab.push_back(kv.second);
+    dprm.upload(tab, s);
+    has_b = std::any_of(b_h.begin(), b_h.end(), [](double v) { return v != 0.0; });
+    if (has_b) db.upload(b_h, s);
+    if (!brick) dthk.upload(thk_h, s);
+    const long long n = np();
+    epn.resize(ncomp() * n);
+    ept.resize(ncomp() * n);
+    xin.resize(n);
+    xit.resize(n);
+    sig.resize(nsig() * n);
+    D.resize(ntan() * n);
+    revert_to_start(d);
+  }
+  void update(Domain& d) override {
+    if (brick)
+      launch_brick_update(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, d.U.p, d.ndf, d.dt, np(),
+                          epn.p, xin.p, ept.p, xit.p, sig.p, D.p, d.nfail.p);
+    else
+      launch_quad_update(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, d.U.p, d.ndf, d.dt, np(),
+                         epn.p, xin.p, ept.p, xit.p, sig.p, D.p, d.nfail.p);
+  }
+  void resid(Domain& d, double* Re) override {
+    if (brick) launch_brick_resid(d.stream, nele(), dconn.p, d.crd.p, has_b ? db.p : nullptr, np(), sig.p, Re);
+    else launch_quad_resid(d.stream, nele(), dconn.p, d.crd.p, dthk.p, has_b ? db.p : nullptr, np(), sig.p, Re);
+  }
+  void tangent(Domain& d, double* Ke) override {
+    if (brick) launch_brick_tangent(d.stream, nele(), dconn.p, d.crd.p, np(), D.p, Ke);
+    else launch_quad_tangent(d.stream, nele(), dconn.p, d.crd.p, dthk.p, np(), D.p, Ke);
+  }
+  void commit(Domain& d) override { launch_j2_commit(d.stream, np(), ncomp(), ept.p, xit.p, epn.p, xin.p); }
+  void revert(Domain&) override {}  // J2Plasticity::revertToLastCommit is a no-op (J2Plasticity.cpp:524)
+  void revert_to_start(Domain& d) override {
+    // J2Plasticity::zero + the constructor's plastic_integrator at zero strain:
+    // zero state and stress, elastic tangent (filled by an update at U = 0).
+    epn.zero(d.stream);
+    ept.zero(d.stream);
+    xin.zero(d.stream);
+    xit.zero(d.stream);
+    sig.zero(d.stream);
+    DVec<double> zeroU;
+    zeroU.resize((size_t)d.nnode * d.ndf);
+    zeroU.zero(d.stream);
+    if (brick)
+      launch_brick_update(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, zeroU.p, d.ndf, 0.0, np(),
+                          epn.p, xin.p, ept.p, xit.p, sig.p, D.p, d.nfail.p);
+    else
+      launch_quad_update(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, zeroU.p, d.ndf, 0.0, np(),
+                         epn.p, xin.p, ept.p, xit.p, sig.p, D.p, d.nfail.p);
+    XG_HIP(hipStreamSynchronize(d.stream));
+  }
+  int point_data(Domain& d, int what, std::vector<double>& out) override {
+    const long long n = np();
+    const int P = npts();
+    auto soa = [&](DVec<double>& a, int nc) {
+      std::vector<double> h = a.download(d.stream);
+      return std::make_pair(h, nc);
+    };
+    if (what == XG_OUT_STRESS) {
+      auto [h, nc] = soa(sig, nsig());
+      out.resize((size_t)n * nc);
+      for (long long p = 0; p < n; p++)
+        for (int c = 0; c < nc; c++) out[p * nc + c] = h[c * n + p];
+      return nc;
+    }
+    if (what == XG_OUT_TANGENT) {
+      auto [h, nt] = soa(D, ntan());
+      const int m = brick ? 6 : 3;
+      out.resize((size_t)n * m * m);
+      for (long long p = 0; p < n; p++)
+        for (int a = 0; a < m; a++)
+          for (int b = a; b < m; b++) {
+            const double v = h[(long long)tri_index(m, a, b) * n + p];
+            out[p * m * m + a * m + b] = v;
+            out[p * m * m + b * m + a] = v;
+          }
+      (void)nt;
+      return m * m;
+    }
+    if (what == XG_OUT_PLASTIC) {
+      std::vector<double> e = ept.download(d.stream), x = xit.download(d.stream);
+      out.assign((size_t)n * 7, 0.0);
+      for (long long p = 0; p < n; p++) {
+        if (brick)
+          for (int c = 0; c < 6; c++) out[p * 7 + c] = e[c * n + p];
+        else
+          for (int c = 0; c < 4; c++) out[p * 7 + c] = e[c * n + p];
+        out[p * 7 + 6] = x[p];
+      }
+      return 7;
+    }
+    (void)P;
+    throw Error(XG_ERR_ARG, "unsupported output for J2 elements");
+  }
+  size_t device_bytes() const override {
+    return (epn.n + ept.n + xin.n + xit.n + sig.n + D.n) * 8 + dconn.n * 4;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Domain
+// ---------------------------------------------------------------------------
+Domain::~Domain() {
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  groups.clear();
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Domain::lanes() {
+  if (spmv_lanes > 0) return spmv_lanes;
+  const double mean = num.neq ? (double)num.colind.size() / num.neq : 1.0;
+  if (mean > 48) return 16;
+  if (mean > 20) return 8;
+  return 4;
+}
+
+void Domain::setup() {
+  XG_HIP(hipSetDevice(device));
+  if (nodedefs.empty()) throw Error(XG_ERR_STATE, "no nodes defined");
+  // nodes in ascending tag order
+  nnode = (int)nodedefs.size();
+  node_tags.clear();
+  node_index.clear();
+  crd_h.assign((size_t)nnode * ndm, 0.0);
+  std::vector<char> fixed((size_t)nnode * ndf, 0);
+  std::vector<double> Pnode((size_t)nnode * ndf, 0.0);
+  int i = 0;
+  for (auto& kv : nodedefs) {
+    node_tags.push_back(kv.first);
+    node_index[kv.first] = i;
+    for (int j = 0; j < ndm; j++) crd_h[(size_t)i * ndm + j] = kv.second.crd[j];
+    for (int j = 0; j < ndf; j++) {
+      fixed[(size_t)i * ndf + j] = kv.second.fix[j] ? 1 : 0;
+      Pnode[(size_t)i * ndf + j] = kv.second.P[j];
+    }
+    i++;
+  }
+  // element groups in ascending tag order
+  groups.clear();
+  ele_order.clear();
+  std::map<int, int> group_of_type;
+  std::vector<std::vector<int>> elem_nodes;
+  elem_nodes.reserve(eledefs.size());
+  std::map<int, int> j2index;
+  {
+    int k = 0;
+    for (auto& kv : j2defs) j2index[kv.first] = k++;
+  }
+  for (auto& kv : eledefs) {
+    const EleDef& ed = kv.second;
+    int gi;
+    auto it = group_of_type.find(ed.type);
+    if (it == group_of_type.end()) {
+      std::unique_ptr<Group> g;
+      if (ed.type == 1 || ed.type == 2) {
+        auto jg = std::make_unique<J2Group>();
+        jg->brick = ed.type == 1;
+        jg->nen = jg->brick ? 8 : 4;
+        jg->nd = jg->nen * ndf;
+        g = std::move(jg);
+      } else {
+        g = make_frame_group(*this);
+      }
+      gi = (int)groups.size();
+      groups.push_back(std::move(g));
+      group_of_type[ed.type] = gi;
+    } else
+      gi = it->second;
+    Group& g = *groups[gi];
+    std::vector<int> idx;
+    for (int nt : ed.nodes) {
+      auto ni = node_index.find(nt);
+      if (ni == node_index.end()) throw Error(XG_ERR_ARG, "element " + std::to_string(kv.first) + ": node not found");
+      idx.push_back(ni->second);
+    }
+    ele_order.push_back({gi, g.nele()});
+    g.tags.push_back(kv.first);
+    g.conn.insert(g.conn.end(), idx.begin(), idx.end());
+    if (ed.type == 1 || ed.type == 2) {
+      J2Group& jg = static_cast<J2Group&>(g);
+      jg.mat_h.push_back(j2index.at(ed.mat));
+      if (jg.brick) jg.b_h.insert(jg.b_h.end(), {ed.b[0], ed.b[1], ed.b[2]});
+      else {
+        jg.b_h.insert(jg.b_h.end(), {ed.b[0], ed.b[1]});
+        jg.thk_h.push_back(ed.thk);
+      }
+    } else {
+      frame_group_add(*this, g, kv.first, ed);
+    }
+    elem_nodes.push_back(std::move(idx));
+  }
+  // numbering + CSR pattern
+  number_and_pattern(nnode, ndf, fixed, elem_nodes, numberer == XG_NUMBERER_RCM, num);
+  const int neq = num.neq;
+  const long long nnz = (long long)num.colind.size();
+  // element matrix / vector offsets
+  ke_total = re_total = 0;
+  for (auto& g : groups) {
+    g->ke_off = ke_total;
+    g->re_off = re_total;
+    ke_total += (long long)g->nele() * g->nd * g->nd;
+    re_total += (long long)g->nele() * g->nd;
+  }
+  if (ke_total > 2147483647LL) throw Error(XG_ERR_ARG, "element matrices exceed 2^31 entries");
+  // gather maps, contributions in ascending element-tag order
+  std::vector<int> cnt(nnz + 1, 0), rcnt(neq + 1, 0);
+  std::vector<int> id;
+  auto element_id = [&](size_t k, std::vector<int>& out) {
+    out.clear();
+    for (int n : elem_nodes[k])
+      for (int j = 0; j < ndf; j++) out.push_back(num.eq[(size_t)n * ndf + j]);
+  };
+  auto slot = [&](int row, int col) -> long long {
+    const int* b = num.colind.data() + num.rowptr[row];
+    const int* e = num.colind.data() + num.rowptr[row + 1];
+    const int* f = std::lower_bound(b, e, col);
+    return f - num.colind.data();
+  };
+  std::vector<int> aidx, ridx, fillp, rfill;
+  for (int pass = 0; pass < 2; pass++) {
+    for (size_t k = 0; k < elem_nodes.size(); k++) {
+      element_id(k, id);
+      const Group& g = *groups[ele_order[k].first];
+      const long long kbase = g.ke_off + (long long)ele_order[k].second * g.nd * g.nd;
+      const long long rbase = g.re_off + (long long)ele_order[k].second * g.nd;
+      const int nd = (int)id.size();
+      for (int a = 0; a < nd; a++) {
+        const int row = id[a];
+        if (row < 0) continue;
+        if (pass == 0) rcnt[row + 1]++;
+        else ridx[rfill[row]++] = (int)(rbase + a);
+        for (int b = 0; b < nd; b++) {
+          const int col = id[b];
+          if (col < 0) continue;
+          const long long sl = slot(row, col);
+          if (pass == 0) cnt[sl + 1]++;
+          else aidx[fillp[sl]++] = (int)(kbase + (long long)a * nd + b);
+        }
+      }
+    }
+    if (pass == 0) {
+      long long tot = 0;
+      for (long long s2 = 0; s2 < nnz; s2++) {
+        tot += cnt[s2 + 1];
+        if (tot > 2147483647LL) throw Error(XG_ERR_ARG, "assembly map exceeds 2^31 entries");
+        cnt[s2 + 1] = (int)tot;
+      }
+      for (int a = 0; a < neq; a++) rcnt[a + 1] += rcnt[a];
+      aidx.resize(cnt[nnz]);
+      ridx.resize(rcnt[neq]);
+      fillp.assign(cnt.begin(), cnt.end() - 1);
+      rfill.assign(rcnt.begin(), rcnt.end() - 1);
+    }
+  }
+  fillp.clear();
+  fillp.shrink_to_fit();
+  // reference loads per equation (DOF-group order is irrelevant per entry)
+  std::vector<double> Pe(neq, 0.0);
+  for (int n = 0; n < nnode; n++)
+    for (int j = 0; j < ndf; j++) {
+      const int a = num.eq[(size_t)n * ndf + j];
+      if (a >= 0) Pe[a] = Pnode[(size_t)n * ndf + j];
+    }
+  // nodal blocks for the block-Jacobi preconditioner
+  std::vector<int> bf(neq, 0), blkf;
+  std::vector<unsigned char> bs(neq, 1), blks;
+  for (int n = 0; n < nnode; n++) {
+    int first = -1, m = 0;
+    for (int j = 0; j < ndf; j++) {
+      const int a = num.eq[(size_t)n * ndf + j];
+      if (a < 0) continue;
+      if (first < 0) first = a;
+      m++;
+    }
+    if (m == 0) continue;
+    blkf.push_back(first);
+    blks.push_back((unsigned char)m);
+    for (int j = 0; j < m; j++) {
+      bf[first + j] = first;
+      bs[first + j] = (unsigned char)m;
+    }
+  }
+  nblk = (int)blkf.size();
+
+  // ---- device upload ----
+  if (!stream) XG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  if (!ev0) {
+    XG_HIP(hipEventCreate(&ev0));
+    XG_HIP(hipEventCreate(&ev1));
+  }
+  hipStream_t s = stream;
+  crd.upload(crd_h, s);
+  U.resize((size_t)nnode * ndf);
+  U.zero(s);
+  Uc.resize(U.n);
+  Uc.zero(s);
+  V.resize(U.n);
+  V.zero(s);
+  Vc.resize(U.n);
+  Vc.zero(s);
+  A.resize(U.n);
+  A.zero(s);
+  Ac.resize(U.n);
+  Ac.zero(s);
+  eqmap.upload(num.eq, s);
+  rowptr.upload(num.rowptr, s);
+  colind.upload(num.colind, s);
+  vals.resize(nnz);
+  vals.zero(s);
+  Ke.resize(ke_total);
+  Re.resize(re_total);
+  asm_ptr.upload(cnt, s);
+  asm_idx.upload(aidx, s);
+  rhs_ptr.upload(rcnt, s);
+  rhs_idx.upload(ridx, s);
+  B.resize(neq);
+  B.zero(s);
+  X.resize(neq);
+  X.zero(s);
+  Peq.upload(Pe, s);
+  nfail.resize(1);
+  nfail.zero(s);
+  r0.resize(neq);
+  r1.resize(neq);
+  z.resize(neq);
+  p.resize(neq);
+  q.resize(neq);
+  tmpv.resize(neq);
+  part.resize(2 * (size_t)std::max(ceil_div(std::max(neq, 1), 256), spmv_blocks(std::max(neq, 1), 64)) + 16);
+  pcbad.resize(1);
+  binv.resize((size_t)neq * ndf);
+  bfirst.upload(bf, s);
+  bsize.upload(bs, s);
+  blk_first.upload(blkf, s);
+  blk_size.upload(blks, s);
+  scal.resize(cg_scal_bytes());
+  dscalar.resize(4);
+  for (auto& g : groups) g->setup(*this);
+  XG_HIP(hipStreamSynchronize(s));
+  lambda = 0;
+  time = 0;
+  dc_ready = false;
+  ready = true;
+}
+
+void Domain::update() {
+  hipStream_t s = stream;
+  XG_HIP(hipEventRecord(ev0, s));
+  for (auto& g : groups) g->update(*this);
+  XG_HIP(hipEventRecord(ev1, s));
+  counters.updates++;
+  for (auto& g : groups) counters.material_points += g->fibers();
+}
+
+void Domain::form_unbalance() {
+  for (auto& g : groups) g->resid(*this, Re.p + g->re_off);
+  launch_assemble_rhs(stream, num.neq, rhs_ptr.p, rhs_idx.p, Re.p, lambda, Peq.p, B.p);
+}
+
+void Domain::form_tangent() {
+  for (auto& g : groups) g->tangent(*this, Ke.p + g->ke_off);
+  launch_assemble_csr(stream, (long long)num.colind.size(), asm_ptr.p, asm_idx.p, Ke.p, vals.p);
+}
+
+void Domain::precond_setup() {
+  const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
+  pcbad.zero(stream);
+  launch_block_inverse(stream, nblk, blk_first.p, blk_size.p, rowptr.p, colind.p, vals.p, bw, binv.p, pcbad.p);
+}
+
+int Domain::solve(const double* b, double* x) {
+  const int n = num.neq;
+  if (n == 0) return 0;
+  const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
+  precond_setup();
+  int bad = 0;
+  pcbad.download(&bad, 1, stream);
+  if (bad) return XG_BAD_LINEAR_SOLVE;
+  cg_start(stream, n, b, x, r0.p, z.p, p.p, bw, binv.p, bfirst.p, bsize.p, part.p, rtol * rtol, maxit, scal.p);
+  int conv = 0, it = 0, brk = 0;
+  double rr = 0, bb = 0;
+  int issued = 0;
+  const int L = lanes();
+  while (true) {
+    cg_read_scal(stream, scal.p, &conv, &it, &rr, &bb, &brk);
+    if (conv) break;
+    if (issued >= maxit) break;
+    const int chunk = std::min(cg_chunk, maxit - issued);
+    cg_iterations(stream, chunk, issued, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, q.p, bw,
+                  binv.p, bfirst.p, bsize.p, part.p, scal.p);
+    issued += chunk;
+  }
+  counters.solves++;
+  counters.cg_iters += it;
+  counters.last_cg_relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+  if (brk && bb > 0 && rr > rtol * rtol * bb) return XG_BAD_LINEAR_SOLVE;
+  if (bb > 0 && !(rr <= rtol * rtol * bb)) return XG_BAD_LINEAR_SOLVE;
+  return 0;
+}
+
+void Domain::incr_disp(const double* dUdev, double scale) {
+  launch_incr_disp(stream, (long long)nnode * ndf, eqmap.p, dUdev, scale, U.p);
+}
+
+void Domain::commit() {
+  for (auto& g : groups) g->commit(*this);
+  Uc.copy_from(U, stream);
+  Vc.copy_from(V, stream);
+  Ac.copy_from(A, stream);
+  XG_HIP(hipStreamSynchronize(stream));
+}
+
+void Domain::revert() {
+  U.copy_from(Uc, stream);
+  V.copy_from(Vc, stream);
+  A.copy_from(Ac, stream);
+  for (auto& g : groups) g->revert(*this);
+  XG_HIP(hipStreamSynchronize(stream));
+}
+
+void Domain::revert_to_start() {
+  U.zero(stream);
+  Uc.zero(stream);
+  V.zero(stream);
+  Vc.zero(stream);
+  A.zero(stream);
+  Ac.zero(stream);
+  for (auto& g : groups) g->revert_to_start(*this);
+  lambda = 0;
+  time = 0;
+  XG_HIP(hipStreamSynchronize(stream));
+}
+
+double Domain::dot_dev(const double* a, const double* b, int n) {
+  launch_dot(stream, n, a, b, part.p, dscalar.p);
+  double h = 0;
+  dscalar.download(&h, 1, stream);
+  return h;
+}
+double Domain::norm2_dev(const double* v, int n) { return std::sqrt(dot_dev(v, v, n)); }
+double Domain::read_eq(const double* v, int a) {
+  double h = 0;
+  XG_HIP(hipMemcpyAsync(&h, v + a, sizeof(double), hipMemcpyDeviceToHost, stream));
+  XG_HIP(hipStreamSynchronize(stream));
+  return h;
+}
+
+// ---- convergence tests ----------------------------------------------------
+int Domain::run_test(int& iter, std::vector<double>& norms, int& numIncr) {
+  double norm;
+  const int n = num.neq;
+  if (test_kind == XG_TEST_NORM_UNBALANCE) norm = norm2_dev(B.p, n);  // CTestNormUnbalance.cpp:81
+  else if (test_kind == XG_TEST_NORM_DISP_INCR) norm = norm2_dev(X.p, n);  // CTestNormDispIncr.cpp:78
+  else norm = 0.5 * std::fabs(dot_dev(X.p, B.p, n));  // CTestEnergyIncr.cpp:85
+  counters.last_norm = norm;
+  if (iter > 1 && !norms.empty() && norms.back() < norm) numIncr++;
+  norms.push_back(norm);
+  if (norm <= test_tol) return iter;
+  if (iter >= test_maxit || numIncr >= test_maxit) {
+    iter++;
+    return -2;
+  }
+  iter++;
+  return -1;
+}
+
+// ---- integrators --------------------------------------------------------------
+int Domain::new_step() {
+  const int n = num.neq;
+  if (integ == 1) {  // LoadControl::newStep, LoadControl.cpp:70-95
+    lc_dlambda *= lc_spec / lc_last;
+    if (lc_dlambda < lc_min) lc_dlambda = lc_min;
+    else if (lc_dlambda > lc_max) lc_dlambda = lc_max;
+    lambda += lc_dlambda;
+    lc_last = 0;
+    return 0;
+  }
+  // DisplacementControl::newStep, DisplacementControl.cpp:121-222
+  dc_incr *= dc_spec / dc_last;
+  if (dc_incr < dc_min) dc_incr = dc_min;
+  else if (dc_incr > dc_max) dc_incr = dc_max;
+  form_tangent();
+  if (solve(phat.p, dUhat.p) < 0) return XG_BAD_STEP;
+  const double dUahat = read_eq(dUhat.p, dc_eq);
+  if (dUahat == 0.0) return XG_BAD_STEP;
+  const double dl = dc_incr / dUahat;
+  lambda += dl;
+  XG_HIP(hipMemsetAsync(dU.p, 0, n * sizeof(double), stream));
+  launch_axpby(stream, n, dl, dUhat.p, 0.0, dU.p);
+  incr_disp(dU.p);
+  update();
+  dc_last = 0;
+  return 0;
+}
+
+int Domain::integ_update(const double* x) {
+  const int n = num.neq;
+  if (integ == 1) {  // LoadControl::update, LoadControl.cpp:102-120
+    incr_disp(x);
+    update();
+    lc_last++;
+    return 0;
+  }
+  // DisplacementControl::update, DisplacementControl.cpp:224-283
+  XG_HIP(hipMemcpyAsync(dUbar.p, x, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  const double dUabar = read_eq(dUbar.p, dc_eq);
+  if (solve(phat.p, dUhat.p) < 0) return XG_BAD_STEP;
+  const double dUahat = read_eq(dUhat.p, dc_eq);
+  if (dUahat == 0.0) return XG_BAD_STEP;
+  const double dL = -dUabar / dUahat;
+  XG_HIP(hipMemcpyAsync(dU.p, dUbar.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  launch_axpby(stream, n, dL, dUhat.p, 1.0, dU.p);
+  lambda += dL;
+  incr_disp(dU.p);
+  update();
+  XG_HIP(hipMemcpyAsync(X.p, dU.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  dc_last++;
+  return 0;
+}
+
+// NewtonRaphson::solveCurrentStep, NewtonRaphson.cpp:125-221
+int Domain::newton() {
+  form_unbalance();
+  int iter = 1, numIncr = 0, result = -1;
+  std::vector<double> norms;
+  do {
+    form_tangent();
+    if (solve(B.p, X.p) < 0) return XG_BAD_LINEAR_SOLVE;
+    if (integ_update(X.p) < 0) return XG_BAD_STEP;
+    form_unbalance();
+    result = run_test(iter, norms, numIncr);
+    counters.newton_iters++;
+  } while (result == -1);
+  if (result == -2) return XG_TEST_FAILED;
+  return 0;
+}
+
+int Domain::analyze(int nsteps, double dtv) {
+  if (!ready) setup();
+  if (integ == 3) throw Error(XG_ERR_STATE, "transient analysis runs through the frame ensemble entry points");
+  if (integ == 2 && !dc_ready) {
+    // DisplacementControl::domainChanged, :286-415: phat = B(lambda+1) at the
+    // current (zero-unbalance) state
+    const int n = num.neq;
+    phat.resize(n);
+    dUhat.resize(n);
+    dUbar.resize(n);
+    dU.resize(n);
+    lambda += 1.0;
+    form_unbalance();
+    XG_HIP(hipMemcpyAsync(phat.p, B.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    lambda -= 1.0;
+    auto it = node_index.find(dc_node);
+    if (it == node_index.end()) throw Error(XG_ERR_ARG, "DisplacementControl: node not found");
+    dc_eq = num.eq[(size_t)it->second * ndf + dc_dof];
+    if (dc_eq < 0) throw Error(XG_ERR_ARG, "DisplacementControl: dof is fixed");
+    if (norm2_dev(phat.p, n) == 0.0) throw Error(XG_ERR_ARG, "DisplacementControl: zero reference load");
+    dc_ready = true;
+  }
+  (void)dtv;
+  for (int st = 0; st < nsteps; st++) {
+    const double lam0 = lambda;
+    int r = new_step();
+    if (r == 0) r = newton();
+    if (r < 0) {
+      revert();
+      lambda = lam0;
+      return r;
+    }
+    commit();
+  }
+  return 0;
+}
+
+}  // namespace xg

@@ -1,0 +1,147 @@
+// One finite-element model resident on one GPU (the xg_domain handle).
+#pragma once
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/xara_gpu.h"
+#include "xg_common.h"
+#include "xg_memory.h"
+#include "xg_numbering.h"
+
+namespace xg {
+
+struct Domain;
+
+// A group of elements of one formulation, stored in ascending tag order.
+struct Group {
+  int nen = 0;                 // nodes per element
+  int nd = 0;                  // element DOFs
+  std::vector<int> tags;       // ascending
+  std::vector<int> conn;       // node indices, nele*nen
+  long long ke_off = 0, re_off = 0;
+  virtual ~Group() {}
+  int nele() const { return (int)tags.size(); }
+  virtual int npts() const = 0;  // material points (or sections) per element
+  virtual long long fibers() const { return (long long)nele() * npts(); }
+  virtual void setup(Domain& d) = 0;
+  virtual void update(Domain& d) = 0;
+  virtual void resid(Domain& d, double* Re) = 0;
+  virtual void tangent(Domain& d, double* Ke) = 0;
+  virtual void mass_diag(Domain&, std::vector<double>&) {}  // lumped element mass per element DOF
+  virtual void commit(Domain& d) = 0;
+  virtual void revert(Domain& d) = 0;
+  virtual void revert_to_start(Domain& d) = 0;
+  // per-point outputs for element i (local index) appended to out; returns width
+  virtual int point_data(Domain& d, int what, std::vector<double>& out) = 0;
+  virtual size_t device_bytes() const { return 0; }
+};
+
+struct NodeDef {
+  double crd[3] = {0, 0, 0};
+  int fix[6] = {0, 0, 0, 0, 0, 0};
+  double mass[6] = {0, 0, 0, 0, 0, 0};
+  double P[6] = {0, 0, 0, 0, 0, 0};
+};
+
+struct EleDef {
+  int type = 0;  // 1 brick, 2 quad, 3 dispbeam
+  std::vector<int> nodes;  // node tags
+  int mat = 0, sec = 0, transf = 0, nip = 0;
+  double thk = 1.0, rho = 0.0, b[3] = {0, 0, 0};
+};
+
+struct Steel02Def {
+  double Fy, E0, b, R0, cR1, cR2, a1, a2, a3, a4, sigini;
+};
+struct Concrete02Def {
+  double fc, epsc0, fcu, epscu, rat, ft, Ets;
+};
+struct FiberSecDef {
+  std::vector<int> kind;  // 1 steel02, 2 concrete02, 3 elastic
+  std::vector<int> mat;   // material tag
+  std::vector<double> y, z, A;
+  double GJ = -1.0;
+  bool centroid = true;
+};
+
+struct Domain {
+  int ndm = 0, ndf = 0, device = 0;
+  hipStream_t stream = nullptr;
+  // ---- definition ----
+  std::map<int, NodeDef> nodedefs;
+  std::map<int, J2Param> j2defs;
+  std::map<int, Steel02Def> steeldefs;
+  std::map<int, Concrete02Def> concdefs;
+  std::map<int, double> elasticdefs;
+  std::map<int, FiberSecDef> secdefs;
+  std::map<int, std::vector<double>> transfdefs;
+  std::map<int, EleDef> eledefs;
+  // ---- after setup ----
+  bool ready = false;
+  int nnode = 0;
+  std::vector<int> node_tags;
+  std::map<int, int> node_index;
+  std::vector<double> crd_h;  // nnode*ndm
+  Numbering num;
+  std::vector<std::unique_ptr<Group>> groups;
+  std::vector<std::pair<int, int>> ele_order;  // ascending tag -> (group, local)
+  long long ke_total = 0, re_total = 0;
+  // device model
+  DVec<double> crd, U, Uc, V, Vc, A, Ac;
+  DVec<int> eqmap;
+  DVec<int> rowptr, colind;
+  DVec<double> vals, Ke, Re, B, X, Peq;
+  DVec<int> asm_ptr, asm_idx, rhs_ptr, rhs_idx;
+  DVec<int> nfail, pcbad;
+  // solver workspace
+  DVec<double> r0, r1, z, p, q, part, binv, tmpv;
+  DVec<int> bfirst, blk_first;
+  DVec<unsigned char> bsize, blk_size;
+  DVec<char> scal;
+  DVec<double> dscalar;
+  int nblk = 0;
+  // analysis configuration
+  int system = XG_SYSTEM_PCG_JACOBI, numberer = XG_NUMBERER_RCM;
+  double rtol = 1e-10;
+  int maxit = 20000;
+  int spmv_lanes = 0;  // 0 = choose from the mean row length
+  int cg_chunk = 16;
+  int test_kind = XG_TEST_NORM_UNBALANCE;
+  double test_tol = 1e-6;
+  int test_maxit = 25;
+  int integ = 1;
+  double lc_dlambda = 1, lc_min = 1, lc_max = 1, lc_spec = 1, lc_last = 1;
+  int dc_node = 0, dc_dof = 0, dc_eq = -1;
+  double dc_incr = 0, dc_min = 0, dc_max = 0, dc_spec = 1, dc_last = 1;
+  bool dc_ready = false;
+  DVec<double> phat, dUhat, dUbar, dU;
+  double lambda = 0, time = 0, dt = 0;
+  xg_counters counters{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  ~Domain();
+  // operations
+  void setup();
+  void update();
+  void form_unbalance();
+  void form_tangent();
+  int solve(const double* b, double* x);  // device pointers, returns <0 on failure
+  void incr_disp(const double* dU_dev, double scale = 1.0);
+  void commit();
+  void revert();
+  void revert_to_start();
+  double norm2_dev(const double* v, int n);
+  double dot_dev(const double* a, const double* b, int n);
+  double read_eq(const double* v, int a);
+  int analyze(int nsteps, double dtv);
+  int newton();
+  int new_step();
+  int integ_update(const double* x);
+  int run_test(int& iter, std::vector<double>& norms, int& numIncr);
+  void precond_setup();
+  int lanes();
+};
+
+}  // namespace xg

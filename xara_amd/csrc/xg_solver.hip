@@ -1,0 +1,462 @@
+// Assembly into the fixed-pattern CSR system and the preconditioned CG
+// LinearSOE on gfx950.
+//
+// Assembly is a gather: every CSR slot (and every right-hand-side entry)
+// owns the list of element-matrix entries that land on it, stored in
+// element-tag order, so each value is summed in the order of
+// IncrementalIntegrator::formTangent / formElementResidual
+// (IncrementalIntegrator.cpp:74-98, :221-236) with no floating-point atomics.
+//
+// Dot products reduce per block with a fixed LDS tree and then over blocks
+// in a single-block pass, so every PCG iteration is bitwise reproducible.
+#include "xg_common.h"
+#include "xg_kernels.h"
+
+namespace xg {
+
+// ---------------------------------------------------------------------------
+// Assembly
+// ---------------------------------------------------------------------------
+__global__ void k_assemble_csr(long long nnz, const int* __restrict__ ptr, const int* __restrict__ idx,
+                               const double* __restrict__ Ke, double* __restrict__ vals) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  double acc = 0.0;
+  const int b = ptr[k], e = ptr[k + 1];
+  for (int j = b; j < e; j++) acc += Ke[idx[j]];
+  vals[k] = acc;
+}
+
+// B[a] = sum over elements (in tag order) of -R_e + lambda * P[a]
+__global__ void k_assemble_rhs(int neq, const int* __restrict__ ptr, const int* __restrict__ idx,
+                               const double* __restrict__ Re, double lambda,
+                               const double* __restrict__ P, double* __restrict__ B) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= neq) return;
+  double acc = 0.0;
+  for (int j = ptr[a]; j < ptr[a + 1]; j++) acc += -Re[idx[j]];
+  if (P) acc += lambda * P[a];
+  B[a] = acc;
+}
+
+// Diagonal add (nodal masses etc.): vals[pos[i]] += coef * m[i]
+__global__ void k_add_diag(int n, const long long* __restrict__ pos, const double* __restrict__ m,
+                           double coef, double* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vals[pos[i]] += coef * m[i];
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions (deterministic tree)
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV y = A x with L lanes per row; per-block partial of x.y
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ __launch_bounds__(256) void k_spmv_dot(int n, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ colind,
+                                                  const double* __restrict__ vals,
+                                                  const double* __restrict__ x, double* __restrict__ y,
+                                                  double* __restrict__ partial,
+                                                  const int* __restrict__ stop) {
+  __shared__ double sh[256];
+  if (stop && stop[0]) return;
+  const int t = threadIdx.x;
+  const int row = blockIdx.x * (256 / L) + t / L;
+  const int lane = t % L;
+  double sum = 0.0;
+  if (row < n) {
+    const int b = rowptr[row], e = rowptr[row + 1];
+    for (int k = b + lane; k < e; k += L) sum += vals[k] * x[colind[k]];
+  }
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
+  double d = 0.0;
+  if (row < n && lane == 0) {
+    y[row] = sum;
+    d = sum * x[row];
+  }
+  if (partial) {
+    const double bs = block_sum<256>(d, sh);
+    if (t == 0) partial[blockIdx.x] = bs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PCG scalar state kept on the device
+// ---------------------------------------------------------------------------
+struct CGScal {
+  double rz, pq, rr, bb, alpha, beta, tol2;
+  int converged, iter, maxit, breakdown;
+};
+
+// Single-block reduction of `nb` partials (column c of a [nb][ncol] array).
+__global__ __launch_bounds__(256) void k_cg_alpha(int nb, const double* __restrict__ part,
+                                                  CGScal* __restrict__ s) {
+  __shared__ double sh[256];
+  if (s->converged) return;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) v += part[i];
+  const double pq = block_sum<256>(v, sh);
+  if (threadIdx.x == 0) {
+    s->pq = pq;
+    if (pq == 0.0 || !isfinite(pq)) {
+      s->breakdown = 1;
+      s->converged = 1;
+    } else {
+      s->alpha = s->rz / pq;
+    }
+  }
+}
+
+// x += alpha p; r_out = r_in - alpha q; z = M^{-1} r_out; partials of r.z
+// and r.r.  M^{-1}: block rows binv[i*bw + j] over the nodal block starting
+// at bfirst[i] with bsize[i] entries (bw = 1 gives Jacobi).  r is double
+// buffered, so a row's block neighbours are recomputed from r_in with the
+// same arithmetic instead of being read while another thread block writes.
+__global__ __launch_bounds__(256) void k_cg_update(int n, const CGScal* __restrict__ s,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ q, double* __restrict__ x,
+                                                   const double* __restrict__ r_in,
+                                                   double* __restrict__ r_out, double* __restrict__ z,
+                                                   int bw, const double* __restrict__ binv,
+                                                   const int* __restrict__ bfirst,
+                                                   const unsigned char* __restrict__ bsize,
+                                                   double* __restrict__ part) {
+  __shared__ double sh[256];
+  if (s->converged) return;
+  const double alpha = s->alpha;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (i < n) {
+    x[i] += alpha * p[i];
+    const double ri = r_in[i] - alpha * q[i];
+    r_out[i] = ri;
+    double zi;
+    if (bw == 1) {
+      zi = binv[i] * ri;
+    } else {
+      const int f = bfirst[i], m = bsize[i];
+      zi = 0.0;
+      for (int j = 0; j < m; j++) {
+        const int c = f + j;
+        zi += binv[(long long)i * bw + j] * (r_in[c] - alpha * q[c]);
+      }
+    }
+    z[i] = zi;
+    rz = ri * zi;
+    rr = ri * ri;
+  }
+  const double s1 = block_sum<256>(rz, sh);
+  const double s2 = block_sum<256>(rr, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cg_beta(int nb, const double* __restrict__ part,
+                                                 CGScal* __restrict__ s) {
+  __shared__ double sh[256];
+  if (s->converged) return;
+  double v1 = 0.0, v2 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    v1 += part[2 * i];
+    v2 += part[2 * i + 1];
+  }
+  const double rz = block_sum<256>(v1, sh);
+  const double rr = block_sum<256>(v2, sh);
+  if (threadIdx.x == 0) {
+    s->iter += 1;
+    s->rr = rr;
+    s->beta = rz / s->rz;
+    s->rz = rz;
+    if (rr <= s->tol2 * s->bb || s->iter >= s->maxit) s->converged = 1;
+  }
+}
+
+__global__ void k_cg_newp(int n, const CGScal* __restrict__ s, const double* __restrict__ z,
+                          double* __restrict__ p) {
+  if (s->converged) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  p[i] = z[i] + s->beta * p[i];
+}
+
+// Start: x = 0, r = b, z = M^{-1} b, p = z; partials of r.z, b.b
+__global__ __launch_bounds__(256) void k_cg_start(int n, const double* __restrict__ b,
+                                                  double* __restrict__ x, double* __restrict__ r,
+                                                  double* __restrict__ z, double* __restrict__ p,
+                                                  int bw, const double* __restrict__ binv,
+                                                  const int* __restrict__ bfirst,
+                                                  const unsigned char* __restrict__ bsize,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double rz = 0.0, bb = 0.0;
+  if (i < n) {
+    x[i] = 0.0;
+    r[i] = b[i];
+    double zi;
+    if (bw == 1) zi = binv[i] * b[i];
+    else {
+      zi = 0.0;
+      const int f = bfirst[i], m = bsize[i];
+      for (int j = 0; j < m; j++) zi += binv[(long long)i * bw + j] * b[f + j];
+    }
+    z[i] = zi;
+    p[i] = zi;
+    rz = b[i] * zi;
+    bb = b[i] * b[i];
+  }
+  const double s1 = block_sum<256>(rz, sh);
+  const double s2 = block_sum<256>(bb, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cg_init_scal(int nb, const double* __restrict__ part,
+                                                      double tol2, int maxit, CGScal* __restrict__ s) {
+  __shared__ double sh[256];
+  double v1 = 0.0, v2 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    v1 += part[2 * i];
+    v2 += part[2 * i + 1];
+  }
+  const double rz = block_sum<256>(v1, sh);
+  const double bb = block_sum<256>(v2, sh);
+  if (threadIdx.x == 0) {
+    s->rz = rz;
+    s->bb = bb;
+    s->rr = bb;
+    s->tol2 = tol2;
+    s->maxit = maxit;
+    s->iter = 0;
+    s->breakdown = 0;
+    s->alpha = s->beta = s->pq = 0.0;
+    s->converged = (bb == 0.0) ? 1 : 0;
+  }
+}
+
+// Jacobi / nodal block-Jacobi inverse from the assembled CSR matrix.
+// One thread per node block: gathers the m x m diagonal block (m <= 6),
+// inverts it by Gauss-Jordan with partial pivoting and writes its rows.
+__global__ void k_block_inverse(int nblk, const int* __restrict__ blk_first,
+                                const unsigned char* __restrict__ blk_size,
+                                const int* __restrict__ rowptr, const int* __restrict__ colind,
+                                const double* __restrict__ vals, int bw, double* __restrict__ binv,
+                                int* __restrict__ bad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nblk) return;
+  const int f = blk_first[k], m = blk_size[k];
+  if (bw == 1) {
+    // Jacobi: each equation its own block
+    for (int j = 0; j < m; j++) {
+      const int row = f + j;
+      double d = 0.0;
+      for (int q = rowptr[row]; q < rowptr[row + 1]; q++)
+        if (colind[q] == row) d = vals[q];
+      if (d == 0.0) bad[0] = 1;
+      binv[row] = (d != 0.0) ? 1.0 / d : 0.0;
+    }
+    return;
+  }
+  double a[6][12];
+  for (int i = 0; i < m; i++) {
+    for (int j = 0; j < m; j++) a[i][j] = 0.0;
+    for (int j = 0; j < m; j++) a[i][m + j] = (i == j) ? 1.0 : 0.0;
+    const int row = f + i;
+    for (int q = rowptr[row]; q < rowptr[row + 1]; q++) {
+      const int c = colind[q] - f;
+      if (c >= 0 && c < m) a[i][c] = vals[q];
+    }
+  }
+  for (int c = 0; c < m; c++) {
+    int piv = c;
+    for (int i = c + 1; i < m; i++)
+      if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
+    if (a[piv][c] == 0.0) {
+      bad[0] = 1;
+      return;
+    }
+    if (piv != c)
+      for (int j = 0; j < 2 * m; j++) {
+        double tmp = a[c][j];
+        a[c][j] = a[piv][j];
+        a[piv][j] = tmp;
+      }
+    const double inv = 1.0 / a[c][c];
+    for (int j = 0; j < 2 * m; j++) a[c][j] *= inv;
+    for (int i = 0; i < m; i++)
+      if (i != c) {
+        const double fct = a[i][c];
+        for (int j = 0; j < 2 * m; j++) a[i][j] -= fct * a[c][j];
+      }
+  }
+  for (int i = 0; i < m; i++)
+    for (int j = 0; j < bw; j++) binv[(long long)(f + i) * bw + j] = (j < m) ? a[i][m + j] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Small vector kernels
+// ---------------------------------------------------------------------------
+// U[node dof] += dU[eq] over the DOF map (AnalysisModel::incrDisp)
+__global__ void k_incr_disp(long long ndofs, const int* __restrict__ eqmap, const double* __restrict__ dU,
+                            double scale, double* __restrict__ U) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ndofs) return;
+  const int a = eqmap[i];
+  if (a >= 0) U[i] += scale * dU[a];
+}
+
+// y = a*x + b*y
+__global__ void k_axpby(int n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = a * x[i] + b * y[i];
+}
+
+// plain copy (HBM traffic calibration of the profiler counters)
+__global__ void k_copy(long long n, const double* __restrict__ x, double* __restrict__ y) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+
+// partial sums of x.y per block
+__global__ __launch_bounds__(256) void k_dot_partial(int n, const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  double v = (i < n) ? x[i] * y[i] : 0.0;
+  const double s = block_sum<256>(v, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_sum_partials(int nb, const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[256];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) v += part[i];
+  const double s = block_sum<256>(v, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+void launch_assemble_csr(hipStream_t s, long long nnz, const int* ptr, const int* idx, const double* Ke,
+                         double* vals) {
+  if (nnz <= 0) return;
+  hipLaunchKernelGGL(k_assemble_csr, dim3(ceil_div(nnz, 256)), dim3(256), 0, s, nnz, ptr, idx, Ke, vals);
+}
+void launch_assemble_rhs(hipStream_t s, int neq, const int* ptr, const int* idx, const double* Re,
+                         double lambda, const double* P, double* B) {
+  if (neq <= 0) return;
+  hipLaunchKernelGGL(k_assemble_rhs, dim3(ceil_div(neq, 256)), dim3(256), 0, s, neq, ptr, idx, Re, lambda,
+                     P, B);
+}
+void launch_add_diag(hipStream_t s, int n, const long long* pos, const double* m, double coef,
+                     double* vals) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_add_diag, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, pos, m, coef, vals);
+}
+int spmv_blocks(int n, int lanes) { return ceil_div(n, 256 / lanes); }
+void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
+                 const double* x, double* y, double* partial, const int* stop) {
+  const dim3 g(spmv_blocks(n, lanes)), b(256);
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL(k_spmv_dot<4>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
+    case 8: hipLaunchKernelGGL(k_spmv_dot<8>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
+    case 16: hipLaunchKernelGGL(k_spmv_dot<16>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
+    case 32: hipLaunchKernelGGL(k_spmv_dot<32>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
+    default: hipLaunchKernelGGL(k_spmv_dot<64>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
+  }
+}
+void launch_block_inverse(hipStream_t s, int nblk, const int* first, const unsigned char* size,
+                          const int* rowptr, const int* colind, const double* vals, int bw, double* binv,
+                          int* bad) {
+  if (nblk <= 0) return;
+  hipLaunchKernelGGL(k_block_inverse, dim3(ceil_div(nblk, 128)), dim3(128), 0, s, nblk, first, size, rowptr,
+                     colind, vals, bw, binv, bad);
+}
+void launch_incr_disp(hipStream_t s, long long ndofs, const int* eqmap, const double* dU, double scale,
+                      double* U) {
+  if (ndofs <= 0) return;
+  hipLaunchKernelGGL(k_incr_disp, dim3(ceil_div(ndofs, 256)), dim3(256), 0, s, ndofs, eqmap, dU, scale, U);
+}
+void launch_copy(hipStream_t s, long long n, const double* x, double* y) {
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, n, x, y);
+}
+void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, a, x, b, y);
+}
+void launch_dot(hipStream_t s, int n, const double* x, const double* y, double* part, double* out) {
+  const int nb = ceil_div(n > 0 ? n : 1, 256);
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(256), 0, s, n, x, y, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, part, out);
+}
+
+// PCG driver pieces ----------------------------------------------------------
+size_t cg_scal_bytes() { return sizeof(CGScal); }
+
+void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, double* z, double* p, int bw,
+              const double* binv, const int* bfirst, const unsigned char* bsize, double* part, double tol2,
+              int maxit, void* scal) {
+  const int nb = ceil_div(n, 256);
+  hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(256), 0, s, n, b, x, r, z, p, bw, binv, bfirst, bsize, part);
+  hipLaunchKernelGGL(k_cg_init_scal, dim3(1), dim3(256), 0, s, nb, part, tol2, maxit, (CGScal*)scal);
+}
+
+void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
+                   const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
+                   double* p, double* q, int bw, const double* binv, const int* bfirst,
+                   const unsigned char* bsize, double* part, void* scal) {
+  CGScal* S = (CGScal*)scal;
+  const int nbs = spmv_blocks(n, lanes);
+  const int nbv = ceil_div(n, 256);
+  const int* stop = &S->converged;
+  for (int it = 0; it < iters; it++) {
+    launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, nbs, part, S);
+    const bool even = ((it0 + it) & 1) == 0;
+    hipLaunchKernelGGL(k_cg_update, dim3(nbv), dim3(256), 0, s, n, S, p, q, x, even ? r0 : r1,
+                       even ? r1 : r0, z, bw, binv, bfirst, bsize, part);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, nbv, part, S);
+    hipLaunchKernelGGL(k_cg_newp, dim3(nbv), dim3(256), 0, s, n, S, z, p);
+  }
+}
+
+void cg_read_scal(hipStream_t s, const void* scal, int* converged, int* iter, double* rr, double* bb,
+                  int* breakdown) {
+  CGScal h;
+  XG_HIP(hipMemcpyAsync(&h, scal, sizeof(CGScal), hipMemcpyDeviceToHost, s));
+  XG_HIP(hipStreamSynchronize(s));
+  *converged = h.converged;
+  *iter = h.iter;
+  *rr = h.rr;
+  *bb = h.bb;
+  *breakdown = h.breakdown;
+}
+
+}  // namespace xg
