@@ -74,10 +74,18 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=24, help="cube edge of the CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run the 256 MiB copy kernel (profiler counter calibration)")
+    ap.add_argument("--mode", choices=["update", "newton"], default="update")
+    ap.add_argument("--newton-steps", type=int, default=40)
+    ap.add_argument("--system", default="GpuBlockPCG")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+
+    if args.mode == "newton":
+        return newton_main(args)
 
     from xara_amd import meshes
     from xara_amd.model import Model
@@ -104,6 +112,14 @@ def main():
     value = npts * args.steps / wall
     bytes_per = update_bytes(nnode, nele, npts)
     achieved = bytes_per / (ev_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on
+    # this workload, scaled by the copy-kernel calibration (profiles/)
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "brick_update_64cube.json")
+    if n == 64 and os.path.exists(prof):
+        traffic = json.load(open(prof))["traffic_bytes"]
+    if args.copy_calib:
+        m.time_op("copy", 3)
     plastic = float(np.mean(m.point_data("plastic", npts)[:, 6] > 0))
     print(f"bench: {ms_per_step:.3f} ms/step (event {ev_ms:.3f} ms), plastic fraction {plastic:.2f}",
           file=sys.stderr, flush=True)
@@ -124,7 +140,7 @@ def main():
                    "elements": nele, "nodes": nnode, "points": npts,
                    "plastic_fraction": round(plastic, 3), "setup_s": round(setup_s, 2)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_launch": bytes_per, "event_ms": ev_ms},
     }
     if not args.no_cpu:
@@ -134,6 +150,67 @@ def main():
         out["gpu_over_cpu"] = value / cb["value"]
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+def newton_main(args):
+    """Newton iterations/s of the full pushover: cube under uniform top
+    pressure ramped to 1.5x the first-yield pressure in `newton_steps`
+    LoadControl steps, NewtonRaphson with NormUnbalance, GPU PCG."""
+    from xara_amd import meshes
+    from xara_amd.model import Model
+
+    n = args.n
+    mesh = meshes.brick_cube(n)
+    m = Model(3, 3, device=0)
+    meshes.build_brick_cube(m, mesh, pressure=1.0)
+    m.system(args.system, 1e-10, 50000)
+    t0 = time.time()
+    m.setup()
+    setup_s = time.time() - t0
+    npts = 8 * len(mesh["ele_tags"])
+    # first-yield pressure from one elastic solve at unit pressure
+    m.set_load_factor(1.0)
+    B = m.form_unbalance()
+    m.form_tangent(values=False)
+    u = m.solve(B)
+    m.incr_disp(u)
+    m.update()
+    sig = m.point_data("stress", npts)
+    s0, s1, s2, s3, s4, s5 = sig.T
+    vm = np.sqrt(0.5 * ((s0 - s1) ** 2 + (s1 - s2) ** 2 + (s2 - s0) ** 2) + 3 * (s3 ** 2 + s4 ** 2 + s5 ** 2))
+    p_y = 250.0 / vm.max()
+    m.revert_to_start()
+    m.set_load_factor(0.0)
+    m.test("NormUnbalance", 1e-6 * p_y * (1.0 / n) ** 2 * n, 25)
+    m.integrator("LoadControl", 1.5 * p_y / args.newton_steps)
+    print(f"bench: first-yield pressure {p_y:.4f}; running {args.newton_steps} steps", file=sys.stderr,
+          flush=True)
+    c0 = m.counters()
+    t0 = time.perf_counter()
+    for st in range(args.newton_steps):
+        rc = m.analyze(1)
+        c = m.counters()
+        print(f"bench: step {st + 1} rc={rc} newton={c['newton_iters'] - c0['newton_iters']} "
+              f"cg={c['cg_iters'] - c0['cg_iters']} t={time.perf_counter() - t0:.1f}s", file=sys.stderr,
+              flush=True)
+        if rc != 0:
+            break
+    wall = time.perf_counter() - t0
+    c = m.counters()
+    iters = c["newton_iters"] - c0["newton_iters"]
+    plastic = float(np.mean(m.point_data("plastic", npts)[:, 6] > 0))
+    out = {
+        "metric": "Newton iterations/s", "value": iters / wall, "unit": "iterations/s", "n_gpus": 1,
+        "steps": args.newton_steps, "warmup": 0, "ms_per_step": 1e3 * wall / max(1, args.newton_steps),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"stdBrick J2 cube {n}^3 pushover to 1.5 p_y, LoadControl, "
+                               f"{args.system} rtol 1e-10", "newton_iters": iters,
+                   "cg_iters": c["cg_iters"] - c0["cg_iters"], "solves": c["solves"] - c0["solves"],
+                   "plastic_fraction": round(plastic, 3), "setup_s": round(setup_s, 2),
+                   "p_y": p_y, "final_load_factor": m.load_factor},
+    }
     print(json.dumps(out))
 
 
