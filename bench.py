@@ -76,7 +76,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run the 256 MiB copy kernel (profiler counter calibration)")
-    ap.add_argument("--mode", choices=["update", "newton"], default="update")
+    ap.add_argument("--mode", choices=["update", "newton", "frame"], default="update")
+    ap.add_argument("--frame", default="10,10,20", help="bays x, bays y, stories of the frame mode")
+    ap.add_argument("--frame-steps", type=int, default=0, help="DisplacementControl steps in frame mode")
     ap.add_argument("--newton-steps", type=int, default=40)
     ap.add_argument("--system", default="GpuBlockPCG")
     args = ap.parse_args()
@@ -86,6 +88,8 @@ def main():
 
     if args.mode == "newton":
         return newton_main(args)
+    if args.mode == "frame":
+        return frame_main(args)
 
     from xara_amd import meshes
     from xara_amd.model import Model
@@ -211,6 +215,83 @@ def newton_main(args):
                    "plastic_fraction": round(plastic, 3), "setup_s": round(setup_s, 2),
                    "p_y": p_y, "final_load_factor": m.load_factor},
     }
+    print(json.dumps(out))
+
+
+def frame_bytes(nsteel, nconc, nsec):
+    # steel fiber: committed history (10 doubles + kon) in, trial out = 168 B;
+    # concrete fiber: committed ecmin, dept, eps, sig + previous trial sig, e
+    # in (48 B), trial history out (40 B) = 88 B; section: deformation,
+    # resultant and 4x4 tangent out = 192 B
+    return nsteel * 168 + nconc * 88 + nsec * 192
+
+
+def frame_field(fr, amp=60.0):
+    z = fr["crds"][:, 2]
+    h = z.max()
+    U = np.zeros((len(z), 6))
+    U[:, 0] = amp * (z / h) * (1.5 - 0.5 * z / h)
+    U[:, 1] = 0.25 * amp * (z / h) ** 2
+    U[:, 2] = -2e-4 * amp * z / h
+    U[:, 4] = 1.5 * amp / h * (1 - z / h)
+    return U
+
+
+def frame_main(args):
+    """Fiber update rate of the 3D RC moment frame (dispBeamColumn3d, 5 Gauss
+    points, 100 Concrete02 + 20 Steel02 fibers per section)."""
+    from xara_amd import meshes
+    from xara_amd.model import Model
+
+    nbx, nby, nst = (int(v) for v in args.frame.split(","))
+    fr = meshes.rc_frame_3d(nbx, nby, nst)
+    m = Model(3, 6, device=0)
+    meshes.build_rc_frame_3d(m, fr)
+    m.system(args.system, 1e-12, 200000)
+    t0 = time.time()
+    m.setup()
+    setup_s = time.time() - t0
+    nele = len(fr["columns"]) + len(fr["beams_x"]) + len(fr["beams_y"])
+    nsec = 5 * nele
+    nsteel, nconc = 20 * nsec, 100 * nsec
+    m.set_trial_disp(frame_field(fr))
+    m.time_op("update", args.warmup)
+    t0 = time.perf_counter()
+    ev_ms = m.time_op("update", args.steps)
+    wall = time.perf_counter() - t0
+    nfib = nsteel + nconc
+    value = nfib * args.steps / wall
+    byt = frame_bytes(nsteel, nconc, nsec)
+    achieved = byt / (ev_ms * 1e-3) / 1e9
+    out = {
+        "metric": "fiber updates/s", "value": value, "unit": "fibers/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D RC frame {nbx}x{nby} bays x {nst} stories, dispBeamColumn3d update",
+                   "elements": nele, "sections": nsec, "fibers": nfib, "neq": m.neq, "setup_s": round(setup_s, 2)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_launch": byt,
+                     "event_ms": ev_ms},
+    }
+    if args.frame_steps > 0:
+        m.revert_to_start()
+        m.test("NormUnbalance", 1e-3, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 0.03 * fr["height"] / 300.0)
+        c0 = m.counters()
+        t0 = time.perf_counter()
+        for st in range(args.frame_steps):
+            rc = m.analyze(1)
+            c = m.counters()
+            print(f"bench: frame step {st + 1} rc={rc} lambda={m.load_factor:.4g} newton="
+                  f"{c['newton_iters'] - c0['newton_iters']} cg={c['cg_iters'] - c0['cg_iters']} "
+                  f"t={time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+            if rc != 0:
+                break
+        wall = time.perf_counter() - t0
+        c = m.counters()
+        out["newton"] = {"steps": args.frame_steps, "newton_iters": c["newton_iters"] - c0["newton_iters"],
+                         "cg_iters": c["cg_iters"] - c0["cg_iters"], "wall_s": wall,
+                         "newton_iters_per_s": (c["newton_iters"] - c0["newton_iters"]) / wall}
     print(json.dumps(out))
 
 

@@ -1,0 +1,121 @@
+"""GPU parity of the frame path (DispBeamColumn2d/3d over Steel02 /
+Concrete02 fiber sections) against the CPU oracle through the C-ABI."""
+import numpy as np
+import pytest
+
+from tests import _oracle
+from xara_amd import meshes
+from xara_amd.model import Model
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-9
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    s = np.max(np.abs(b))
+    return 0.0 if s == 0 else float(np.max(np.abs(a - b)) / s)
+
+
+def frame3d_pair(nbx=2, nby=2, nst=3):
+    fr = meshes.rc_frame_3d(nbx, nby, nst)
+    ms = []
+    for m in (Model(3, 6), _oracle.oracle_model(3, 6)):
+        meshes.build_rc_frame_3d(m, fr)
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        m.setup()
+        ms.append(m)
+    return fr, ms
+
+
+def lateral_field(fr, amp, seed):
+    rng = np.random.default_rng(seed)
+    z = fr["crds"][:, 2]
+    U = np.zeros((len(z), 6))
+    U[:, 0] = amp * z / z.max()
+    U[:, 1] = 0.3 * amp * (z / z.max()) ** 2
+    U[:, 2] = -1e-3 * amp * z / z.max()
+    U += rng.normal(0, abs(amp) * 0.05, U.shape) * (z[:, None] > 0)
+    U[:, 3:] *= 1e-4
+    return U
+
+
+def test_frame3d_numbering_bit_exact():
+    print("case: frame3d numbering", flush=True)
+    fr, (g, o) = frame3d_pair()
+    assert np.array_equal(g.node_eqns(), o.node_eqns())
+    for a, b in zip(g.csr_pattern(), o.csr_pattern()):
+        assert np.array_equal(a, b)
+
+
+def test_frame3d_update_sections_and_assembly():
+    print("case: frame3d update/assembly", flush=True)
+    fr, (g, o) = frame3d_pair()
+    nsec = 5 * (len(fr["columns"]) + len(fr["beams_x"]) + len(fr["beams_y"]))
+    for step, amp in enumerate((20.0, 60.0, -35.0)):
+        U = lateral_field(fr, amp, step)
+        for m in (g, o):
+            m.set_trial_disp(U)
+            m.update()
+        for what in ("strain", "stress", "tangent"):
+            assert relerr(g.point_data(what, nsec, 16), o.point_data(what, nsec, 16)) <= RTOL, (what, step)
+        for m in (g, o):
+            m.set_load_factor(1000.0 * (step + 1))
+        assert relerr(g.form_unbalance(), o.form_unbalance()) <= RTOL
+        assert relerr(g.form_tangent(), o.form_tangent()) <= RTOL
+        for m in (g, o):
+            m.commit()
+    tag = len(fr["columns"]) + 3
+    assert relerr(g.element_tangent(tag), o.element_tangent(tag)) <= RTOL
+    assert relerr(g.element_resisting_force(tag), o.element_resisting_force(tag)) <= RTOL
+
+
+def test_frame3d_revert_matches_reference_semantics():
+    print("case: frame3d revert", flush=True)
+    fr, (g, o) = frame3d_pair(1, 1, 2)
+    nsec = 5 * (len(fr["columns"]) + len(fr["beams_x"]) + len(fr["beams_y"]))
+    for m in (g, o):
+        m.set_trial_disp(lateral_field(fr, 40.0, 1))
+        m.update()
+        m.commit()
+        m.set_trial_disp(lateral_field(fr, 90.0, 2))
+        m.update()
+        m.revert()
+    for what in ("stress", "tangent"):
+        assert relerr(g.point_data(what, nsec, 16), o.point_data(what, nsec, 16)) <= RTOL
+
+
+def test_frame3d_pushover_trajectory():
+    print("case: frame3d pushover", flush=True)
+    fr, (g, o) = frame3d_pair(2, 1, 3)
+    for m in (g, o):
+        m.test("NormUnbalance", 1e-5, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 6.0)
+    for step in range(12):
+        for m in (g, o):
+            assert m.analyze(1) == 0
+        assert relerr(g.trial_disp(), o.trial_disp()) <= RTOL, step
+        assert abs(g.load_factor - o.load_factor) <= RTOL * abs(o.load_factor)
+
+
+def test_frame2d_pushover_trajectory():
+    print("case: frame2d pushover", flush=True)
+    fr = meshes.rc_frame_2d(4, 2)
+    res = []
+    for m in (Model(2, 3), _oracle.oracle_model(2, 3)):
+        meshes.build_rc_frame_2d(m, fr)
+        P = np.zeros((len(fr["floor_nodes"]), 3))
+        P[:, 0] = fr["floor_level"] / fr["floor_level"].max()
+        m.loads(fr["floor_nodes"], P)
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        m.test("NormUnbalance", 1e-5, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 8.0)
+        m.setup()
+        traj = []
+        for step in range(15):
+            assert m.analyze(1) == 0
+            traj.append((m.trial_disp().copy(), m.load_factor))
+        res.append(traj)
+    for (ug, lg), (uo, lo) in zip(*res):
+        assert relerr(ug, uo) <= RTOL
+        assert abs(lg - lo) <= RTOL * abs(lo)

@@ -74,3 +74,114 @@ def build_brick_cube(model, mesh, mat_tag=1, mat=J2_STEEL, pressure=1.0):
         P = np.zeros((len(mesh["load_tags"]), 3))
         P[:, 2] = pressure * mesh["Pz"]
         model.loads(mesh["load_tags"], P)
+
+
+# ---------------------------------------------------------------------------
+# Reinforced-concrete frames
+# ---------------------------------------------------------------------------
+STEEL02 = (420.0, 200e3, 0.01, 18.0, 0.925, 0.15)           # fy E b R0 cR1 cR2
+CONCRETE02 = (-30.0, -0.002, -6.0, -0.02, 0.1, 3.0, 1500.0)  # fc epsc0 fcu epscu rat ft Ets
+
+
+def rc_section_fibers(width=500.0, depth=500.0, cover=50.0, nc=10, bar_area=314.0,
+                      conc_tag=1, steel_tag=2):
+    """500 x 500 RC section: a 10 x 10 concrete patch (100 Concrete02 fibers)
+    and 20 Steel02 bars on the perimeter (6 + 6 on two faces, 4 + 4 on the
+    others).  Returns rows (y, z, A, matTag)."""
+    dy, dz = depth / nc, width / nc
+    rows = []
+    for i in range(nc):
+        for j in range(nc):
+            rows.append((-depth / 2 + (i + 0.5) * dy, -width / 2 + (j + 0.5) * dz, dy * dz, conc_tag))
+    yb = depth / 2 - cover
+    zb = width / 2 - cover
+    for z in np.linspace(-zb, zb, 6):
+        rows.append((yb, z, bar_area, steel_tag))
+        rows.append((-yb, z, bar_area, steel_tag))
+    for y in np.linspace(-yb, yb, 6)[1:-1]:
+        rows.append((y, zb, bar_area, steel_tag))
+        rows.append((y, -zb, bar_area, steel_tag))
+    return rows
+
+
+def rc_frame_3d(nbx=10, nby=10, nst=20, bay=6000.0, story=3500.0):
+    """Space frame: columns (i,j,k)-(i,j,k+1), beams along x and y at every
+    floor.  Node tag 1 + i + (nbx+1) j + (nbx+1)(nby+1) k; columns first, then
+    x beams, then y beams.  Base nodes fixed.  Returns arrays plus the
+    inverted-triangle lateral pattern (x) and the roof control node."""
+    mx, my = nbx + 1, nby + 1
+    k, j, i = np.meshgrid(np.arange(nst + 1), np.arange(my), np.arange(mx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    node_tags = (1 + i + mx * j + mx * my * k).astype(np.int32)
+    crds = np.stack([bay * i, bay * j, story * k], axis=1).astype(np.float64)
+
+    def tag(ii, jj, kk):
+        return 1 + ii + mx * jj + mx * my * kk
+
+    cols, bx, by = [], [], []
+    for kk in range(nst):
+        for jj in range(my):
+            for ii in range(mx):
+                cols.append((tag(ii, jj, kk), tag(ii, jj, kk + 1)))
+    for kk in range(1, nst + 1):
+        for jj in range(my):
+            for ii in range(nbx):
+                bx.append((tag(ii, jj, kk), tag(ii + 1, jj, kk)))
+        for jj in range(nby):
+            for ii in range(mx):
+                by.append((tag(ii, jj, kk), tag(ii, jj + 1, kk)))
+    cols, bx, by = (np.array(a, np.int32).reshape(-1, 2) for a in (cols, bx, by))
+    fixed = node_tags[k == 0]
+    floors = k > 0
+    load_tags = node_tags[floors]
+    Px = (k[floors] / nst).astype(np.float64)  # inverted triangle
+    roof = int(tag(nbx // 2, nby // 2, nst))
+    return dict(node_tags=node_tags, crds=crds, columns=cols, beams_x=bx, beams_y=by,
+                fixed_nodes=fixed.astype(np.int32), load_tags=load_tags.astype(np.int32), Px=Px,
+                roof=roof, height=story * nst)
+
+
+def build_rc_frame_3d(model, fr, GJ=1.1e14):
+    model.nodes(fr["node_tags"], fr["crds"])
+    fx = fr["fixed_nodes"]
+    model.fix_many(fx, np.ones((len(fx), 6), np.int32))
+    model.uniaxialMaterial("Concrete02", 1, *CONCRETE02)
+    model.uniaxialMaterial("Steel02", 2, *STEEL02)
+    model.section("Fiber", 1, rc_section_fibers(), GJ=GJ)
+    model.geomTransf("Linear", 1, 1.0, 0.0, 0.0)  # columns (axis z)
+    model.geomTransf("Linear", 2, 0.0, 0.0, 1.0)  # beams
+    nc, nx_, ny_ = len(fr["columns"]), len(fr["beams_x"]), len(fr["beams_y"])
+    model.elements("dispBeamColumn", np.arange(1, nc + 1), fr["columns"], 1, 1, 5)
+    model.elements("dispBeamColumn", np.arange(nc + 1, nc + nx_ + 1), fr["beams_x"], 2, 1, 5)
+    model.elements("dispBeamColumn", np.arange(nc + nx_ + 1, nc + nx_ + ny_ + 1), fr["beams_y"], 2, 1, 5)
+    P = np.zeros((len(fr["load_tags"]), 6))
+    P[:, 0] = fr["Px"]
+    model.loads(fr["load_tags"], P)
+
+
+def rc_frame_2d(nst, nbay, bay=6000.0, story=3500.0, tag0=0):
+    """Plane frame: (nbay+1) columns lines, nst stories; node tag
+    tag0 + 1 + i + (nbay+1) k; columns then beams.  Base fixed."""
+    m = nbay + 1
+    k, i = np.meshgrid(np.arange(nst + 1), np.arange(m), indexing="ij")
+    i, k = i.ravel(), k.ravel()
+    node_tags = (tag0 + 1 + i + m * k).astype(np.int32)
+    crds = np.stack([bay * i, story * k], axis=1).astype(np.float64)
+    cols = [(tag0 + 1 + ii + m * kk, tag0 + 1 + ii + m * (kk + 1)) for kk in range(nst) for ii in range(m)]
+    beams = [(tag0 + 1 + ii + m * kk, tag0 + 2 + ii + m * kk) for kk in range(1, nst + 1) for ii in range(nbay)]
+    return dict(node_tags=node_tags, crds=crds, columns=np.array(cols, np.int32),
+                beams=np.array(beams, np.int32), fixed_nodes=node_tags[k == 0],
+                floor_nodes=node_tags[k > 0], floor_level=k[k > 0], roof=int(tag0 + 1 + m * nst))
+
+
+def build_rc_frame_2d(model, fr, ele0=1):
+    model.nodes(fr["node_tags"], fr["crds"])
+    fx = fr["fixed_nodes"]
+    model.fix_many(fx, np.ones((len(fx), 3), np.int32))
+    model.uniaxialMaterial("Concrete02", 1, *CONCRETE02)
+    model.uniaxialMaterial("Steel02", 2, *STEEL02)
+    model.section("Fiber", 1, rc_section_fibers())
+    model.geomTransf("Linear", 1)
+    nc, nb = len(fr["columns"]), len(fr["beams"])
+    model.elements("dispBeamColumn", np.arange(ele0, ele0 + nc), fr["columns"], 1, 1, 5)
+    model.elements("dispBeamColumn", np.arange(ele0 + nc, ele0 + nc + nb), fr["beams"], 1, 1, 5)
