@@ -507,12 +507,13 @@ static int form_unbalance(Dom& d) {
       int a = n.eq[j];
       if (a < 0) continue;
       double u = d.lambda * n.P[j];
-      if (d.integ == 3) {
-        // nodal inertia -m(a + ag r) and mass-proportional damping
-        double r = (j == d.gm_dir) ? 1.0 : 0.0;
-        u -= n.mass[j] * (n.A[j] + ag * r);
-        u -= d.alphaM * n.mass[j] * n.V[j];
-        // element-mass ground loads
+      if (d.integ == 3 && n.mass[j] != 0.0) {
+        // Node::addInertiaLoadToUnbalance (-M R ag, Node.cpp:668-688), then
+        // Node::getUnbalancedLoadIncInertia (-M A, -alphaM M V; :740-763)
+        const double r = (j == d.gm_dir) ? 1.0 : 0.0;
+        if (r != 0.0) u += -(n.mass[j] * r) * ag;
+        u += -n.mass[j] * n.A[j];
+        if (d.alphaM != 0.0) u += -d.alphaM * n.mass[j] * n.V[j];
       }
       d.B[a] += u;
     }

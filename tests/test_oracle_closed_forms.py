@@ -398,3 +398,71 @@ def test_newton_elastic_one_step_equals_linear_solve():
             if eq[i, j] >= 0:
                 got[eq[i, j]] = U[i, j]
     assert np.allclose(got, x, rtol=1e-10, atol=1e-14)
+
+
+def test_newmark_linear_matches_textbook_recursion():
+    """Elastic 2D frame under ground acceleration: the oracle's Newmark
+    (average acceleration, mass-proportional damping) equals the textbook
+    recursion on the assembled K and M (Chopra, Table 5.4.2)."""
+    from xara_amd import ground_motion
+    fr = meshes.rc_frame_2d(3, 2)
+    m = _oracle.oracle_model(2, 3)
+    m.nodes(fr["node_tags"], fr["crds"])
+    m.fix_many(fr["fixed_nodes"], np.ones((len(fr["fixed_nodes"]), 3), np.int32))
+    m.uniaxialMaterial("Elastic", 1, 25000.0)
+    m.section("Fiber", 1, [(y, 0.0, a, 1) for (y, z, a, t) in meshes.rc_section_fibers()])
+    m.geomTransf("Linear", 1)
+    nc, nb = len(fr["columns"]), len(fr["beams"])
+    m.elements("dispBeamColumn", np.arange(1, nc + 1), fr["columns"], 1, 1, 5)
+    m.elements("dispBeamColumn", np.arange(nc + 1, nc + nb + 1), fr["beams"], 1, 1, 5)
+    mass = np.zeros((len(fr["floor_nodes"]), 3))
+    mass[:, 0] = mass[:, 1] = 40.0
+    m.masses(fr["floor_nodes"], mass)
+    dt, alphaM = 0.01, 0.4
+    ag = ground_motion.kanai_tajimi(3, 200, dt, 0.3)
+    m._check(m._b.uniform_excitation(m._h, 0, len(ag), dt, ag.ctypes.data_as(__import__("xara_amd")._lib.P_D), 1.0))
+    m._check(m._b.rayleigh(m._h, alphaM, 0.0, 0.0, 0.0))
+    m.integrator("LoadControl", 0.0)
+    m.test("NormDispIncr", 1e-12, 10)
+    m.setup()
+    rp, ci = m.csr_pattern()
+    Kv = m.form_tangent()
+    m.integrator("Newmark", 0.5, 0.25)
+    n = m.neq
+    Kd = np.zeros((n, n))
+    for a in range(n):
+        Kd[a, ci[rp[a]:rp[a + 1]]] = Kv[rp[a]:rp[a + 1]]
+    eq = m.node_eqns()
+    tags = np.sort(fr["node_tags"])
+    Mv = np.zeros(n)
+    rvec = np.zeros(n)
+    for i, t in enumerate(tags):
+        if t in set(fr["floor_nodes"].tolist()):
+            for j in (0, 1):
+                Mv[eq[i, j]] = 40.0
+            rvec[eq[i, 0]] = 1.0
+    M = np.diag(Mv)
+    C = alphaM * M
+    g, b = 0.5, 0.25
+    u = np.zeros(n); v = np.zeros(n); a = np.zeros(n)
+    Khat = Kd + g / (b * dt) * C + 1 / (b * dt * dt) * M
+    traj = []
+    for k in range(1, 60):
+        t = k * dt
+        p = -M @ rvec * np.interp(t, np.arange(len(ag)) * dt, ag)
+        rhs = p + M @ (u / (b * dt * dt) + v / (b * dt) + (1 / (2 * b) - 1) * a) \
+            + C @ (g / (b * dt) * u + (g / b - 1) * v + dt * (g / (2 * b) - 1) * a)
+        un = np.linalg.solve(Khat, rhs)
+        vn = g / (b * dt) * (un - u) + (1 - g / b) * v + dt * (1 - g / (2 * b)) * a
+        an = (un - u) / (b * dt * dt) - v / (b * dt) - (1 / (2 * b) - 1) * a
+        u, v, a = un, vn, an
+        traj.append(u.copy())
+    for k in range(1, 60):
+        assert m.analyze(1, dt) == 0
+        U = m.trial_disp()
+        got = np.zeros(n)
+        for i in range(len(U)):
+            for j in range(3):
+                if eq[i, j] >= 0:
+                    got[eq[i, j]] = U[i, j]
+        assert np.allclose(got, traj[k - 1], rtol=1e-8, atol=1e-9 * np.abs(traj[k - 1]).max() + 1e-14), k

@@ -308,25 +308,27 @@ int xg_nodal_loads(xg_domain* h, int n, const int* tags, const double* P) {
 
 int xg_uniform_excitation(xg_domain* h, int dir, int npts, double dt, const double* accel, double factor) {
   return guard([&] {
-    (void)h;
-    (void)dir;
-    (void)npts;
-    (void)dt;
-    (void)accel;
-    (void)factor;
-    throw xg::Error(XG_ERR_ARG, "UniformExcitation runs through the frame ensemble entry points");
+    auto& d = D(h);
+    if (dir < 0 || dir >= d.ndm) throw xg::Error(XG_ERR_ARG, "UniformExcitation: translational direction only");
+    if (npts < 1 || !(dt > 0.0)) throw xg::Error(XG_ERR_ARG, "UniformExcitation: empty record");
+    d.gm_dir = dir;
+    d.gm_dt = dt;
+    d.gm_factor = factor;
+    d.gm_accel.assign(accel, accel + npts);
+    if (d.ready) {
+      d.setup_transient();
+      XG_HIP(hipStreamSynchronize(d.stream));
+    }
     return 0;
   });
 }
 
 int xg_rayleigh(xg_domain* h, double alphaM, double betaK, double betaKinit, double betaKcomm) {
   return guard([&] {
-    (void)h;
-    (void)alphaM;
-    (void)betaK;
-    (void)betaKinit;
-    (void)betaKcomm;
-    throw xg::Error(XG_ERR_ARG, "rayleigh runs through the frame ensemble entry points");
+    auto& d = D(h);
+    if (betaK != 0.0 || betaKinit != 0.0 || betaKcomm != 0.0)
+      throw xg::Error(XG_ERR_ARG, "rayleigh: only mass-proportional damping (alphaM) is supported");
+    d.alphaM = alphaM;
     return 0;
   });
 }
@@ -390,10 +392,11 @@ int xg_integrator_disp_control(xg_domain* h, int node_tag, int dof, double incr)
 
 int xg_integrator_newmark(xg_domain* h, double gamma, double beta) {
   return guard([&] {
-    (void)h;
-    (void)gamma;
-    (void)beta;
-    throw xg::Error(XG_ERR_ARG, "Newmark runs through the frame ensemble entry points");
+    auto& d = D(h);
+    if (!(beta > 0.0)) throw xg::Error(XG_ERR_ARG, "Newmark: beta must be positive");
+    d.integ = 3;
+    d.nm_gamma = gamma;
+    d.nm_beta = beta;
     return 0;
   });
 }

@@ -376,11 +376,56 @@ void Domain::setup() {
   scal.resize(cg_scal_bytes());
   dscalar.resize(4);
   for (auto& g : groups) g->setup(*this);
+  setup_transient();
   XG_HIP(hipStreamSynchronize(s));
   lambda = 0;
   time = 0;
   dc_ready = false;
   ready = true;
+}
+
+// Per-equation nodal mass, influence vector of the UniformExcitation
+// direction, equation -> node DOF map and the CSR positions of massive
+// diagonals.
+void Domain::setup_transient() {
+  const int neq = num.neq;
+  std::vector<double> m(neq, 0.0), r(neq, 0.0), mv;
+  std::vector<int> e2d(neq, 0);
+  std::vector<long long> pos;
+  for (int n = 0; n < nnode; n++) {
+    const NodeDef& nd = nodedefs.at(node_tags[n]);
+    for (int j = 0; j < ndf; j++) {
+      const int a = num.eq[(size_t)n * ndf + j];
+      if (a < 0) continue;
+      e2d[a] = n * ndf + j;
+      m[a] = nd.mass[j];
+      r[a] = (j == gm_dir) ? 1.0 : 0.0;
+    }
+  }
+  for (int a = 0; a < neq; a++)
+    if (m[a] != 0.0) {
+      const int* b = num.colind.data() + num.rowptr[a];
+      const int* e = num.colind.data() + num.rowptr[a + 1];
+      pos.push_back(std::lower_bound(b, e, a) - num.colind.data());
+      mv.push_back(m[a]);
+    }
+  nmass_eq = (int)pos.size();
+  Meq.upload(m, stream);
+  Req.upload(r, stream);
+  eq2dof.upload(e2d, stream);
+  mass_pos.upload(pos, stream);
+  mass_val.upload(mv, stream);
+}
+
+// PathSeries::getFactor (PathSeries.cpp:334-356) scaled by the motion factor
+double Domain::ground_accel(double t) const {
+  if (gm_accel.empty() || gm_dt <= 0.0 || t < 0.0) return 0.0;
+  const double incr = t / gm_dt;
+  const int i1 = (int)std::floor(incr);
+  const int i2 = i1 + 1;
+  if (i2 >= (int)gm_accel.size()) return 0.0;
+  const double v1 = gm_accel[i1], v2 = gm_accel[i2];
+  return gm_factor * (v1 + (v2 - v1) * (incr - i1));
 }
 
 void Domain::update() {
@@ -394,12 +439,18 @@ void Domain::update() {
 
 void Domain::form_unbalance() {
   for (auto& g : groups) g->resid(*this, Re.p + g->re_off);
-  launch_assemble_rhs(stream, num.neq, rhs_ptr.p, rhs_idx.p, Re.p, lambda, Peq.p, B.p);
+  if (integ == 3)
+    launch_assemble_rhs_dyn(stream, num.neq, rhs_ptr.p, rhs_idx.p, Re.p, lambda, Peq.p, Meq.p, Req.p, ag, alphaM,
+                            eq2dof.p, V.p, A.p, B.p);
+  else
+    launch_assemble_rhs(stream, num.neq, rhs_ptr.p, rhs_idx.p, Re.p, lambda, Peq.p, B.p);
 }
 
 void Domain::form_tangent() {
   for (auto& g : groups) g->tangent(*this, Ke.p + g->ke_off);
   launch_assemble_csr(stream, (long long)num.colind.size(), asm_ptr.p, asm_idx.p, Ke.p, vals.p);
+  // Newmark::formNodTangent: c2 C + c3 M of the nodal masses (C = alphaM M)
+  if (integ == 3) launch_add_diag(stream, nmass_eq, mass_pos.p, mass_val.p, nm_c3 + nm_c2 * alphaM, vals.p);
 }
 
 void Domain::precond_setup() {
@@ -507,6 +558,20 @@ int Domain::run_test(int& iter, std::vector<double>& norms, int& numIncr) {
 // ---- integrators --------------------------------------------------------------
 int Domain::new_step() {
   const int n = num.neq;
+  if (integ == 3) {  // Newmark::newStep, Newmark.cpp:105-170 (displacement formulation)
+    const double g = nm_gamma, b = nm_beta, h = dt;
+    nm_c2 = g / (b * h);
+    nm_c3 = 1.0 / (b * h * h);
+    const double a1 = (1.0 - g / b);
+    const double a2 = h * (1.0 - 0.5 * g / b);
+    const double a3 = -1.0 / (b * h);
+    const double a4 = 1.0 - 0.5 / b;
+    launch_newmark_predict(stream, (long long)nnode * ndf, eqmap.p, Uc.p, Vc.p, Ac.p, a1, a2, a3, a4, U.p, V.p, A.p);
+    time += h;
+    ag = ground_accel(time);  // EarthquakePattern::applyLoad(time)
+    update();
+    return 0;
+  }
   if (integ == 1) {  // LoadControl::newStep, LoadControl.cpp:70-95
     lc_dlambda *= lc_spec / lc_last;
     if (lc_dlambda < lc_min) lc_dlambda = lc_min;
@@ -535,6 +600,11 @@ int Domain::new_step() {
 
 int Domain::integ_update(const double* x) {
   const int n = num.neq;
+  if (integ == 3) {  // Newmark::update, Newmark.cpp:411-458
+    launch_newmark_correct(stream, (long long)nnode * ndf, eqmap.p, x, nm_c2, nm_c3, U.p, V.p, A.p);
+    update();
+    return 0;
+  }
   if (integ == 1) {  // LoadControl::update, LoadControl.cpp:102-120
     incr_disp(x);
     update();
@@ -577,7 +647,12 @@ int Domain::newton() {
 
 int Domain::analyze(int nsteps, double dtv) {
   if (!ready) setup();
-  if (integ == 3) throw Error(XG_ERR_STATE, "transient analysis runs through the frame ensemble entry points");
+  if (integ == 3) {
+    if (!(dtv > 0.0)) throw Error(XG_ERR_ARG, "analyze: transient analysis needs dt > 0");
+    for (auto& kv : eledefs)
+      if (kv.second.rho != 0.0) throw Error(XG_ERR_ARG, "transient analysis: element mass density is not supported");
+    dt = dtv;
+  }
   if (integ == 2 && !dc_ready) {
     // DisplacementControl::domainChanged, :286-415: phat = B(lambda+1) at the
     // current (zero-unbalance) state
@@ -599,12 +674,13 @@ int Domain::analyze(int nsteps, double dtv) {
   }
   (void)dtv;
   for (int st = 0; st < nsteps; st++) {
-    const double lam0 = lambda;
+    const double lam0 = lambda, t0 = time;
     int r = new_step();
     if (r == 0) r = newton();
     if (r < 0) {
       revert();
       lambda = lam0;
+      time = t0;
       return r;
     }
     commit();

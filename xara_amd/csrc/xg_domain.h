@@ -118,6 +118,18 @@ struct Domain {
   bool dc_ready = false;
   DVec<double> phat, dUhat, dUbar, dU;
   double lambda = 0, time = 0, dt = 0;
+  // transient (Newmark, UniformExcitation + PathSeries, mass-proportional damping)
+  double nm_gamma = 0.5, nm_beta = 0.25, nm_c2 = 0, nm_c3 = 0;
+  double alphaM = 0;
+  int gm_dir = -1;
+  double gm_dt = 0, gm_factor = 1, ag = 0;
+  std::vector<double> gm_accel;
+  DVec<double> Meq, Req, mass_val;
+  DVec<int> eq2dof;
+  DVec<long long> mass_pos;
+  int nmass_eq = 0;
+  void setup_transient();
+  double ground_accel(double t) const;
   xg_counters counters{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 

@@ -33,6 +33,14 @@ void launch_assemble_rhs(hipStream_t s, int neq, const int* ptr, const int* idx,
                          double lambda, const double* P, double* B);
 void launch_add_diag(hipStream_t s, int n, const long long* pos, const double* m, double coef,
                      double* vals);
+void launch_assemble_rhs_dyn(hipStream_t s, int neq, const int* ptr, const int* idx, const double* Re,
+                             double lambda, const double* P, const double* Meq, const double* Req, double ag,
+                             double alphaM, const int* eq2dof, const double* V, const double* A, double* B);
+void launch_newmark_predict(hipStream_t s, long long ndofs, const int* eqmap, const double* Uc, const double* Vc,
+                            const double* Ac, double a1, double a2, double a3, double a4, double* U, double* V,
+                            double* A);
+void launch_newmark_correct(hipStream_t s, long long ndofs, const int* eqmap, const double* dU, double c2, double c3,
+                            double* U, double* V, double* A);
 int spmv_blocks(int n, int lanes);
 void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
                  const double* x, double* y, double* partial, const int* stop);

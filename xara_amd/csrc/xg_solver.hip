@@ -39,6 +39,58 @@ __global__ void k_assemble_rhs(int neq, const int* __restrict__ ptr, const int* 
   B[a] = acc;
 }
 
+// Transient right-hand side (TransientIntegrator::formUnbalance with
+// Node::getUnbalancedLoadIncInertia, Node.cpp:740-763, and the
+// EarthquakePattern inertia load -M R a_g, Node.cpp:668-688):
+//   B[a] = sum_e -R_e + (lambda P[a] - m r a_g - m A - alphaM m V)
+__global__ void k_assemble_rhs_dyn(int neq, const int* __restrict__ ptr, const int* __restrict__ idx,
+                                   const double* __restrict__ Re, double lambda, const double* __restrict__ P,
+                                   const double* __restrict__ Meq, const double* __restrict__ Req, double ag,
+                                   double alphaM, const int* __restrict__ eq2dof, const double* __restrict__ V,
+                                   const double* __restrict__ A, double* __restrict__ B) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= neq) return;
+  double acc = 0.0;
+  for (int j = ptr[a]; j < ptr[a + 1]; j++) acc += -Re[idx[j]];
+  double u = 0.0;
+  if (P) u = lambda * P[a];
+  const double m = Meq[a];
+  if (m != 0.0) {
+    const long long d = eq2dof[a];
+    u += -(m * Req[a]) * ag;
+    u += -m * A[d];
+    if (alphaM != 0.0) u += -alphaM * m * V[d];
+  }
+  B[a] = acc + u;
+}
+
+// Newmark predictor on free DOFs (Newmark::newStep, Newmark.cpp:105-170):
+// U = U_n, V = a1 V_n + a2 A_n, A = a4 A_n + a3 V_n
+__global__ void k_newmark_predict(long long ndofs, const int* __restrict__ eqmap, const double* __restrict__ Uc,
+                                  const double* __restrict__ Vc, const double* __restrict__ Ac, double a1,
+                                  double a2, double a3, double a4, double* __restrict__ U, double* __restrict__ V,
+                                  double* __restrict__ A) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ndofs || eqmap[i] < 0) return;
+  U[i] = Uc[i];
+  V[i] = Vc[i] * a1 + Ac[i] * a2;
+  A[i] = Ac[i] * a4 + Vc[i] * a3;
+}
+
+// Newmark::update (Newmark.cpp:411-458): U += dU, V += c2 dU, A += c3 dU
+__global__ void k_newmark_correct(long long ndofs, const int* __restrict__ eqmap, const double* __restrict__ dU,
+                                  double c2, double c3, double* __restrict__ U, double* __restrict__ V,
+                                  double* __restrict__ A) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ndofs) return;
+  const int a = eqmap[i];
+  if (a < 0) return;
+  const double d = dU[a];
+  U[i] += d;
+  V[i] += c2 * d;
+  A[i] += c3 * d;
+}
+
 // Diagonal add (nodal masses etc.): vals[pos[i]] += coef * m[i]
 __global__ void k_add_diag(int n, const long long* __restrict__ pos, const double* __restrict__ m,
                            double coef, double* __restrict__ vals) {
@@ -374,6 +426,26 @@ void launch_assemble_rhs(hipStream_t s, int neq, const int* ptr, const int* idx,
   if (neq <= 0) return;
   hipLaunchKernelGGL(k_assemble_rhs, dim3(ceil_div(neq, 256)), dim3(256), 0, s, neq, ptr, idx, Re, lambda,
                      P, B);
+}
+void launch_assemble_rhs_dyn(hipStream_t s, int neq, const int* ptr, const int* idx, const double* Re,
+                             double lambda, const double* P, const double* Meq, const double* Req, double ag,
+                             double alphaM, const int* eq2dof, const double* V, const double* A, double* B) {
+  if (neq <= 0) return;
+  hipLaunchKernelGGL(k_assemble_rhs_dyn, dim3(ceil_div(neq, 256)), dim3(256), 0, s, neq, ptr, idx, Re, lambda, P,
+                     Meq, Req, ag, alphaM, eq2dof, V, A, B);
+}
+void launch_newmark_predict(hipStream_t s, long long ndofs, const int* eqmap, const double* Uc, const double* Vc,
+                            const double* Ac, double a1, double a2, double a3, double a4, double* U, double* V,
+                            double* A) {
+  if (ndofs <= 0) return;
+  hipLaunchKernelGGL(k_newmark_predict, dim3(ceil_div(ndofs, 256)), dim3(256), 0, s, ndofs, eqmap, Uc, Vc, Ac, a1,
+                     a2, a3, a4, U, V, A);
+}
+void launch_newmark_correct(hipStream_t s, long long ndofs, const int* eqmap, const double* dU, double c2, double c3,
+                            double* U, double* V, double* A) {
+  if (ndofs <= 0) return;
+  hipLaunchKernelGGL(k_newmark_correct, dim3(ceil_div(ndofs, 256)), dim3(256), 0, s, ndofs, eqmap, dU, c2, c3, U, V,
+                     A);
 }
 void launch_add_diag(hipStream_t s, int n, const long long* pos, const double* m, double coef,
                      double* vals) {

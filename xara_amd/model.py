@@ -200,6 +200,15 @@ class Model:
         P = _f64(P).reshape(len(tags), self.ndf)
         self._check(self._b.nodal_loads(self._h, len(tags), _i(tags), _d(P)))
 
+    def uniformExcitation(self, dof, accel, dt, factor=1.0):
+        """``pattern UniformExcitation tag dof -accel {Path -dt dt -values ...}``
+        (dof 1-based, translational)."""
+        a = _f64(accel)
+        self._check(self._b.uniform_excitation(self._h, int(dof) - 1, len(a), float(dt), _d(a), float(factor)))
+
+    def rayleigh(self, alphaM, betaK=0.0, betaKinit=0.0, betaKcomm=0.0):
+        self._check(self._b.rayleigh(self._h, alphaM, betaK, betaKinit, betaKcomm))
+
     # -- analysis objects -------------------------------------------------
     def system(self, kind, rtol=0.0, maxit=0):
         self._check(self._b.system(self._h, SYSTEMS[kind.lower()], rtol, maxit))
