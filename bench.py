@@ -76,7 +76,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run the 256 MiB copy kernel (profiler counter calibration)")
-    ap.add_argument("--mode", choices=["update", "newton", "frame"], default="update")
+    ap.add_argument("--mode", choices=["update", "newton", "frame", "ensemble"], default="update")
+    ap.add_argument("--models", type=int, default=4096, help="frames in the ensemble mode")
+    ap.add_argument("--record-steps", type=int, default=2000, help="ground-motion record length (steps)")
+    ap.add_argument("--looped", type=int, default=16, help="frames timed alone for the looped comparison")
+    ap.add_argument("--looped-steps", type=int, default=20)
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--frame", default="10,10,20", help="bays x, bays y, stories of the frame mode")
     ap.add_argument("--frame-steps", type=int, default=0, help="DisplacementControl steps in frame mode")
     ap.add_argument("--newton-steps", type=int, default=40)
@@ -90,6 +95,8 @@ def main():
         return newton_main(args)
     if args.mode == "frame":
         return frame_main(args)
+    if args.mode == "ensemble":
+        return ensemble_main(args)
 
     from xara_amd import meshes
     from xara_amd.model import Model
@@ -293,6 +300,141 @@ def frame_main(args):
                          "cg_iters": c["cg_iters"] - c0["cg_iters"], "wall_s": wall,
                          "newton_iters_per_s": (c["newton_iters"] - c0["newton_iters"]) / wall}
     print(json.dumps(out))
+
+
+def ensemble_main(args):
+    """Batched ensemble: model-steps/s of N uneven 2D RC frames advanced
+    together by Newmark + Newton under their own synthetic records, against
+    the same frames analysed one at a time on the GPU (looped single-model
+    path) and on one CPU core (oracle)."""
+    from xara_amd import ground_motion, meshes
+    from xara_amd.ensemble import Ensemble
+    from xara_amd.model import Model, XaraError
+
+    n, dt = args.models, 0.01
+    stories, bays, pga = meshes.frame_population(n)
+    t0 = time.time()
+    rec = ground_motion.kanai_tajimi_batch(np.arange(n), args.record_steps, dt, pga)
+    print(f"bench: {n} records in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    def define(m, k):
+        fr = meshes.rc_frame_2d(int(stories[k]), int(bays[k]))
+        meshes.define_transient_rc_frame_2d(m, fr, rec[k], dt)
+        return fr
+
+    t0 = time.time()
+    ens = Ensemble()
+    nsec = np.zeros(n, np.int64)
+    for k in range(n):
+        m = Model(2, 3)
+        fr = define(m, k)
+        nsec[k] = 5 * (len(fr["columns"]) + len(fr["beams"]))
+        ens.add(m)
+        m.close()
+    ens.setup()
+    setup_s = time.time() - t0
+    print(f"bench: ensemble of {n} frames set up in {setup_s:.1f}s", file=sys.stderr, flush=True)
+    byt_model = np.array([frame_bytes(20 * s, 100 * s, s) for s in nsec], np.float64)
+    tune = None
+    if args.warmup > 0:
+        ens.analyze(args.warmup, dt)
+    if not args.no_tune:
+        table = ens.tune()
+        best = table[np.argmin(table[:, 3])]
+        tune = {"block": int(best[0]), "grid": int(best[1]), "order": int(best[2]), "ms": float(best[3]),
+                "candidates": int(len(table))}
+        print(f"bench: autotune {tune}", file=sys.stderr, flush=True)
+    st0, it0 = ens.status()
+    alive0 = st0 > -2
+    t0 = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(50, args.steps - done)
+        stopped = ens.analyze(k, dt)
+        done += k
+        print(f"bench: ensemble step {args.warmup + done} stopped={stopped} t={time.perf_counter() - t0:.1f}s",
+              file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    st1, it1 = ens.status()
+    newton = (it1 - it0).astype(np.float64)
+    # a frame that stopped inside the timed region counts its steps up to the failure
+    model_steps = float(np.sum(alive0)) * args.steps
+    value = model_steps / wall
+    updates_bytes = float(np.sum(byt_model * (newton + alive0 * args.steps)))
+    achieved = updates_bytes / wall / 1e9
+
+    # looped single-model GPU path on the first frames of the same population
+    L = min(args.looped, n)
+    t_loop, steps_loop = 0.0, 0
+    for k in range(L):
+        m = Model(2, 3)
+        define(m, k)
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        m.setup()
+        for _ in range(args.warmup):
+            m.analyze(1, dt)
+        t1 = time.perf_counter()
+        ok = 0
+        try:
+            for _ in range(args.looped_steps):
+                m.analyze(1, dt)
+                ok += 1
+        except XaraError:
+            pass
+        t_loop += time.perf_counter() - t1
+        steps_loop += ok
+        m.close()
+    looped = steps_loop / t_loop if t_loop > 0 else None
+    print(f"bench: looped single-model path {looped:.1f} model-steps/s", file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "ensemble model-steps/s", "value": value, "unit": "model-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Kanai-Tajimi records, PGA 0.1-0.6 g)",
+        "config": {"workload": f"ensemble of {n} 2D RC frames, stories 3-20, bays 1-6, Newmark dt {dt}",
+                   "models": n, "sections": int(nsec.sum()), "fibers": int(120 * nsec.sum()),
+                   "equations": int(sum(3 * (int(s) * (int(b) + 1)) for s, b in zip(stories, bays))),
+                   "newton_iters_per_model_step": float(newton.sum() / max(model_steps, 1.0)),
+                   "stopped": int(np.sum(st1 <= -2)), "setup_s": round(setup_s, 1), "autotune": tune},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "note": "fiber-update bytes of every model update over the whole step time"},
+        "looped_single_model": {"models": L, "steps_each": args.looped_steps, "model_steps_per_s": looped,
+                                "ensemble_ratio": value / looped if looped else None},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = ensemble_cpu_baseline(define, min(n, 8), args.cpu_seconds, dt)
+    print(json.dumps(out))
+
+
+def ensemble_cpu_baseline(define, nsample, seconds, dt):
+    """The oracle (one core) stepping the first frames of the population."""
+    from tests import _oracle
+    from xara_amd.model import XaraError
+
+    models = []
+    for k in range(nsample):
+        o = _oracle.oracle_model(2, 3)
+        define(o, k)
+        o.setup()
+        models.append(o)
+    t0 = time.perf_counter()
+    steps = 0
+    live = list(models)
+    while live and time.perf_counter() - t0 < seconds:
+        nxt = []
+        for o in live:
+            try:
+                o.analyze(1, dt)
+                steps += 1
+                nxt.append(o)
+            except XaraError:
+                pass
+        live = nxt
+    wall = time.perf_counter() - t0
+    return {"value": steps / wall, "unit": "model-steps/s", "cores": 1, "kind": "port",
+            "sample": f"first {nsample} frames of the population, {steps} Newmark steps from rest"}
 
 
 if __name__ == "__main__":

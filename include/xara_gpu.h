@@ -215,6 +215,36 @@ int  xg_get_counters(xg_domain* d, xg_counters* c);
    HIP events on the stream the kernels run on.  *ms = average per launch.    */
 int  xg_time_op(xg_domain* d, int op, int reps, double* ms);
 
+
+/* ---- batched ensemble of independent 2D frame models ----------------------
+   No counterpart in the reference, which analyses one model per interpreter:
+   each member is a model defined through the entry points above (2D,
+   dispBeamColumn over fiber sections, nodal masses, UniformExcitation,
+   Newmark, NormDispIncr/NormUnbalance test) whose `analyze n dt` would run
+   BasicAnalysisBuilder::analyzeTransient (BasicAnalysisBuilder.cpp:422-520).
+   All members advance together: one persistent launch per Newton iteration,
+   one captured hipGraph per time step.  A member whose step fails is
+   reverted to its last committed state and stops, as its analyze would. */
+typedef struct xg_ensemble xg_ensemble;
+int  xg_ensemble_create(int device, xg_ensemble** out);
+int  xg_ensemble_destroy(xg_ensemble* e);
+/* snapshot of a defined (not necessarily set up) model; the handle may be
+   destroyed afterwards */
+int  xg_ensemble_add(xg_ensemble* e, xg_domain* model);
+int  xg_ensemble_setup(xg_ensemble* e);
+int  xg_ensemble_num_models(xg_ensemble* e);
+/* autotuner: times every (block, grid, model order) mapping and keeps the
+   fastest; writes up to `cap` rows of (block, grid, order, ms) into `table` */
+int  xg_ensemble_tune(xg_ensemble* e, double* table, int cap, int* nrows);
+int  xg_ensemble_set_mapping(xg_ensemble* e, int block, int grid, int order, int use_graph);
+/* nsteps Newmark steps of size dt for every running member; *nstopped =
+   members stopped by a failed step so far */
+int  xg_ensemble_analyze(xg_ensemble* e, int nsteps, double dt, int* nstopped);
+/* trial displacements of member `model` (its nodes in ascending tag order) */
+int  xg_ensemble_model_disp(xg_ensemble* e, int model, double* U, int* nnode);
+/* per-member status (0/1 running, -2 stopped) and Newton iteration totals */
+int  xg_ensemble_status(xg_ensemble* e, int* status, long long* newton_iters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -635,7 +635,7 @@ static int run_test(Dom& d, Test& t) {
   if (t.currentIter > 1 && !t.norms.empty() && t.norms.back() < norm) t.numIncr++;
   t.norms.push_back(norm);
   if (norm <= d.test_tol) return t.currentIter;
-  if (t.currentIter >= d.test_maxit || t.numIncr >= d.test_maxit) {
+  if (t.currentIter >= d.test_maxit || (d.test_kind == XG_TEST_NORM_UNBALANCE && t.numIncr >= d.test_maxit)) {
     t.currentIter++;
     return -2;
   }

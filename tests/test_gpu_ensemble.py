@@ -1,0 +1,145 @@
+"""GPU parity of the batched frame ensemble: many independent 2D RC frames of
+uneven size advanced together by Newmark + Newton (one persistent launch per
+Newton iteration, one hipGraph per step) must follow the same trajectories as
+each frame analysed alone, on the CPU oracle, to 1e-9 relative."""
+import numpy as np
+import pytest
+
+from tests import _oracle
+from xara_amd import ground_motion, meshes
+from xara_amd.ensemble import Ensemble
+from xara_amd.model import Model, XaraError
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-9
+DT = 0.01
+
+# (stories, bays, seed, pga in g)
+CASES = [(3, 2, 0, 0.5), (1, 1, 1, 0.3), (5, 1, 2, 0.6), (2, 4, 3, 0.45), (4, 3, 4, 0.2)]
+
+
+def relerr(a, b):
+    s = np.max(np.abs(b))
+    return 0.0 if s == 0 else float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / s)
+
+
+def define(m, case, tol=1e-9, maxit=30):
+    nst, nbay, seed, pga = case
+    fr = meshes.rc_frame_2d(nst, nbay)
+    ag = ground_motion.kanai_tajimi(seed, 400, DT, pga)
+    meshes.define_transient_rc_frame_2d(m, fr, ag, DT, tol=tol, maxit=maxit)
+    return fr
+
+
+def step_ok(m):
+    """One Newmark step; False when the convergence test failed."""
+    try:
+        return m.analyze(1, DT) == 0
+    except XaraError as e:
+        if e.code != -6:
+            raise
+        return False
+
+
+def make_ensemble(cases, **kw):
+    ens = Ensemble()
+    models = []
+    for c in cases:
+        g = Model(2, 3)
+        define(g, c, **kw)
+        ens.add(g)
+        models.append(g)
+    ens.setup()
+    return ens, models
+
+
+def oracles(cases, **kw):
+    out = []
+    for c in cases:
+        o = _oracle.oracle_model(2, 3)
+        define(o, c, **kw)
+        o.setup()
+        out.append(o)
+    return out
+
+
+def test_ensemble_matches_each_frame_alone():
+    print("case: ensemble of 5 uneven frames, 150 steps", flush=True)
+    ens, _ = make_ensemble(CASES)
+    orc = oracles(CASES)
+    for step in range(150):
+        assert ens.analyze(1, DT) == 0, step
+        for k, o in enumerate(orc):
+            assert o.analyze(1, DT) == 0, (step, k)
+            assert relerr(ens.disp(k), o.trial_disp()) <= RTOL, (step, k)
+    st, it = ens.status()
+    assert (st == 1).all()
+    assert (it >= 150).all()
+    # the records drove the frames past first yield
+    fr = meshes.rc_frame_2d(*CASES[0][:2])
+    nsec = 5 * (len(fr["columns"]) + len(fr["beams"]))
+    ks = orc[0].point_data("tangent", nsec, 4)
+    assert ks[:, 0].min() < 0.95 * ks[:, 0].max()
+
+
+def test_ensemble_mappings_agree_bitwise():
+    """Block size, grid size, model order and graph capture change only the
+    schedule: every model's trajectory is computed by one workgroup in the
+    same order of operations, so results are bit-identical."""
+    print("case: ensemble mappings", flush=True)
+    results = []
+    for block, grid, order, graph in [(256, 0, 0, True), (128, 2, 1, False), (512, 3, 0, True)]:
+        ens, _ = make_ensemble(CASES)
+        ens.set_mapping(block, grid, order, graph)
+        assert ens.analyze(60, DT) == 0
+        results.append([ens.disp(k) for k in range(len(CASES))])
+    for r in results[1:]:
+        for a, b in zip(results[0], r):
+            assert np.array_equal(a, b)
+
+
+def test_ensemble_autotune_keeps_state():
+    """The autotuner runs trial iterations over the live ensemble; afterwards
+    the analysis must continue exactly as if it had not run."""
+    print("case: ensemble autotune", flush=True)
+    ens, _ = make_ensemble(CASES)
+    ref, _ = make_ensemble(CASES)
+    assert ens.analyze(20, DT) == 0 and ref.analyze(20, DT) == 0
+    table = ens.tune()
+    assert table.shape[1] == 4 and len(table) >= 10
+    assert (table[:, 3] > 0).all()
+    assert ens.analyze(40, DT) == 0 and ref.analyze(40, DT) == 0
+    for k in range(len(CASES)):
+        assert relerr(ens.disp(k), ref.disp(k)) <= RTOL
+
+
+def test_ensemble_single_model():
+    print("case: ensemble of one frame", flush=True)
+    case = (4, 2, 7, 0.55)
+    ens, _ = make_ensemble([case])
+    (o,) = oracles([case])
+    for step in range(100):
+        assert ens.analyze(1, DT) == 0
+        assert o.analyze(1, DT) == 0
+        assert relerr(ens.disp(0), o.trial_disp()) <= RTOL, step
+
+
+def test_ensemble_failed_models_stop():
+    """With a two-iteration Newton limit the frames fail once they yield: a
+    failed frame reverts to its last committed state and stops (analyze
+    returns < 0 in the reference); the others go on."""
+    print("case: ensemble failure handling", flush=True)
+    kw = dict(tol=1e-9, maxit=2)
+    ens, _ = make_ensemble(CASES, **kw)
+    orc = oracles(CASES, **kw)
+    alive = [True] * len(CASES)
+    for step in range(150):
+        ens.analyze(1, DT)
+        for k, o in enumerate(orc):
+            if alive[k] and not step_ok(o):
+                alive[k] = False
+        st, _ = ens.status()
+        assert list(st > -2) == alive, step
+        for k, o in enumerate(orc):
+            assert relerr(ens.disp(k), o.trial_disp()) <= RTOL, (step, k)
+    assert not all(alive), "no frame failed: the case does not exercise the failure path"

@@ -86,7 +86,19 @@ SIGNATURES = {
     "element_resisting_force": (C.c_int, [VP, C.c_int, P_D, P_I]),
     "get_counters": (C.c_int, [VP, C.POINTER(Counters)]),
     "time_op": (C.c_int, [VP, C.c_int, C.c_int, P_D]),
+    # batched ensemble (GPU library only)
+    "ensemble_create": (C.c_int, [C.c_int, C.POINTER(VP)]),
+    "ensemble_destroy": (C.c_int, [VP]),
+    "ensemble_add": (C.c_int, [VP, VP]),
+    "ensemble_setup": (C.c_int, [VP]),
+    "ensemble_num_models": (C.c_int, [VP]),
+    "ensemble_tune": (C.c_int, [VP, P_D, C.c_int, P_I]),
+    "ensemble_set_mapping": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ensemble_analyze": (C.c_int, [VP, C.c_int, C.c_double, P_I]),
+    "ensemble_model_disp": (C.c_int, [VP, C.c_int, P_D, P_I]),
+    "ensemble_status": (C.c_int, [VP, P_I, C.POINTER(C.c_longlong)]),
 }
+GPU_ONLY = {k for k in SIGNATURES if k.startswith("ensemble_")} | {"device_name"}
 
 
 class Binding:
@@ -99,7 +111,7 @@ class Binding:
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, prefix + name, None)
             if fn is None:
-                if prefix == "orc_" and name == "device_name":
+                if prefix != "xg_" and name in GPU_ONLY:
                     continue
                 raise RuntimeError(f"{path or lib}: missing symbol {prefix + name}")
             fn.restype = res

@@ -6,6 +6,7 @@
 
 #include "xg_domain.h"
 #include "xg_kernels.h"
+#include "xg_ensemble_host.h"
 
 struct xg_domain {
   xg::Domain d;
@@ -647,6 +648,119 @@ int xg_time_op(xg_domain* h, int op, int reps, double* ms) {
       for (auto& g : d.groups) d.counters.material_points += g->fibers() * reps;
     }
     *ms = (double)t / reps;
+    return 0;
+  });
+}
+
+
+struct xg_ensemble {
+  xg::Ensemble e;
+};
+
+static xg::Ensemble& ENS(xg_ensemble* h) {
+  if (!h) throw xg::Error(XG_ERR_ARG, "null ensemble");
+  XG_HIP(hipSetDevice(h->e.device));
+  return h->e;
+}
+
+int xg_ensemble_create(int device, xg_ensemble** out) {
+  return guard([&] {
+    int count = 0;
+    XG_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) throw xg::Error(XG_ERR_DEVICE, "no such GPU");
+    XG_HIP(hipSetDevice(device));
+    xg_ensemble* h = new xg_ensemble();
+    h->e.device = device;
+    *out = h;
+    return 0;
+  });
+}
+
+int xg_ensemble_destroy(xg_ensemble* h) {
+  return guard([&] {
+    if (h) {
+      (void)hipSetDevice(h->e.device);
+      delete h;
+    }
+    return 0;
+  });
+}
+
+int xg_ensemble_add(xg_ensemble* h, xg_domain* model) {
+  return guard([&] {
+    ENS(h).add(D(model));
+    return 0;
+  });
+}
+
+int xg_ensemble_setup(xg_ensemble* h) {
+  return guard([&] {
+    ENS(h).setup();
+    return 0;
+  });
+}
+
+int xg_ensemble_num_models(xg_ensemble* h) { return h ? (int)h->e.models.size() : 0; }
+
+int xg_ensemble_tune(xg_ensemble* h, double* table, int cap, int* nrows) {
+  return guard([&] {
+    auto res = ENS(h).tune();
+    int n = 0;
+    for (auto& r : res) {
+      if (n >= cap) break;
+      table[4 * n + 0] = r.block;
+      table[4 * n + 1] = r.grid;
+      table[4 * n + 2] = r.order;
+      table[4 * n + 3] = r.ms;
+      n++;
+    }
+    if (nrows) *nrows = n;
+    return 0;
+  });
+}
+
+int xg_ensemble_set_mapping(xg_ensemble* h, int block, int grid, int order, int use_graph) {
+  return guard([&] {
+    auto& e = ENS(h);
+    if (block != 128 && block != 256 && block != 512) throw xg::Error(XG_ERR_ARG, "block must be 128, 256 or 512");
+    if (!e.ready) e.setup();
+    e.block = block;
+    e.grid = grid > 0 ? grid : (int)e.models.size();
+    e.use_graph = use_graph != 0;
+    e.set_order(order);
+    return 0;
+  });
+}
+
+int xg_ensemble_analyze(xg_ensemble* h, int nsteps, double dt, int* nstopped) {
+  return guard([&] {
+    int s = ENS(h).analyze(nsteps, dt);
+    if (nstopped) *nstopped = s;
+    return 0;
+  });
+}
+
+int xg_ensemble_model_disp(xg_ensemble* h, int model, double* U, int* nnode) {
+  return guard([&] {
+    auto& e = ENS(h);
+    if (!e.ready) throw xg::Error(XG_ERR_STATE, "ensemble not set up");
+    if (model < 0 || model >= (int)e.models.size()) throw xg::Error(XG_ERR_ARG, "model index");
+    const int n0 = e.node_off_h[model], n1 = e.node_off_h[model + 1];
+    XG_HIP(hipMemcpyAsync(U, e.U.p + (size_t)n0 * 3, (size_t)(n1 - n0) * 3 * sizeof(double),
+                          hipMemcpyDeviceToHost, e.merged.stream));
+    XG_HIP(hipStreamSynchronize(e.merged.stream));
+    if (nnode) *nnode = n1 - n0;
+    return 0;
+  });
+}
+
+int xg_ensemble_status(xg_ensemble* h, int* status, long long* newton_iters) {
+  return guard([&] {
+    auto& e = ENS(h);
+    if (!e.ready) throw xg::Error(XG_ERR_STATE, "ensemble not set up");
+    const size_t n = e.models.size();
+    if (status) e.status.download(status, n, e.merged.stream);
+    if (newton_iters) e.newton_count.download(newton_iters, n, e.merged.stream);
     return 0;
   });
 }

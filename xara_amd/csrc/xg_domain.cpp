@@ -547,7 +547,8 @@ int Domain::run_test(int& iter, std::vector<double>& norms, int& numIncr) {
   if (iter > 1 && !norms.empty() && norms.back() < norm) numIncr++;
   norms.push_back(norm);
   if (norm <= test_tol) return iter;
-  if (iter >= test_maxit || numIncr >= test_maxit) {
+  // numIncr (norm increases) only fails CTestNormUnbalance (CTestNormUnbalance.cpp:172)
+  if (iter >= test_maxit || (test_kind == XG_TEST_NORM_UNBALANCE && numIncr >= test_maxit)) {
     iter++;
     return -2;
   }

@@ -1,0 +1,266 @@
+// Batched ensemble of 2D frame models on gfx950.
+//
+// Three persistent kernels, each one launch over the whole ensemble; every
+// workgroup walks the models assigned to it (m = order[k], k = blockIdx.x,
+// blockIdx.x + gridDim.x, ...), so a model never spans workgroups and needs
+// no grid-wide synchronisation:
+//   k_ens_predict  -- Newmark::newStep (Newmark.cpp:105-170): predictor,
+//                     time advance, ground acceleration (PathSeries), domain
+//                     update, transient unbalance; starts the Newton test.
+//   k_ens_iterate  -- one NewtonRaphson iteration (NewtonRaphson.cpp:160-215)
+//                     for every model still iterating: element tangents,
+//                     banded assembly of c1 K + c2 C + c3 M in element order,
+//                     banded Cholesky, solve, Newmark::update, fiber/section
+//                     update, unbalance, CTestNormDispIncr / NormUnbalance.
+//   k_ens_commit   -- Domain::commit of converged models; revert of models
+//                     whose Newton failed (they stop, as analyze returns < 0).
+#include "xg_ensemble.h"
+#include "xg_frame_impl.h"
+
+namespace xg {
+
+template <int NT>
+__device__ __forceinline__ double block_sum_t(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double path_accel(const EnsDev& E, int m, double t) {
+  const int n = E.gm_n[m];
+  const double h = E.gm_dt[m];
+  if (n <= 0 || !(h > 0.0) || t < 0.0) return 0.0;
+  const double incr = t / h;
+  const int i1 = (int)floor(incr);
+  const int i2 = i1 + 1;
+  if (i2 >= n) return 0.0;
+  const double* g = E.gm + E.gm_off[m];
+  const double v1 = g[i1], v2 = g[i2];
+  return E.gm_fac[m] * (v1 + (v2 - v1) * (incr - i1));
+}
+
+// domain update + transient unbalance of model m (all threads of the block)
+template <int NT>
+__device__ void model_update_and_unbalance(const EnsDev& E, int m, double ag) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long long s0 = (long long)E.ele_off[m] * E.f.nip, s1 = (long long)E.ele_off[m + 1] * E.f.nip;
+  for (long long sec = s0 + wave; sec < s1; sec += NT / 64) section_update<false>(E.f, sec, lane, E.U);
+  __syncthreads();
+  for (int e = E.ele_off[m] + tid; e < E.ele_off[m + 1]; e += NT) element_resid<false>(E.f, e, E.Re);
+  __syncthreads();
+  const double alphaM = E.alphaM[m];
+  for (int a = E.eq_off[m] + tid; a < E.eq_off[m + 1]; a += NT) {
+    double acc = 0.0;
+    for (int j = E.rhs_ptr[a]; j < E.rhs_ptr[a + 1]; j++) acc += -E.Re[E.rhs_idx[j]];
+    double u = 0.0;
+    const double mm = E.M[a];
+    if (mm != 0.0) {
+      const int d = E.eq2dof[a];
+      u += -(mm * E.R[a]) * ag;
+      u += -mm * E.A[d];
+      if (alphaM != 0.0) u += -alphaM * mm * E.V[d];
+    }
+    E.B[a] = acc + u;
+  }
+  __syncthreads();
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_ens_predict(EnsDev E) {
+  for (int k = blockIdx.x; k < E.nmodel; k += gridDim.x) {
+    const int m = E.order[k];
+    if (E.status[m] <= -2) continue;  // stopped after a failed step
+    const int tid = threadIdx.x;
+    const double g = E.gamma, b = E.beta, h = E.dt;
+    const double a1 = (1.0 - g / b), a2 = h * (1.0 - 0.5 * g / b), a3 = -1.0 / (b * h), a4 = 1.0 - 0.5 / b;
+    const int d0 = E.node_off[m] * E.ndf, d1 = E.node_off[m + 1] * E.ndf;
+    for (int d = d0 + tid; d < d1; d += NT) {
+      if (E.eqmap[d] < 0) continue;
+      E.U[d] = E.Uc[d];
+      E.V[d] = E.Vc[d] * a1 + E.Ac[d] * a2;
+      E.A[d] = E.Ac[d] * a4 + E.Vc[d] * a3;
+    }
+    // E.time holds the committed time; the trial time is E.time + dt
+    // (Newmark.cpp newStep: time += deltaT), formed the same way everywhere
+    const double ag = path_accel(E, m, E.time[m] + h);
+    __syncthreads();
+    model_update_and_unbalance<NT>(E, m, ag);
+    if (tid == 0) {
+      E.iter[m] = 0;
+      E.status[m] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_ens_iterate(EnsDev E) {
+  __shared__ double sh[NT];
+  __shared__ int s_bad;
+  for (int k = blockIdx.x; k < E.nmodel; k += gridDim.x) {
+    const int m = E.order[k];
+    if (E.status[m] != 0) continue;
+    const int tid = threadIdx.x;
+    const int eq0 = E.eq_off[m], n = E.eq_off[m + 1] - eq0;
+    const int bw = E.bw[m], ld = bw + 1;
+    double* L = E.band + E.band_off[m];
+    const double g = E.gamma, b = E.beta, h = E.dt;
+    const double c2 = g / (b * h), c3 = 1.0 / (b * h * h);
+    // ---- tangent: element K, then c1 K + c2 C + c3 M into the band ----
+    for (int e = E.ele_off[m] + tid; e < E.ele_off[m + 1]; e += NT) element_tangent<false>(E.f, e, E.Ke);
+    for (long long i = tid; i < (long long)n * ld; i += NT) L[i] = 0.0;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    for (int q = E.bent_off[m] + tid; q < E.bent_off[m + 1]; q += NT) {
+      double acc = 0.0;
+      for (int j = E.bent_ptr[q]; j < E.bent_ptr[q + 1]; j++) acc += E.Ke[E.bent_idx[j]];
+      E.band[E.bent_pos[q]] = acc;
+    }
+    __syncthreads();
+    const double alphaM = E.alphaM[m];
+    for (int a = tid; a < n; a += NT) {
+      const double mm = E.M[eq0 + a];
+      if (mm != 0.0) L[(long long)a * ld + bw] += (c3 + c2 * alphaM) * mm;
+    }
+    __syncthreads();
+    // ---- banded Cholesky, left-looking by columns ----
+    for (int j = 0; j < n; j++) {
+      if (tid == 0) {
+        double s = L[(long long)j * ld + bw];
+        const int k0 = j - bw > 0 ? j - bw : 0;
+        for (int kk = k0; kk < j; kk++) {
+          const double v = L[(long long)j * ld + (kk - j + bw)];
+          s -= v * v;
+        }
+        if (!(s > 0.0)) s_bad = 1;
+        L[(long long)j * ld + bw] = sqrt(s > 0.0 ? s : 1.0);
+      }
+      __syncthreads();
+      const double djj = L[(long long)j * ld + bw];
+      for (int r = tid; r < bw; r += NT) {
+        const int i = j + 1 + r;
+        if (i >= n) break;
+        double s = L[(long long)i * ld + (j - i + bw)];
+        const int k0 = i - bw > 0 ? i - bw : 0;
+        for (int kk = k0; kk < j; kk++)
+          s -= L[(long long)i * ld + (kk - i + bw)] * L[(long long)j * ld + (kk - j + bw)];
+        L[(long long)i * ld + (j - i + bw)] = s / djj;
+      }
+      __syncthreads();
+    }
+    // ---- solve L L^T x = B ----
+    double* x = E.X + eq0;
+    const double* rhs = E.B + eq0;
+    if (tid == 0) {
+      for (int i = 0; i < n; i++) {
+        double s = rhs[i];
+        const int k0 = i - bw > 0 ? i - bw : 0;
+        for (int kk = k0; kk < i; kk++) s -= L[(long long)i * ld + (kk - i + bw)] * x[kk];
+        x[i] = s / L[(long long)i * ld + bw];
+      }
+      for (int i = n - 1; i >= 0; i--) {
+        double s = x[i];
+        const int k1 = i + bw < n - 1 ? i + bw : n - 1;
+        for (int kk = i + 1; kk <= k1; kk++) s -= L[(long long)kk * ld + (i - kk + bw)] * x[kk];
+        x[i] = s / L[(long long)i * ld + bw];
+      }
+    }
+    __syncthreads();
+    if (s_bad) {
+      if (tid == 0) E.status[m] = -1;  // BadLinearSolve
+      __syncthreads();
+      continue;
+    }
+    // ---- Newmark::update ----
+    const int d0 = E.node_off[m] * E.ndf, d1 = E.node_off[m + 1] * E.ndf;
+    for (int d = d0 + tid; d < d1; d += NT) {
+      const int a = E.eqmap[d];
+      if (a < 0) continue;
+      const double du = E.X[a];
+      E.U[d] += du;
+      E.V[d] += c2 * du;
+      E.A[d] += c3 * du;
+    }
+    __syncthreads();
+    const double ag = path_accel(E, m, E.time[m] + h);
+    model_update_and_unbalance<NT>(E, m, ag);
+    // ---- convergence test ----
+    double v = 0.0;
+    const double* w = E.test_kind == 1 ? E.B : E.X;
+    for (int a = eq0 + tid; a < eq0 + n; a += NT) v += w[a] * w[a];
+    const double norm = sqrt(block_sum_t<NT>(v, sh));
+    if (tid == 0) {
+      const int it = E.iter[m] + 1;
+      E.iter[m] = it;
+      E.norm[m] = norm;
+      E.newton_count[m] += 1;
+      if (norm <= E.tol) E.status[m] = 1;
+      else if (it >= E.maxit) E.status[m] = -1;
+    }
+    __syncthreads();
+  }
+}
+
+// commit converged models; revert (and stop) models whose step failed
+template <int NT>
+__global__ __launch_bounds__(NT) void k_ens_commit(EnsDev E) {
+  for (int k = blockIdx.x; k < E.nmodel; k += gridDim.x) {
+    const int m = E.order[k];
+    const int st = E.status[m];
+    if (st <= -2) continue;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int d0 = E.node_off[m] * E.ndf, d1 = E.node_off[m + 1] * E.ndf;
+    const long long s0 = (long long)E.ele_off[m] * E.f.nip, s1 = (long long)E.ele_off[m + 1] * E.f.nip;
+    if (st == 1) {
+      if (tid == 0) E.time[m] += E.dt;
+      for (int d = d0 + tid; d < d1; d += NT) {
+        E.Uc[d] = E.U[d];
+        E.Vc[d] = E.V[d];
+        E.Ac[d] = E.A[d];
+      }
+      const FrameDev& f = E.f;
+      const long long sb0 = f.steel_base[s0], sb1 = s1 < f.nsec ? f.steel_base[s1] : f.nsteel;
+      const long long cb0 = f.conc_base[s0], cb1 = s1 < f.nsec ? f.conc_base[s1] : f.nconc;
+      const long long eb0 = f.elas_base[s0], eb1 = s1 < f.nsec ? f.elas_base[s1] : f.nelas;
+      for (int c = 0; c < kSteelHist; c++)
+        for (long long i = sb0 + tid; i < sb1; i += NT) f.st_c[c * f.nsteel + i] = f.st_t[c * f.nsteel + i];
+      for (long long i = sb0 + tid; i < sb1; i += NT) f.kon_c[i] = f.kon_t[i];
+      for (int c = 0; c < kConcHist; c++)
+        for (long long i = cb0 + tid; i < cb1; i += NT) f.cs_c[c * f.nconc + i] = f.cs_t[c * f.nconc + i];
+      for (long long i = eb0 + tid; i < eb1; i += NT) f.el_c[i] = f.el_t[i];
+    } else {
+      // failed (or never converged within the launched iterations):
+      // Domain::revertToLastCommit and stop this model
+      for (int d = d0 + tid; d < d1; d += NT) {
+        E.U[d] = E.Uc[d];
+        E.V[d] = E.Vc[d];
+        E.A[d] = E.Ac[d];
+      }
+      for (long long sec = s0 + wave; sec < s1; sec += NT / 64) section_revert<false>(E.f, sec, lane);
+      __syncthreads();
+      if (tid == 0) E.status[m] = -2;
+    }
+    __syncthreads();
+  }
+}
+
+#define XG_ENS_DISPATCH(KERNEL, ...)                                                                    \
+  switch (block) {                                                                                     \
+    case 128: hipLaunchKernelGGL(KERNEL<128>, dim3(grid), dim3(128), 0, s, __VA_ARGS__); break;         \
+    case 512: hipLaunchKernelGGL(KERNEL<512>, dim3(grid), dim3(512), 0, s, __VA_ARGS__); break;         \
+    default: hipLaunchKernelGGL(KERNEL<256>, dim3(grid), dim3(256), 0, s, __VA_ARGS__); break;          \
+  }
+
+void launch_ens_predict(hipStream_t s, const EnsDev& E, int grid, int block) { XG_ENS_DISPATCH(k_ens_predict, E); }
+void launch_ens_iterate(hipStream_t s, const EnsDev& E, int grid, int block) { XG_ENS_DISPATCH(k_ens_iterate, E); }
+void launch_ens_commit(hipStream_t s, const EnsDev& E, int grid, int block) { XG_ENS_DISPATCH(k_ens_commit, E); }
+
+}  // namespace xg

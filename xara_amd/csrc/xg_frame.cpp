@@ -312,6 +312,8 @@ struct FrameGroup : Group {
   }
 };
 
+FrameDev frame_dev(Group& g) { return static_cast<FrameGroup&>(g).f; }
+
 std::unique_ptr<Group> make_frame_group(Domain& d) {
   auto g = std::make_unique<FrameGroup>();
   g->d3 = d.ndm == 3;

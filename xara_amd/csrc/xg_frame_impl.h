@@ -1,0 +1,569 @@
+// Device bodies of the frame path, shared by the per-operation kernels
+// (xg_frame.hip) and the batched ensemble (xg_ensemble.hip).
+//
+// One 64-lane wavefront per section (integration point): the lanes evaluate
+// the element's basic deformations (LinearCrdTransf2d/3d::getBasicTrialDisp,
+// LinearCrdTransf2d.cpp:248-298, LinearCrdTransf3d.cpp:345-420), the section
+// deformation at xi (DispBeamColumn2d.cpp:235-284, DispBeamColumn3d.cpp:
+// 235-292), update their fibers (Steel02 / Concrete02 / Elastic, fibers
+// f = lane, lane+64, ...) and reduce the section resultant and tangent with
+// wavefront shuffles (FiberSection2d.cpp:225-262, FiberSection3d.cpp:423-476).
+#pragma once
+#include "xg_frame_dev.h"
+#include "xg_uniaxial.h"
+
+namespace xg {
+
+// GaussLegendre<1,5> on [0,1] (GaussLegendre1D.hpp:74-92, FrameQuadrature.hpp)
+__device__ __forceinline__ double gl5_pt(int i) {
+  switch (i) {
+    case 0: return -0.90617984593866396;
+    case 1: return -0.53846931010568311;
+    case 2: return 0.0;
+    case 3: return 0.53846931010568311;
+    default: return 0.90617984593866396;
+  }
+}
+__device__ __forceinline__ double gl5_wt(int i) {
+  switch (i) {
+    case 0: return 0.23692688505618942;
+    case 1: return 0.47862867049936619;
+    case 2: return 0.568888888888889;
+    case 3: return 0.47862867049936619;
+    default: return 0.23692688505618942;
+  }
+}
+
+template <bool D3>
+__device__ __forceinline__ void basic_disp(const FrameDev& f, int e, const double* __restrict__ U, double v[6]) {
+  const int ndf = D3 ? 6 : 3;
+  const int ni = f.conn[2 * e], nj = f.conn[2 * e + 1];
+  const double* g = f.geo + (long long)e * 10;
+  const double L = g[0];
+  const double oneOverL = 1.0 / L;
+  double ug[12];
+#pragma unroll
+  for (int k = 0; k < ndf; k++) {
+    ug[k] = U[(long long)ndf * ni + k];
+    ug[ndf + k] = U[(long long)ndf * nj + k];
+  }
+  if (!D3) {
+    const double c = g[1], s = g[2];
+    const double sl = s * oneOverL, cl = c * oneOverL;
+    v[0] = -c * ug[0] - s * ug[1] + c * ug[3] + s * ug[4];
+    v[1] = -sl * ug[0] + cl * ug[1] + ug[2] + sl * ug[3] - cl * ug[4];
+    v[2] = v[1] + ug[5] - ug[2];
+  } else {
+    const double* R = g + 1;  // R[r*3 + c]
+    double ul[12];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        ul[3 * b + r] = R[3 * r + 0] * ug[3 * b] + R[3 * r + 1] * ug[3 * b + 1] + R[3 * r + 2] * ug[3 * b + 2];
+    v[0] = ul[6] - ul[0];
+    double tmp = oneOverL * (ul[1] - ul[7]);
+    v[1] = ul[5] + tmp;
+    v[2] = ul[11] + tmp;
+    tmp = oneOverL * (ul[8] - ul[2]);
+    v[3] = ul[4] + tmp;
+    v[4] = ul[10] + tmp;
+    v[5] = ul[9] - ul[3];
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+  return x;
+}
+
+// One fiber: committed state -> trial state; returns stress and tangent.
+__device__ __forceinline__ void fiber_trial(const FrameDev& f, int kind, int mat, long long idx, double strain,
+                                            double& stress, double& tangent) {
+  if (kind == 1) {
+    const Steel02Param p = f.steelp[mat];
+    const long long n = f.nsteel;
+    Steel02State c, t;
+    c.epsmin = f.st_c[0 * n + idx];
+    c.epsmax = f.st_c[1 * n + idx];
+    c.epspl = f.st_c[2 * n + idx];
+    c.epss0 = f.st_c[3 * n + idx];
+    c.sigs0 = f.st_c[4 * n + idx];
+    c.epsr = f.st_c[5 * n + idx];
+    c.sigr = f.st_c[6 * n + idx];
+    c.eps = f.st_c[7 * n + idx];
+    c.sig = f.st_c[8 * n + idx];
+    c.e = f.st_c[9 * n + idx];
+    c.kon = f.kon_c[idx];
+    steel02_trial(p, c, strain, t);
+    f.st_t[0 * n + idx] = t.epsmin;
+    f.st_t[1 * n + idx] = t.epsmax;
+    f.st_t[2 * n + idx] = t.epspl;
+    f.st_t[3 * n + idx] = t.epss0;
+    f.st_t[4 * n + idx] = t.sigs0;
+    f.st_t[5 * n + idx] = t.epsr;
+    f.st_t[6 * n + idx] = t.sigr;
+    f.st_t[7 * n + idx] = t.eps;
+    f.st_t[8 * n + idx] = t.sig;
+    f.st_t[9 * n + idx] = t.e;
+    f.kon_t[idx] = t.kon;
+    stress = t.sig;
+    tangent = t.e;
+  } else if (kind == 2) {
+    const Concrete02Param p = f.concp[mat];
+    const long long n = f.nconc;
+    double ecmin, dept, eps, sig = f.cs_t[3 * n + idx], e = f.cs_t[4 * n + idx];
+    concrete02_trial(p, f.cs_c[0 * n + idx], f.cs_c[1 * n + idx], f.cs_c[2 * n + idx], f.cs_c[3 * n + idx],
+                     strain, ecmin, dept, eps, sig, e);
+    f.cs_t[0 * n + idx] = ecmin;
+    f.cs_t[1 * n + idx] = dept;
+    f.cs_t[2 * n + idx] = eps;
+    f.cs_t[3 * n + idx] = sig;
+    f.cs_t[4 * n + idx] = e;
+    stress = sig;
+    tangent = e;
+  } else {
+    const double E = f.elasE[mat];
+    f.el_t[idx] = strain;
+    stress = E * strain;
+    tangent = E;
+  }
+}
+
+// Section update, one wavefront per section instance.
+template <bool D3>
+__device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
+  const int e = (int)(sec / f.nip), ip = (int)(sec % f.nip);
+  double v[6];
+  basic_disp<D3>(f, e, U, v);
+  const double L = f.geo[(long long)e * 10];
+  const double oneOverL = 1.0 / L;
+  const double xi = 0.5 * (1.0 + gl5_pt(ip));
+  const double xi6 = 6.0 * xi;
+  double d[4];
+  d[0] = oneOverL * v[0];
+  d[1] = oneOverL * ((xi6 - 4.0) * v[1] + (xi6 - 2.0) * v[2]);
+  if (D3) {
+    d[2] = oneOverL * ((xi6 - 4.0) * v[3] + (xi6 - 2.0) * v[4]);
+    d[3] = oneOverL * v[5];
+  }
+  const int def = f.secdef[e];
+  const int f0 = f.def_off[def], nf = f.def_off[def + 1] - f0;
+  const long long sb = f.steel_base[sec], cb = f.conc_base[sec], eb = f.elas_base[sec];
+  double k0 = 0, k1 = 0, k2 = 0, k5 = 0, k6 = 0, k10 = 0, s0 = 0, s1 = 0, s2 = 0;
+  for (int i = lane; i < nf; i += 64) {
+    const int fi = f0 + i;
+    const double y = f.fy[fi], A = f.fA[fi];
+    const int kind = f.fkind[fi];
+    const long long base = kind == 1 ? sb : (kind == 2 ? cb : eb);
+    double strain;
+    if (D3) {
+      const double z = f.fz[fi];
+      strain = d[0] - y * d[1] + z * d[2];
+    } else
+      strain = d[0] - y * d[1];
+    double stress, tangent;
+    fiber_trial(f, kind, f.fmat[fi], base + f.flocal[fi], strain, stress, tangent);
+    if (D3) {
+      const double z = f.fz[fi];
+      const double EA = tangent * A;
+      k0 += EA;
+      k1 += -y * EA;
+      k2 += z * EA;
+      k5 += y * y * EA;
+      k10 += z * z * EA;
+      k6 += -y * z * EA;
+      const double fs0 = stress * A;
+      s0 += fs0;
+      s1 += -y * fs0;
+      s2 += z * fs0;
+    } else {
+      const double ks0 = tangent * A;
+      const double ks1 = ks0 * -y;
+      k0 += ks0;
+      k1 += ks1;
+      k5 += ks1 * -y;
+      const double fs0 = stress * A;
+      s0 += fs0;
+      s1 += fs0 * -y;
+    }
+  }
+  k0 = wave_sum(k0);
+  k1 = wave_sum(k1);
+  k5 = wave_sum(k5);
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (D3) {
+    k2 = wave_sum(k2);
+    k6 = wave_sum(k6);
+    k10 = wave_sum(k10);
+    s2 = wave_sum(s2);
+  }
+  if (lane == 0) {
+    const long long n = f.nsec;
+    if (D3) {
+      double kd[16] = {k0, k1, k2, 0, k1, k5, k6, 0, k2, k6, k10, 0, 0, 0, 0, 0};
+      double sd[4] = {s0, s1, s2, 0.0};
+      const double GJ = f.GJ[def];
+      if (GJ >= 0.0) {
+        sd[3] = GJ * d[3];
+        kd[15] = GJ;
+      }
+#pragma unroll
+      for (int c = 0; c < 16; c++) f.ks[c * n + sec] = kd[c];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        f.s[c * n + sec] = sd[c];
+        f.e[c * n + sec] = d[c];
+      }
+    } else {
+      f.ks[0 * n + sec] = k0;
+      f.ks[1 * n + sec] = k1;
+      f.ks[2 * n + sec] = k1;
+      f.ks[3 * n + sec] = k5;
+      f.s[0 * n + sec] = s0;
+      f.s[1 * n + sec] = s1;
+      f.e[0 * n + sec] = d[0];
+      f.e[1 * n + sec] = d[1];
+    }
+  }
+}
+
+// revertToLastCommit of every fiber followed by the section recomputation of
+// FiberSection2d::revertToLastCommit (FiberSection2d.cpp:376-409: the
+// resultant keeps only the last fiber's contribution, `sData[0] = fs0`) /
+// FiberSection3d::revertToLastCommit (FiberSection3d.cpp:613-665: kData[10]
+// is not zeroed first, torsion stress stays zero).
+template <bool D3>
+__device__ __forceinline__ void section_revert(const FrameDev& f, long long sec, int lane) {
+  const int e = (int)(sec / f.nip);
+  const int def = f.secdef[e];
+  const int f0 = f.def_off[def], nf = f.def_off[def + 1] - f0;
+  const long long sb = f.steel_base[sec], cb = f.conc_base[sec], eb = f.elas_base[sec];
+  double k0 = 0, k1 = 0, k2 = 0, k5 = 0, k6 = 0, k10 = 0, s0 = 0, s1 = 0, s2 = 0;
+  double last_s0 = 0, last_s1 = 0;
+  for (int i = lane; i < nf; i += 64) {
+    const int fi = f0 + i;
+    const int kind = f.fkind[fi];
+    const long long idx = (kind == 1 ? sb : (kind == 2 ? cb : eb)) + f.flocal[fi];
+    double stress, tangent;
+    if (kind == 1) {
+      const long long n = f.nsteel;
+#pragma unroll
+      for (int c = 0; c < kSteelHist; c++) f.st_t[c * n + idx] = f.st_c[c * n + idx];
+      f.kon_t[idx] = f.kon_c[idx];
+      stress = f.st_c[8 * n + idx];
+      tangent = f.st_c[9 * n + idx];
+    } else if (kind == 2) {
+      const long long n = f.nconc;
+#pragma unroll
+      for (int c = 0; c < kConcHist; c++) f.cs_t[c * n + idx] = f.cs_c[c * n + idx];
+      stress = f.cs_c[3 * n + idx];
+      tangent = f.cs_c[4 * n + idx];
+    } else {
+      const double E = f.elasE[f.fmat[fi]];
+      f.el_t[idx] = f.el_c[idx];
+      stress = E * f.el_c[idx];
+      tangent = E;
+    }
+    const double y = f.fy[fi], A = f.fA[fi];
+    if (D3) {
+      const double z = f.fz[fi];
+      const double value = tangent * A;
+      const double vas1 = -y * value;
+      const double vas2 = z * value;
+      const double vas1as2 = vas1 * z;
+      k0 += value;
+      k1 += vas1;
+      k2 += vas2;
+      k5 += vas1 * -y;
+      k6 += vas1as2;
+      k10 += vas2 * z;
+      const double fs0 = stress * A;
+      s0 += fs0;
+      s1 += fs0 * -y;
+      s2 += fs0 * z;
+    } else {
+      const double ks0 = tangent * A;
+      const double ks1 = ks0 * -y;
+      k0 += ks0;
+      k1 += ks1;
+      k5 += ks1 * -y;
+      if (i == nf - 1) {
+        last_s0 = stress * A;
+        last_s1 = last_s0 * -y;
+      }
+    }
+  }
+  k0 = wave_sum(k0);
+  k1 = wave_sum(k1);
+  k5 = wave_sum(k5);
+  if (D3) {
+    k2 = wave_sum(k2);
+    k6 = wave_sum(k6);
+    k10 = wave_sum(k10);
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+  } else {
+    s0 = wave_sum(last_s0);  // only the lane owning fiber nf-1 is non-zero
+    s1 = wave_sum(last_s1);
+  }
+  if (lane == 0) {
+    const long long n = f.nsec;
+    if (D3) {
+      const double k10_prev = f.ks[10 * n + sec];
+      double kd[16] = {k0, k1, k2, 0, k1, k5, k6, 0, k2, k6, k10_prev + k10, 0, 0, 0, 0, 0};
+      // kData[11..14] keep their previous values (zero on this path)
+      for (int c = 11; c < 15; c++) kd[c] = f.ks[c * n + sec];
+      kd[3] = 0.0;
+      kd[7] = 0.0;
+      const double GJ = f.GJ[def];
+      kd[15] = GJ >= 0.0 ? GJ : 0.0;
+#pragma unroll
+      for (int c = 0; c < 16; c++) f.ks[c * n + sec] = kd[c];
+      f.s[0 * n + sec] = s0;
+      f.s[1 * n + sec] = s1;
+      f.s[2 * n + sec] = s2;
+      f.s[3 * n + sec] = 0.0;
+    } else {
+      f.ks[0 * n + sec] = k0;
+      f.ks[1 * n + sec] = k1;
+      f.ks[2 * n + sec] = k1;
+      f.ks[3 * n + sec] = k5;
+      f.s[0 * n + sec] = s0;
+      f.s[1 * n + sec] = s1;
+    }
+  }
+}
+
+// Element resisting force: q = sum_i B_i^T s_i wt_i, P = T^T q
+// (DispBeamColumn2d::getResistingForce :665-726, LinearCrdTransf2d::
+// getGlobalResistingForce :473-525; 3D :295-360 and :700).
+template <bool D3>
+__device__ __forceinline__ void element_resid(const FrameDev& f, int e, double* __restrict__ Re) {
+  const long long n = f.nsec;
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < f.nip; i++) {
+    const long long sec = (long long)e * f.nip + i;
+    const double xi6 = 6.0 * (0.5 * (1.0 + gl5_pt(i)));
+    const double wt = gl5_wt(i) * 0.5;
+    q[0] += f.s[0 * n + sec] * wt;
+    double si = f.s[1 * n + sec] * wt;
+    q[1] += (xi6 - 4.0) * si;
+    q[2] += (xi6 - 2.0) * si;
+    if (D3) {
+      si = f.s[2 * n + sec] * wt;
+      q[3] += (xi6 - 4.0) * si;
+      q[4] += (xi6 - 2.0) * si;
+      q[5] += f.s[3 * n + sec] * wt;
+    }
+  }
+  const double* g = f.geo + (long long)e * 10;
+  const double oneOverL = 1.0 / g[0];
+  if (!D3) {
+    const double c = g[1], s = g[2];
+    const double V = oneOverL * (q[1] + q[2]);
+    const double pl[6] = {-q[0], V, q[1], q[0], -V, q[2]};
+    double* P = Re + (long long)e * 6;
+    P[0] = c * pl[0] - s * pl[1];
+    P[1] = s * pl[0] + c * pl[1];
+    P[3] = c * pl[3] - s * pl[4];
+    P[4] = s * pl[3] + c * pl[4];
+    P[2] = pl[2];
+    P[5] = pl[5];
+  } else {
+    const double* R = g + 1;
+    double pl[12];
+    pl[0] = -q[0];
+    pl[1] = oneOverL * (q[1] + q[2]);
+    pl[2] = -oneOverL * (q[3] + q[4]);
+    pl[3] = -q[5];
+    pl[4] = q[3];
+    pl[5] = q[1];
+    pl[6] = q[0];
+    pl[7] = -pl[1];
+    pl[8] = -pl[2];
+    pl[9] = q[5];
+    pl[10] = q[4];
+    pl[11] = q[2];
+    double* P = Re + (long long)e * 12;
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        P[3 * b + c] = R[0 * 3 + c] * pl[3 * b] + R[1 * 3 + c] * pl[3 * b + 1] + R[2 * 3 + c] * pl[3 * b + 2];
+  }
+}
+
+// Element tangent: kb = sum_i B_i^T ks_i B_i wt_i / L (getBasicStiff, 2D
+// :287-355; 3D getTangentStiff :407-537), K = T^T kb T (LinearCrdTransf2d::
+// getGlobalStiffMatrix :610-733, LinearCrdTransf3d :768).  Row-major output.
+template <bool D3>
+__device__ __forceinline__ void element_tangent(const FrameDev& f, int e, double* __restrict__ Ke) {
+  constexpr int nb = D3 ? 6 : 3;
+  constexpr int ord = D3 ? 4 : 2;
+  constexpr int ld = D3 ? 4 : 2;
+  const long long n = f.nsec;
+  const double* g = f.geo + (long long)e * 10;
+  const double oneOverL = 1.0 / g[0];
+  double kb[6][6];
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+#pragma unroll
+    for (int j = 0; j < 6; j++) kb[i][j] = 0.0;
+  for (int i = 0; i < f.nip; i++) {
+    const long long sec = (long long)e * f.nip + i;
+    const double xi6 = 6.0 * (0.5 * (1.0 + gl5_pt(i)));
+    const double wti = (gl5_wt(i) * 0.5) * oneOverL;
+    double ks[ord][ord];
+#pragma unroll
+    for (int a = 0; a < ord; a++)
+#pragma unroll
+      for (int b = 0; b < ord; b++) ks[a][b] = f.ks[(long long)(b * ld + a) * n + sec];
+    double ka[ord][6];
+#pragma unroll
+    for (int a = 0; a < ord; a++)
+#pragma unroll
+      for (int b = 0; b < 6; b++) ka[a][b] = 0.0;
+#pragma unroll
+    for (int j = 0; j < ord; j++) {
+      if (j == 0) {
+#pragma unroll
+        for (int k = 0; k < ord; k++) ka[k][0] += ks[k][j] * wti;
+      } else if (j == 1) {
+#pragma unroll
+        for (int k = 0; k < ord; k++) {
+          const double tmp = ks[k][j] * wti;
+          ka[k][1] += (xi6 - 4.0) * tmp;
+          ka[k][2] += (xi6 - 2.0) * tmp;
+        }
+      } else if (j == 2) {
+#pragma unroll
+        for (int k = 0; k < ord; k++) {
+          const double tmp = ks[k][j] * wti;
+          ka[k][3] += (xi6 - 4.0) * tmp;
+          ka[k][4] += (xi6 - 2.0) * tmp;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < ord; k++) ka[k][5] += ks[k][j] * wti;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < ord; j++) {
+      if (j == 0) {
+#pragma unroll
+        for (int k = 0; k < nb; k++) kb[0][k] += ka[j][k];
+      } else if (j == 1) {
+#pragma unroll
+        for (int k = 0; k < nb; k++) {
+          const double tmp = ka[j][k];
+          kb[1][k] += (xi6 - 4.0) * tmp;
+          kb[2][k] += (xi6 - 2.0) * tmp;
+        }
+      } else if (j == 2) {
+#pragma unroll
+        for (int k = 0; k < nb; k++) {
+          const double tmp = ka[j][k];
+          kb[3][k] += (xi6 - 4.0) * tmp;
+          kb[4][k] += (xi6 - 2.0) * tmp;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < nb; k++) kb[5][k] += ka[j][k];
+      }
+    }
+  }
+  if (!D3) {
+    const double c = g[1], s = g[2];
+    const double sl = s * oneOverL, cl = c * oneOverL;
+    double tmp[3][6];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      tmp[r][0] = -c * kb[r][0] - sl * (kb[r][1] + kb[r][2]);
+      tmp[r][1] = -s * kb[r][0] + cl * (kb[r][1] + kb[r][2]);
+      tmp[r][2] = kb[r][1];
+      tmp[r][3] = -tmp[r][0];
+      tmp[r][4] = -tmp[r][1];
+      tmp[r][5] = kb[r][2];
+    }
+    double* K = Ke + (long long)e * 36;
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      const double k0 = -c * tmp[0][m] - sl * (tmp[1][m] + tmp[2][m]);
+      const double k1 = -s * tmp[0][m] + cl * (tmp[1][m] + tmp[2][m]);
+      K[0 * 6 + m] = k0;
+      K[1 * 6 + m] = k1;
+      K[2 * 6 + m] = tmp[1][m];
+      K[3 * 6 + m] = -k0;
+      K[4 * 6 + m] = -k1;
+      K[5 * 6 + m] = tmp[2][m];
+    }
+    return;
+  }
+  const double* R = g + 1;
+  double tmp[6][12];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    tmp[i][0] = -kb[i][0];
+    tmp[i][1] = oneOverL * (kb[i][1] + kb[i][2]);
+    tmp[i][2] = -oneOverL * (kb[i][3] + kb[i][4]);
+    tmp[i][3] = -kb[i][5];
+    tmp[i][4] = kb[i][3];
+    tmp[i][5] = kb[i][1];
+    tmp[i][6] = kb[i][0];
+    tmp[i][7] = -tmp[i][1];
+    tmp[i][8] = -tmp[i][2];
+    tmp[i][9] = kb[i][5];
+    tmp[i][10] = kb[i][4];
+    tmp[i][11] = kb[i][2];
+  }
+  double* K = Ke + (long long)e * 144;
+  // column m of kl, rotated, then the row blocks of kg
+  for (int m = 0; m < 12; m++) {
+    double kl[12];
+    kl[0] = -tmp[0][m];
+    kl[1] = oneOverL * (tmp[1][m] + tmp[2][m]);
+    kl[2] = -oneOverL * (tmp[3][m] + tmp[4][m]);
+    kl[3] = -tmp[5][m];
+    kl[4] = tmp[3][m];
+    kl[5] = tmp[1][m];
+    kl[6] = tmp[0][m];
+    kl[7] = -kl[1];
+    kl[8] = -kl[2];
+    kl[9] = tmp[5][m];
+    kl[10] = tmp[4][m];
+    kl[11] = tmp[2][m];
+    // kl column m holds kl[r][m]; store temporarily in K row-major [r][m]
+#pragma unroll
+    for (int r = 0; r < 12; r++) K[r * 12 + m] = kl[r];
+  }
+  // tmp2[r][3b+c] = sum_k kl[r][3b+k] R[k][c]
+  for (int r = 0; r < 12; r++) {
+    double row[12];
+#pragma unroll
+    for (int c = 0; c < 12; c++) row[c] = K[r * 12 + c];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        K[r * 12 + 3 * b + c] =
+            row[3 * b] * R[0 * 3 + c] + row[3 * b + 1] * R[1 * 3 + c] + row[3 * b + 2] * R[2 * 3 + c];
+  }
+  // kg[3b+c][m] = sum_k R[k][c] tmp2[3b+k][m]
+  for (int m = 0; m < 12; m++) {
+    double col[12];
+#pragma unroll
+    for (int r = 0; r < 12; r++) col[r] = K[r * 12 + m];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        K[(3 * b + c) * 12 + m] =
+            R[0 * 3 + c] * col[3 * b] + R[1 * 3 + c] * col[3 * b + 1] + R[2 * 3 + c] * col[3 * b + 2];
+  }
+}
+
+}  // namespace xg

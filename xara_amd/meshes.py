@@ -185,3 +185,27 @@ def build_rc_frame_2d(model, fr, ele0=1):
     nc, nb = len(fr["columns"]), len(fr["beams"])
     model.elements("dispBeamColumn", np.arange(ele0, ele0 + nc), fr["columns"], 1, 1, 5)
     model.elements("dispBeamColumn", np.arange(ele0 + nc, ele0 + nc + nb), fr["beams"], 1, 1, 5)
+
+
+def define_transient_rc_frame_2d(model, fr, ag, dt, mass=60.0, alphaM=0.35, tol=1e-8, maxit=25):
+    """Plane RC frame under a UniformExcitation record in the x direction:
+    lumped floor masses (x and y), mass-proportional damping, Newmark
+    average acceleration, NormDispIncr test."""
+    build_rc_frame_2d(model, fr)
+    m = np.zeros((len(fr["floor_nodes"]), 3))
+    m[:, 0] = m[:, 1] = mass
+    model.masses(fr["floor_nodes"], m)
+    model.uniformExcitation(1, ag, dt)
+    model.rayleigh(alphaM)
+    model.test("NormDispIncr", tol, maxit)
+    model.integrator("Newmark", 0.5, 0.25)
+
+
+def frame_population(n, seed=2024):
+    """Stories uniform in 3..20 and bays uniform in 1..6 for n frames, and
+    the PGA of each (uniform 0.1-0.6 g), all seeded."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    stories = rng.integers(3, 21, n)
+    bays = rng.integers(1, 7, n)
+    pga = rng.uniform(0.1, 0.6, n)
+    return stories, bays, pga
