@@ -245,6 +245,41 @@ int  xg_ensemble_model_disp(xg_ensemble* e, int model, double* U, int* nnode);
 /* per-member status (0/1 running, -2 stopped) and Newton iteration totals */
 int  xg_ensemble_status(xg_ensemble* e, int* status, long long* newton_iters);
 
+/* ---- Adjoint sensitivity of a frame transient ----------------------------
+   Gradients of
+       loss = sum over the steps n of a forward run of sum_i U_n(node_i, dof)^2
+   with respect to every Steel02 material's (Fy, E0, b) and every
+   Concrete02 material's fc (as stored, compression negative), by a discrete
+   adjoint of the converged Newmark + Newton steps.  It stands in for the
+   reference's direct-differentiation path for such a gradient
+   (sensitivityAlgorithm / computeGradients, SRC/runtime/commands/analysis/
+   sensitivity.cpp; Newmark::computeSensitivities, Newmark.cpp:904-950),
+   which has no Steel02/Concrete02 material partials
+   (UniaxialMaterial.cpp:340-379).  Models: dispBeamColumn2d/3d over fiber
+   sections only, Newmark, nodal masses, mass-proportional damping.  The
+   state at the start of the forward run is treated as fixed. */
+/* loss nodes (tags) and DOF (1-based) */
+int  xg_adjoint_loss(xg_domain* d, int n, const int* node_tags, int dof);
+/* nsteps Newmark steps of size dt from the current committed state,
+   checkpointing the committed state every `ckpt` steps; *loss = loss */
+int  xg_adjoint_forward(xg_domain* d, int nsteps, double dt, int ckpt, double* loss);
+/* reverse sweep over the last forward run, recomputing each checkpoint
+   segment; gradients in ascending material-tag order (3 per Steel02,
+   1 per Concrete02).  The domain is left at the end of the forward run. */
+int  xg_adjoint_backward(xg_domain* d, double* grad_steel, double* grad_conc);
+/* wall times (ms) of the last forward and backward runs and the number of
+   CG iterations of the backward linear solves */
+int  xg_adjoint_stats(xg_domain* d, double* forward_ms, double* backward_ms, long long* backward_cg);
+/* Host evaluation of the per-fiber adjoint the kernels run: for a Steel02
+   (kind 1, prm = Fy E0 b R0 cR1 cR2 a1 a2 a3 a4 sigini) or Concrete02
+   (kind 2, prm = fc epsc0 fcu epscu rat ft Ets) trial update from the
+   committed history h (pool order: 10 / 5 values; kon for Steel02) at
+   `strain`, pull the adjoint hbar of the trial history back to the
+   committed history (cbar), the parameters (thbar: Fy E0 b / fc) and the
+   strain (*sbar).  The adjoint of the tangent entry (e) is not propagated. */
+int  xg_material_vjp(int kind, const double* prm, const double* h, int kon, double strain, const double* hbar,
+                     double* cbar, double* thbar, double* sbar);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 // tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it;
 // the product never does.  It exports the same C-ABI as include/xara_gpu.h
 // with the prefix `orc_` instead of `xg_`.
+#include "orc_ddm.hpp"
 #include "orc_elements.hpp"
 #include "orc_frame.hpp"
 
@@ -791,6 +792,229 @@ static int analyze(Dom& d, int nsteps, double dt) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Direct differentiation of a Newmark transient of a frame model (the
+// forward-mode counterpart of the GPU adjoint).  Per converged step, as
+// Newmark::computeSensitivities (Newmark.cpp:904-950): the sensitivity rhs
+// (formSensitivityRHS, :691) = -(dF_int/dtheta at fixed displacement, with
+// the history sensitivities of the last commit) - M (dA_pred) - alphaM M
+// (dV_pred), one solve with the converged effective tangent, the response
+// sensitivities (saveSensitivity, :791) and the material history
+// sensitivities at the converged strain (commitSensitivity, :880).  The
+// material partials, absent from the reference, come from the forward-mode
+// restatement in orc_ddm.hpp.
+// ---------------------------------------------------------------------------
+struct DdmDir {
+  std::map<int, SteelTheta<double>> steel;  // material tag -> (dFy, dE0, db)
+  std::map<int, double> conc;               // material tag -> dfc
+};
+
+struct DdmState {
+  std::vector<double> hdot;  // per fiber, 10 history sensitivities
+  std::vector<double> Ud, Vd, Ad;  // per node x 6
+};
+
+static double section_strain(const FrameElement& fe, const FiberSection& S, const FiberDef& F, const double* dfm) {
+  const double y = F.y - S.yBar;
+  if (!fe.is3d) return dfm[0] - y * dfm[1];
+  const double z = F.z - S.zBar;
+  return dfm[0] - y * dfm[1] + z * dfm[2];
+}
+
+static void section_deformations(const FrameElement& fe, const double* ug, double dfm[5][4]) {
+  double v[6];
+  fe.basic_disp(ug, v);
+  const double oneOverL = 1.0 / fe.L;
+  for (int i = 0; i < fe.nsec; i++) {
+    const double xi6 = 6.0 * (0.5 * (1.0 + kGL5pts[i]));
+    dfm[i][0] = oneOverL * v[0];
+    dfm[i][1] = oneOverL * ((xi6 - 4.0) * v[1] + (xi6 - 2.0) * v[2]);
+    dfm[i][2] = dfm[i][3] = 0.0;
+    if (fe.is3d) {
+      dfm[i][2] = oneOverL * ((xi6 - 4.0) * v[3] + (xi6 - 2.0) * v[4]);
+      dfm[i][3] = oneOverL * v[5];
+    }
+  }
+}
+
+// one fiber: committed state (+ history sensitivities) and strain (+ its
+// sensitivity) -> trial state sensitivities in pool order; returns dsigma
+static double fiber_ddm(const DdmDir& dir, const FiberDef& F, const FiberState& st, double strain, double dstrain,
+                        const double* hd, double* hout) {
+  if (F.kind == 1) {
+    SteelTheta<Dual> th{Dual(F.steel.Fy), Dual(F.steel.E0), Dual(F.steel.b)};
+    auto it = dir.steel.find(F.mat_tag);
+    if (it != dir.steel.end()) {
+      th.Fy.d = it->second.Fy;
+      th.E0.d = it->second.E0;
+      th.b.d = it->second.b;
+    }
+    const Steel02State& s = st.st;
+    SteelH<Dual> c{{s.epsminP, hd[0]}, {s.epsmaxP, hd[1]}, {s.epsplP, hd[2]}, {s.epss0P, hd[3]},
+                   {s.sigs0P, hd[4]},  {s.epssrP, hd[5]}, {s.sigsrP, hd[6]}, {s.epsP, hd[7]},
+                   {s.sigP, hd[8]},    {s.eP, hd[9]},     s.konP};
+    SteelH<Dual> t;
+    steel02_trial_t(F.steel, th, c, Dual(strain, dstrain), t);
+    const Dual o[10] = {t.epsmin, t.epsmax, t.epspl, t.epss0, t.sigs0, t.epsr, t.sigr, t.eps, t.sig, t.e};
+    for (int k = 0; k < 10; k++) hout[k] = o[k].d;
+    return t.sig.d;
+  }
+  if (F.kind == 2) {
+    Dual fc(F.conc.fc);
+    auto it = dir.conc.find(F.mat_tag);
+    if (it != dir.conc.end()) fc.d = it->second;
+    const Concrete02State& s = st.cs;
+    ConcH<Dual> c{{s.ecminP, hd[0]}, {s.deptP, hd[1]}, {s.epsP, hd[2]}, {s.sigP, hd[3]}, {s.eP, hd[4]}};
+    ConcH<Dual> t;
+    concrete02_trial_t(F.conc, fc, c, c.sig, c.e, Dual(strain, dstrain), t);
+    const Dual o[5] = {t.ecmin, t.dept, t.eps, t.sig, t.e};
+    for (int k = 0; k < 5; k++) hout[k] = o[k].d;
+    for (int k = 5; k < 10; k++) hout[k] = 0.0;
+    return t.sig.d;
+  }
+  for (int k = 0; k < 10; k++) hout[k] = 0.0;
+  return F.E * dstrain;
+}
+
+static int ddm_step(Dom& d, const DdmDir& dir, DdmState& S) {
+  const double g = d.nm_gamma, b = d.nm_beta, h = d.dt;
+  const double c2 = g / (b * h), c3 = 1.0 / (b * h * h);
+  const double a1 = 1.0 - g / b, a2 = h * (1.0 - 0.5 * g / b), a3 = -1.0 / (b * h), a4 = 1.0 - 0.5 / b;
+  const int ndf = d.ndf;
+  std::vector<double> rhs(d.neq, 0.0), x;
+  std::vector<int> id;
+  double hout[10];
+  // -dF_int/dtheta at the converged displacement
+  size_t fib = 0;
+  for (auto* e : d.eles) {
+    auto* be = dynamic_cast<OBeam*>(e);
+    if (!be) return fail(XG_ERR_ARG, "ddm: frame elements only");
+    FrameElement& fe = be->fe;
+    if (fe.rho != 0.0) return fail(XG_ERR_ARG, "ddm: element mass density is not supported");
+    double ci[3], cj[3], ug[12];
+    be->gather(d, ci, cj, ug);
+    double dfm[5][4];
+    section_deformations(fe, ug, dfm);
+    double saved[5][4];
+    for (int i = 0; i < fe.nsec; i++) {
+      FiberSection& sec = fe.secs[i];
+      double sd[4] = {0, 0, 0, 0};
+      for (size_t f = 0; f < sec.fs.size(); f++, fib++) {
+        const FiberDef& F = sec.def->fibers[f];
+        const double eps = section_strain(fe, sec, F, dfm[i]);
+        const double dsig = fiber_ddm(dir, F, sec.fs[f], eps, 0.0, &S.hdot[fib * 10], hout);
+        const double y = F.y - sec.yBar, z = fe.is3d ? F.z - sec.zBar : 0.0;
+        sd[0] += F.A * dsig;
+        sd[1] += -y * F.A * dsig;
+        sd[2] += z * F.A * dsig;
+      }
+      for (int k = 0; k < 4; k++) {
+        saved[i][k] = sec.s[k];
+        sec.s[k] = fe.is3d ? sd[k] : (k < 2 ? sd[k] : 0.0);
+      }
+    }
+    std::vector<double> P(fe.ndof());
+    fe.resisting_force(P.data());
+    for (int i = 0; i < fe.nsec; i++)
+      for (int k = 0; k < 4; k++) fe.secs[i].s[k] = saved[i][k];
+    element_id(d, e, id);
+    for (int i = 0; i < fe.ndof(); i++)
+      if (id[i] >= 0) rhs[id[i]] -= P[i];
+  }
+  // inertia and damping terms of the previous step's sensitivities
+  for (size_t k = 0; k < d.nodes.size(); k++) {
+    const NodeRec& n = d.nodes[k];
+    for (int j = 0; j < ndf; j++) {
+      const int a = n.eq[j];
+      if (a < 0 || n.mass[j] == 0.0) continue;
+      const double Ud = S.Ud[k * 6 + j], Vd = S.Vd[k * 6 + j], Ad = S.Ad[k * 6 + j];
+      const double apred = -c3 * Ud + a3 * Vd + a4 * Ad;
+      const double vpred = -c2 * Ud + a1 * Vd + a2 * Ad;
+      rhs[a] -= n.mass[j] * apred + d.alphaM * n.mass[j] * vpred;
+    }
+  }
+  if (form_tangent(d) < 0) return XG_BAD_STEP;
+  if (direct_solve(d, rhs, x) < 0) return XG_BAD_LINEAR_SOLVE;
+  for (size_t k = 0; k < d.nodes.size(); k++) {
+    const NodeRec& n = d.nodes[k];
+    for (int j = 0; j < ndf; j++) {
+      const int a = n.eq[j];
+      const double Un = a >= 0 ? x[a] : 0.0;
+      const double Ud = S.Ud[k * 6 + j], Vd = S.Vd[k * 6 + j], Ad = S.Ad[k * 6 + j];
+      S.Vd[k * 6 + j] = c2 * (Un - Ud) + a1 * Vd + a2 * Ad;
+      S.Ad[k * 6 + j] = c3 * (Un - Ud) + a3 * Vd + a4 * Ad;
+      S.Ud[k * 6 + j] = Un;
+    }
+  }
+  // history sensitivities at the converged strain (commitSensitivity)
+  fib = 0;
+  for (auto* e : d.eles) {
+    auto* be = static_cast<OBeam*>(e);
+    FrameElement& fe = be->fe;
+    double ci[3], cj[3], ug[12], ugd[12];
+    be->gather(d, ci, cj, ug);
+    const int nd = fe.is3d ? 6 : 3;
+    for (int a = 0; a < 2; a++)
+      for (int j = 0; j < nd; j++) ugd[a * nd + j] = S.Ud[(size_t)e->nodes[a] * 6 + j];
+    double dfm[5][4], dfmd[5][4];
+    section_deformations(fe, ug, dfm);
+    section_deformations(fe, ugd, dfmd);
+    for (int i = 0; i < fe.nsec; i++) {
+      FiberSection& sec = fe.secs[i];
+      for (size_t f = 0; f < sec.fs.size(); f++, fib++) {
+        const FiberDef& F = sec.def->fibers[f];
+        const double eps = section_strain(fe, sec, F, dfm[i]);
+        const double deps = section_strain(fe, sec, F, dfmd[i]);
+        fiber_ddm(dir, F, sec.fs[f], eps, deps, &S.hdot[fib * 10], hout);
+        for (int k = 0; k < 10; k++) S.hdot[fib * 10 + k] = hout[k];
+      }
+    }
+  }
+  return 0;
+}
+
+// loss = sum over steps of sum_i U(node_i, dof)^2 and its derivative along dir
+static int ddm_transient(Dom& d, int nsteps, double dt, const DdmDir& dir, const std::vector<int>& loss_nodes,
+                         int loss_dof, double& loss, double& dloss) {
+  if (!d.ready) {
+    int r = setup(d);
+    if (r < 0) return r;
+  }
+  if (d.integ != 3) return fail(XG_ERR_STATE, "ddm: the Newmark integrator is required");
+  if (d.betaK != 0 || d.betaKinit != 0 || d.betaKcomm != 0)
+    return fail(XG_ERR_ARG, "ddm: stiffness-proportional damping is not supported");
+  DdmState S;
+  size_t nfib = 0;
+  for (auto* e : d.eles) {
+    auto* be = dynamic_cast<OBeam*>(e);
+    if (!be) return fail(XG_ERR_ARG, "ddm: frame elements only");
+    for (auto& sec : be->fe.secs) nfib += sec.fs.size();
+  }
+  S.hdot.assign(nfib * 10, 0.0);
+  S.Ud.assign(d.nodes.size() * 6, 0.0);
+  S.Vd = S.Ud;
+  S.Ad = S.Ud;
+  loss = dloss = 0.0;
+  for (int s = 0; s < nsteps; s++) {
+    const double t0 = d.time;
+    int r = integ_new_step(d, dt);
+    if (r == 0) r = newton(d);
+    if (r == 0) r = ddm_step(d, dir, S);
+    if (r < 0) {
+      revert_domain(d);
+      d.time = t0;
+      return r;
+    }
+    for (int k : loss_nodes) {
+      const double u = d.nodes[k].U[loss_dof];
+      loss += u * u;
+      dloss += 2.0 * u * S.Ud[(size_t)k * 6 + loss_dof];
+    }
+    commit_domain(d);
+  }
+  return 0;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -878,6 +1102,7 @@ int orc_section_fiber(xg_domain* d, int tag, int nfib, const int* mat_tags, cons
     f.z = z ? z[i] : 0.0;
     f.A = A[i];
     int mt = mat_tags[i];
+    f.mat_tag = mt;
     if (d->steel.count(mt)) {
       f.kind = 1;
       f.steel = d->steel[mt];
@@ -1270,6 +1495,74 @@ int orc_rcm(int nv, const int* ptr, const int* adj, int* order) {
   std::vector<int> res = rcm_number(a);
   for (int i = 0; i < nv; i++) order[i] = res[i];
   return 0;
+}
+
+
+// Newmark transient with direct differentiation along one parameter
+// direction: dsteel = (dFy, dE0, db) per Steel02 material and dconc = dfc per
+// Concrete02 material, both in ascending material-tag order.  loss = sum
+// over the steps of sum_i U(node_i, dof)^2; dloss = its directional
+// derivative.
+int orc_ddm_transient(xg_domain* d, int nsteps, double dt, const double* dsteel, const double* dconc, int nloss,
+                      const int* loss_tags, int loss_dof, double* loss, double* dloss) {
+  DdmDir dir;
+  int k = 0;
+  for (auto& kv : d->steel) {
+    dir.steel[kv.first] = SteelTheta<double>{dsteel[3 * k], dsteel[3 * k + 1], dsteel[3 * k + 2]};
+    k++;
+  }
+  k = 0;
+  for (auto& kv : d->conc) dir.conc[kv.first] = dconc[k++];
+  if (!d->ready) {
+    int r = setup(*d);
+    if (r < 0) return r;
+  }
+  std::vector<int> nodes;
+  for (int i = 0; i < nloss; i++) {
+    auto it = d->node_index.find(loss_tags[i]);
+    if (it == d->node_index.end()) return fail(XG_ERR_ARG, "ddm: loss node not found");
+    nodes.push_back(it->second);
+  }
+  if (loss_dof < 1 || loss_dof > d->ndf) return fail(XG_ERR_ARG, "ddm: loss dof out of range");
+  return ddm_transient(*d, nsteps, dt, dir, nodes, loss_dof - 1, *loss, *dloss);
+}
+
+// One material trial update along a direction: committed history (pool
+// order: Steel02 10 values, Concrete02 5) and strain -> trial history, with
+// the forward-mode derivative for the input direction (dh, dtheta =
+// (dFy, dE0, db) or dfc, dstrain).  prm as orc_uniaxial_path.  For
+// Concrete02 the previous trial sig / e are taken equal to the committed
+// ones (the state right after a commit).
+int orc_material_jvp(int kind, const double* prm, const double* h, int kon, double strain, const double* dh,
+                     const double* dtheta, double dstrain, double* hout, int* kon_out, double* dhout) {
+  FiberDef F = make_fiber(kind, prm);
+  if (kind == 1) {
+    SteelTheta<Dual> th{Dual(F.steel.Fy, dtheta[0]), Dual(F.steel.E0, dtheta[1]), Dual(F.steel.b, dtheta[2])};
+    SteelH<Dual> c{{h[0], dh[0]}, {h[1], dh[1]}, {h[2], dh[2]}, {h[3], dh[3]}, {h[4], dh[4]},
+                   {h[5], dh[5]}, {h[6], dh[6]}, {h[7], dh[7]}, {h[8], dh[8]}, {h[9], dh[9]}, kon};
+    SteelH<Dual> t;
+    steel02_trial_t(F.steel, th, c, Dual(strain, dstrain), t);
+    const Dual o[10] = {t.epsmin, t.epsmax, t.epspl, t.epss0, t.sigs0, t.epsr, t.sigr, t.eps, t.sig, t.e};
+    for (int i = 0; i < 10; i++) {
+      hout[i] = o[i].v;
+      dhout[i] = o[i].d;
+    }
+    *kon_out = t.kon;
+    return 0;
+  }
+  if (kind == 2) {
+    ConcH<Dual> c{{h[0], dh[0]}, {h[1], dh[1]}, {h[2], dh[2]}, {h[3], dh[3]}, {h[4], dh[4]}};
+    ConcH<Dual> t;
+    concrete02_trial_t(F.conc, Dual(F.conc.fc, dtheta[0]), c, c.sig, c.e, Dual(strain, dstrain), t);
+    const Dual o[5] = {t.ecmin, t.dept, t.eps, t.sig, t.e};
+    for (int i = 0; i < 5; i++) {
+      hout[i] = o[i].v;
+      dhout[i] = o[i].d;
+    }
+    *kon_out = 0;
+    return 0;
+  }
+  return fail(XG_ERR_ARG, "material_jvp: kind must be 1 (Steel02) or 2 (Concrete02)");
 }
 
 }  // extern "C"

@@ -19,6 +19,7 @@ static const double kGL5wts[5] = {0.23692688505618942, 0.47862867049936619, 0.56
 
 struct FiberDef {
   int kind = 0;  // 1 steel02, 2 concrete02, 3 elastic
+  int mat_tag = 0;
   double y = 0, z = 0, A = 0;
   Steel02Params steel{};
   Concrete02Params conc{};

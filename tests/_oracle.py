@@ -32,12 +32,51 @@ def load():
             "orc_fiber_section_path": [C.c_int, C.c_int, P_I, P_D, P_D, P_D, P_D, C.c_double, C.c_int,
                                        C.c_int, P_D, P_I, P_D, P_D],
             "orc_rcm": [C.c_int, P_I, P_I, P_I],
+            "orc_ddm_transient": [C.c_void_p, C.c_int, C.c_double, P_D, P_D, C.c_int, P_I, C.c_int, P_D, P_D],
+            "orc_material_jvp": [C.c_int, P_D, P_D, C.c_int, C.c_double, P_D, P_D, C.c_double, P_D, P_I, P_D],
         }.items():
             f = getattr(lib, name)
             f.argtypes = args
             f.restype = C.c_int
             setattr(_orc, name[4:], f)
     return _orc
+
+
+def ddm_transient(model, nsteps, dt, dsteel, dconc, loss_tags, loss_dof=1):
+    """Run nsteps Newmark steps of an oracle model with direct
+    differentiation along (dsteel (n_steel, 3), dconc (n_conc,)); returns
+    (loss, directional derivative)."""
+    import numpy as np
+    from xara_amd import _lib
+    ds = np.ascontiguousarray(dsteel, np.float64).ravel()
+    dc = np.ascontiguousarray(dconc, np.float64).ravel()
+    tags = np.ascontiguousarray(loss_tags, np.int32)
+    loss, dloss = C.c_double(0), C.c_double(0)
+    rc = load().ddm_transient(model._h, nsteps, dt, ds.ctypes.data_as(_lib.P_D), dc.ctypes.data_as(_lib.P_D),
+                            len(tags), tags.ctypes.data_as(_lib.P_I), loss_dof, C.byref(loss), C.byref(dloss))
+    if rc < 0:
+        raise RuntimeError(f"ddm_transient failed [{rc}]: {load().error()}")
+    return loss.value, dloss.value
+
+
+def material_jvp(kind, prm, h, kon, strain, dh, dtheta, dstrain):
+    """One Steel02 (kind 1) / Concrete02 (kind 2) trial update with its
+    forward-mode derivative; returns (h_trial, kon_trial, dh_trial)."""
+    import numpy as np
+    from xara_amd import _lib
+    n = 10 if kind == 1 else 5
+    prm = np.ascontiguousarray(prm, np.float64)
+    h = np.ascontiguousarray(h, np.float64)
+    dh = np.ascontiguousarray(dh, np.float64)
+    dth = np.ascontiguousarray(dtheta, np.float64)
+    hout, dhout = np.zeros(n), np.zeros(n)
+    kout = C.c_int(0)
+    rc = load().material_jvp(kind, prm.ctypes.data_as(_lib.P_D), h.ctypes.data_as(_lib.P_D), int(kon), float(strain),
+                           dh.ctypes.data_as(_lib.P_D), dth.ctypes.data_as(_lib.P_D), float(dstrain),
+                           hout.ctypes.data_as(_lib.P_D), C.byref(kout), dhout.ctypes.data_as(_lib.P_D))
+    if rc < 0:
+        raise RuntimeError(load().error())
+    return hout, kout.value, dhout
 
 
 def oracle_model(ndm, ndf=None):

@@ -97,8 +97,14 @@ SIGNATURES = {
     "ensemble_analyze": (C.c_int, [VP, C.c_int, C.c_double, P_I]),
     "ensemble_model_disp": (C.c_int, [VP, C.c_int, P_D, P_I]),
     "ensemble_status": (C.c_int, [VP, P_I, C.POINTER(C.c_longlong)]),
+    # adjoint (GPU library only)
+    "adjoint_loss": (C.c_int, [VP, C.c_int, P_I, C.c_int]),
+    "adjoint_forward": (C.c_int, [VP, C.c_int, C.c_double, C.c_int, P_D]),
+    "adjoint_backward": (C.c_int, [VP, P_D, P_D]),
+    "adjoint_stats": (C.c_int, [VP, P_D, P_D, C.POINTER(C.c_longlong)]),
+    "material_vjp": (C.c_int, [C.c_int, P_D, P_D, C.c_int, C.c_double, P_D, P_D, P_D, P_D]),
 }
-GPU_ONLY = {k for k in SIGNATURES if k.startswith("ensemble_")} | {"device_name"}
+GPU_ONLY = {k for k in SIGNATURES if k.startswith(("ensemble_", "adjoint_"))} | {"device_name", "material_vjp"}
 
 
 class Binding:

@@ -4,12 +4,17 @@
 #include <cstring>
 #include <string>
 
+#include <memory>
+
+#include "xg_adjoint.h"
 #include "xg_domain.h"
-#include "xg_kernels.h"
 #include "xg_ensemble_host.h"
+#include "xg_kernels.h"
+#include "xg_material_adj.h"
 
 struct xg_domain {
   xg::Domain d;
+  std::unique_ptr<xg::Adjoint> adj;
 };
 
 namespace {
@@ -762,6 +767,75 @@ int xg_ensemble_status(xg_ensemble* h, int* status, long long* newton_iters) {
     if (status) e.status.download(status, n, e.merged.stream);
     if (newton_iters) e.newton_count.download(newton_iters, n, e.merged.stream);
     return 0;
+  });
+}
+
+
+// ---- adjoint ----------------------------------------------------------------
+static xg::Adjoint& ADJ(xg_domain* h) {
+  if (!h->adj) h->adj = std::make_unique<xg::Adjoint>();
+  return *h->adj;
+}
+
+int xg_adjoint_loss(xg_domain* h, int n, const int* node_tags, int dof) {
+  return guard([&] {
+    auto& d = D(h);
+    if (n <= 0 || !node_tags) throw xg::Error(XG_ERR_ARG, "adjoint loss: no nodes");
+    ADJ(h).set_loss(d, std::vector<int>(node_tags, node_tags + n), dof);
+    return 0;
+  });
+}
+
+int xg_adjoint_forward(xg_domain* h, int nsteps, double dt, int ckpt, double* loss) {
+  return guard([&] {
+    auto& d = D(h);
+    const double L = ADJ(h).forward(d, nsteps, dt, ckpt);
+    if (loss) *loss = L;
+    return 0;
+  });
+}
+
+int xg_adjoint_backward(xg_domain* h, double* grad_steel, double* grad_conc) {
+  return guard([&] {
+    auto& d = R(h);
+    if (!h->adj) throw xg::Error(XG_ERR_STATE, "adjoint: no forward run");
+    h->adj->backward(d, grad_steel, grad_conc);
+    return 0;
+  });
+}
+
+int xg_adjoint_stats(xg_domain* h, double* forward_ms, double* backward_ms, long long* backward_cg) {
+  return guard([&] {
+    (void)D(h);
+    if (!h->adj) throw xg::Error(XG_ERR_STATE, "adjoint: no forward run");
+    if (forward_ms) *forward_ms = h->adj->fwd_ms;
+    if (backward_ms) *backward_ms = h->adj->bwd_ms;
+    if (backward_cg) *backward_cg = h->adj->bwd_cg;
+    return 0;
+  });
+}
+
+int xg_material_vjp(int kind, const double* prm, const double* h, int kon, double strain, const double* hbar,
+                    double* cbar, double* thbar, double* sbar) {
+  return guard([&] {
+    if (!prm || !h || !hbar || !cbar || !thbar || !sbar) throw xg::Error(XG_ERR_ARG, "material_vjp: null pointer");
+    if (kind == 1) {
+      const xg::Steel02Param p{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5],
+                               prm[6], prm[7], prm[8], prm[9], prm[10]};
+      xg::steel02_vjp(p, h, kon, strain, hbar, cbar, thbar, *sbar);
+      return 0;
+    }
+    if (kind == 2) {
+      xg::Concrete02Param p{prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6]};
+      // sign normalisation of the constructor, Concrete02.cpp:93-112
+      if (p.fc > 0) p.fc = -p.fc;
+      if (p.epsc0 > 0) p.epsc0 = -p.epsc0;
+      if (p.fcu > 0) p.fcu = -p.fcu;
+      if (p.epscu > 0) p.epscu = -p.epscu;
+      xg::concrete02_vjp(p, h, strain, hbar, cbar, thbar[0], *sbar);
+      return 0;
+    }
+    throw xg::Error(XG_ERR_ARG, "material_vjp: kind must be 1 (Steel02) or 2 (Concrete02)");
   });
 }
 

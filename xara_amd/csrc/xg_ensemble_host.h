@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "xg_domain.h"
+#include "xg_frame.h"
 #include "xg_ensemble.h"
 
 namespace xg {
@@ -62,7 +63,5 @@ struct Ensemble {
   void enqueue_step(hipStream_t s);
   void drop_graph();
 };
-
-FrameDev frame_dev(Group& g);
 
 }  // namespace xg

@@ -312,7 +312,11 @@ struct FrameGroup : Group {
   }
 };
 
-FrameDev frame_dev(Group& g) { return static_cast<FrameGroup&>(g).f; }
+FrameDev frame_dev(Group& g) {
+  auto* fg = dynamic_cast<FrameGroup*>(&g);
+  if (!fg) throw Error(XG_ERR_ARG, "not a frame-element group");
+  return fg->f;
+}
 
 std::unique_ptr<Group> make_frame_group(Domain& d) {
   auto g = std::make_unique<FrameGroup>();

@@ -3,10 +3,13 @@
 #include <memory>
 
 #include "xg_domain.h"
+#include "xg_frame_dev.h"
 
 namespace xg {
 
 std::unique_ptr<Group> make_frame_group(Domain& d);
 void frame_group_add(Domain& d, Group& g, int tag, const EleDef& ed);
+// device view of a frame group (throws for any other element group)
+FrameDev frame_dev(Group& g);
 
 }  // namespace xg

@@ -131,9 +131,11 @@ __device__ __forceinline__ void fiber_trial(const FrameDev& f, int kind, int mat
   }
 }
 
-// Section update, one wavefront per section instance.
+// Section deformation at the section's Gauss point from nodal
+// displacements U (DispBeamColumn2d.cpp:235-284, DispBeamColumn3d.cpp:235-292).
 template <bool D3>
-__device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
+__device__ __forceinline__ void section_deformation(const FrameDev& f, long long sec, const double* __restrict__ U,
+                                                    double d[4]) {
   const int e = (int)(sec / f.nip), ip = (int)(sec % f.nip);
   double v[6];
   basic_disp<D3>(f, e, U, v);
@@ -141,13 +143,20 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
   const double oneOverL = 1.0 / L;
   const double xi = 0.5 * (1.0 + gl5_pt(ip));
   const double xi6 = 6.0 * xi;
-  double d[4];
   d[0] = oneOverL * v[0];
   d[1] = oneOverL * ((xi6 - 4.0) * v[1] + (xi6 - 2.0) * v[2]);
   if (D3) {
     d[2] = oneOverL * ((xi6 - 4.0) * v[3] + (xi6 - 2.0) * v[4]);
     d[3] = oneOverL * v[5];
   }
+}
+
+// Section update, one wavefront per section instance.
+template <bool D3>
+__device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
+  const int e = (int)(sec / f.nip);
+  double d[4];
+  section_deformation<D3>(f, sec, U, d);
   const int def = f.secdef[e];
   const int f0 = f.def_off[def], nf = f.def_off[def + 1] - f0;
   const long long sb = f.steel_base[sec], cb = f.conc_base[sec], eb = f.elas_base[sec];

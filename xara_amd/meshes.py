@@ -141,7 +141,7 @@ def rc_frame_3d(nbx=10, nby=10, nst=20, bay=6000.0, story=3500.0):
                 roof=roof, height=story * nst)
 
 
-def build_rc_frame_3d(model, fr, GJ=1.1e14):
+def build_rc_frame_3d(model, fr, GJ=1.1e14, loads=True):
     model.nodes(fr["node_tags"], fr["crds"])
     fx = fr["fixed_nodes"]
     model.fix_many(fx, np.ones((len(fx), 6), np.int32))
@@ -154,9 +154,10 @@ def build_rc_frame_3d(model, fr, GJ=1.1e14):
     model.elements("dispBeamColumn", np.arange(1, nc + 1), fr["columns"], 1, 1, 5)
     model.elements("dispBeamColumn", np.arange(nc + 1, nc + nx_ + 1), fr["beams_x"], 2, 1, 5)
     model.elements("dispBeamColumn", np.arange(nc + nx_ + 1, nc + nx_ + ny_ + 1), fr["beams_y"], 2, 1, 5)
-    P = np.zeros((len(fr["load_tags"]), 6))
-    P[:, 0] = fr["Px"]
-    model.loads(fr["load_tags"], P)
+    if loads:
+        P = np.zeros((len(fr["load_tags"]), 6))
+        P[:, 0] = fr["Px"]
+        model.loads(fr["load_tags"], P)
 
 
 def rc_frame_2d(nst, nbay, bay=6000.0, story=3500.0, tag0=0):
@@ -209,3 +210,76 @@ def frame_population(n, seed=2024):
     bays = rng.integers(1, 7, n)
     pga = rng.uniform(0.1, 0.6, n)
     return stories, bays, pga
+
+
+# ---------------------------------------------------------------------------
+# Frames with their own materials per element (parameter studies)
+# ---------------------------------------------------------------------------
+THETA_NAMES = ("fy", "E", "b", "fc")
+STEEL_TAG0, CONC_TAG0 = 100000, 200000
+
+
+def default_theta(nele):
+    """Per-element parameters (fy, E, b of Steel02; fc of Concrete02)."""
+    return np.tile(np.array([STEEL02[0], STEEL02[1], STEEL02[2], CONCRETE02[0]]), (nele, 1))
+
+
+def frame_elements(fr):
+    """(connectivity, geomTransf tag) of every element in tag order."""
+    if "beams" in fr:
+        conn = np.concatenate([fr["columns"], fr["beams"]])
+        return conn, np.ones(len(conn), np.int32)
+    conn = np.concatenate([fr["columns"], fr["beams_x"], fr["beams_y"]])
+    transf = np.full(len(conn), 2, np.int32)
+    transf[: len(fr["columns"])] = 1
+    return conn, transf
+
+
+def build_rc_frame_per_element(model, fr, theta, GJ=1.1e14):
+    """RC frame (rc_frame_2d or rc_frame_3d layout, no loads) where element
+    e (tag e, 1-based) has its own Steel02 (tag STEEL_TAG0 + e), Concrete02
+    (tag CONC_TAG0 + e) and fiber section (tag e) with the parameters of row
+    e - 1 of theta: (fy, E, b, fc)."""
+    three_d = "beams_x" in fr
+    ndf = 6 if three_d else 3
+    model.nodes(fr["node_tags"], fr["crds"])
+    fx = fr["fixed_nodes"]
+    model.fix_many(fx, np.ones((len(fx), ndf), np.int32))
+    if three_d:
+        model.geomTransf("Linear", 1, 1.0, 0.0, 0.0)
+        model.geomTransf("Linear", 2, 0.0, 0.0, 1.0)
+    else:
+        model.geomTransf("Linear", 1)
+    conn, transf = frame_elements(fr)
+    theta = np.asarray(theta, np.float64).reshape(len(conn), 4)
+    for e in range(len(conn)):
+        tag = e + 1
+        fy, E, b, fc = theta[e]
+        model.uniaxialMaterial("Steel02", STEEL_TAG0 + tag, fy, E, b, *STEEL02[3:])
+        model.uniaxialMaterial("Concrete02", CONC_TAG0 + tag, fc, *CONCRETE02[1:])
+        model.section("Fiber", tag, rc_section_fibers(conc_tag=CONC_TAG0 + tag, steel_tag=STEEL_TAG0 + tag),
+                      GJ=GJ if three_d else None)
+        model.elements("dispBeamColumn", [tag], conn[e:e + 1], int(transf[e]), tag, 5)
+
+
+def define_transient_frame(model, fr, theta, ag, dt, mass=60.0, alphaM=0.35, tol=1e-8, maxit=25):
+    """Per-element RC frame (2D or 3D) under a UniformExcitation record in x:
+    lumped translational floor masses, mass-proportional damping, Newmark
+    average acceleration, NormDispIncr test."""
+    three_d = "beams_x" in fr
+    build_rc_frame_per_element(model, fr, theta)
+    floor = fr["floor_nodes"] if "floor_nodes" in fr else fr["load_tags"]
+    ndf = 6 if three_d else 3
+    m = np.zeros((len(floor), ndf))
+    m[:, : (3 if three_d else 2)] = mass
+    model.masses(floor, m)
+    model.uniformExcitation(1, ag, dt)
+    model.rayleigh(alphaM)
+    model.test("NormDispIncr", tol, maxit)
+    model.integrator("Newmark", 0.5, 0.25)
+
+
+def roof_nodes(fr):
+    """Tags of the nodes on the top floor."""
+    z = fr["crds"][:, -1]
+    return fr["node_tags"][z == z.max()].astype(np.int32)

@@ -57,6 +57,7 @@ class Model:
         self._h = h
         self.ndm, self.ndf = ndm, ndf
         self._node_tags = None
+        self._mat_tags = {"Steel02": set(), "Concrete02": set()}
 
     # -- plumbing ---------------------------------------------------------
     def _check(self, rc):
@@ -122,6 +123,7 @@ class Model:
             a1, a2, a3, a4 = (list(args[6:10]) + [0.0, 1.0, 0.0, 1.0][len(args[6:10]):])
             sig0 = args[10] if len(args) > 10 else 0.0
             self._check(self._b.uniaxial_steel02(self._h, tag, fy, E, b, R0, cR1, cR2, a1, a2, a3, a4, sig0))
+            self._mat_tags["Steel02"].add(tag)
         elif kind == "Concrete02":
             fc, epsc0, fcu, epscu = args[:4]
             if len(args) >= 7:
@@ -129,10 +131,16 @@ class Model:
             else:  # Concrete02.cpp:134-138
                 rat, ft, Ets = 0.1, abs(0.1 * fc), 0.1 * fc / epsc0
             self._check(self._b.uniaxial_concrete02(self._h, tag, fc, epsc0, fcu, epscu, rat, ft, Ets))
+            self._mat_tags["Concrete02"].add(tag)
         elif kind == "Elastic":
             self._check(self._b.uniaxial_elastic(self._h, tag, args[0]))
         else:
             raise XaraError(-1, f"uniaxialMaterial {kind} is not on the GPU path")
+
+    def material_tags(self, kind):
+        """Tags of the defined materials of one kind, ascending (the order of
+        per-material results such as the adjoint gradients)."""
+        return sorted(self._mat_tags[kind])
 
     def section(self, kind, tag, fibers, GJ=None, centroid=True):
         """``section Fiber tag <-GJ GJ> {fiber y z A mat ...}``; fibers is a
