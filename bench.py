@@ -76,7 +76,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run the 256 MiB copy kernel (profiler counter calibration)")
-    ap.add_argument("--mode", choices=["update", "newton", "frame", "ensemble"], default="update")
+    ap.add_argument("--mode", choices=["update", "newton", "frame", "ensemble", "adjoint"], default="update")
+    ap.add_argument("--adj-frame", default="5,5,10", help="bays x, bays y, stories of the adjoint mode")
+    ap.add_argument("--adj-steps", type=int, default=500)
+    ap.add_argument("--ckpt", type=int, default=25)
+    ap.add_argument("--ddm", action="store_true", help="also run the CPU oracle's forward-mode check")
     ap.add_argument("--models", type=int, default=4096, help="frames in the ensemble mode")
     ap.add_argument("--record-steps", type=int, default=2000, help="ground-motion record length (steps)")
     ap.add_argument("--looped", type=int, default=16, help="frames timed alone for the looped comparison")
@@ -97,6 +101,8 @@ def main():
         return frame_main(args)
     if args.mode == "ensemble":
         return ensemble_main(args)
+    if args.mode == "adjoint":
+        return adjoint_main(args)
 
     from xara_amd import meshes
     from xara_amd.model import Model
@@ -435,6 +441,91 @@ def ensemble_cpu_baseline(define, nsample, seconds, dt):
     wall = time.perf_counter() - t0
     return {"value": steps / wall, "unit": "model-steps/s", "cores": 1, "kind": "port",
             "sample": f"first {nsample} frames of the population, {steps} Newmark steps from rest"}
+
+
+def adjoint_main(args):
+    """Adjoint gradients of a 3D RC frame transient: loss = sum over the
+    steps of the squared roof x displacements, parameters = per-element
+    Steel02 fy, E, b and Concrete02 fc.  Reports adjoint (reverse sweep with
+    checkpoint recompute) / forward wall time and checks the gradient along
+    a random direction against central differences of the GPU forward."""
+    from xara_amd import ground_motion, meshes
+    from xara_amd.adjoint import FrameTransient
+    from xara_amd.model import Model
+
+    nbx, nby, nst = (int(v) for v in args.adj_frame.split(","))
+    fr = meshes.rc_frame_3d(nbx, nby, nst)
+    ne = len(meshes.frame_elements(fr)[0])
+    dt, N = 0.01, args.adj_steps
+    pga = float(meshes.frame_population(4096)[2][0])  # member 0 of the ensemble population
+    ag = ground_motion.kanai_tajimi(0, N + 5, dt, pga)
+
+    def build(m, theta):
+        meshes.define_transient_frame(m, fr, theta, ag, dt, tol=1e-10, maxit=40)
+        m.system(args.system, 1e-12, 200000)
+
+    roof = meshes.roof_nodes(fr)
+    prob = FrameTransient(3, 6, build, roof, N, dt, ckpt=args.ckpt)
+    theta = meshes.default_theta(ne)
+    # plain forward (no checkpoints), for the ratio
+    m = Model(3, 6)
+    build(m, theta)
+    m.setup()
+    t0 = time.perf_counter()
+    for k in range(N):
+        m.analyze(1, dt)
+        if (k + 1) % 100 == 0:
+            print(f"bench: plain forward step {k + 1} t={time.perf_counter() - t0:.1f}s", file=sys.stderr,
+                  flush=True)
+    plain_s = time.perf_counter() - t0
+    m.close()
+    t0 = time.perf_counter()
+    loss = prob.run(theta)
+    fwd_s = time.perf_counter() - t0
+    print(f"bench: checkpointed forward {fwd_s:.2f}s loss={loss:.6e}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    grad = prob.gradient()
+    bwd_s = time.perf_counter() - t0
+    st = prob.stats()
+    print(f"bench: adjoint sweep {bwd_s:.2f}s", file=sys.stderr, flush=True)
+    # gradient check along a random direction: central differences (h = 1e-6 relative)
+    rng = np.random.default_rng(7)
+    d = rng.standard_normal((ne, 4)) * theta
+    h = 1e-6
+    Lp = prob.run(theta + h * d)
+    Lm = prob.run(theta - h * d)
+    fd = (Lp - Lm) / (2 * h)
+    adj = float(np.sum(grad * d))
+    out = {
+        "metric": "adjoint/forward wall-time ratio", "value": (fwd_s + bwd_s) / plain_s, "unit": "x",
+        "n_gpus": 1, "steps": N, "warmup": 0, "ms_per_step": 1e3 * (fwd_s + bwd_s) / N,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Kanai-Tajimi record)",
+        "config": {"workload": f"3D RC frame {nbx}x{nby} bays x {nst} stories, {N}-step Newmark transient, "
+                               f"loss = sum of squared roof x displacements",
+                   "elements": ne, "parameters": 4 * ne, "checkpoint_every": args.ckpt, "pga_g": pga,
+                   "plain_forward_s": plain_s, "checkpointed_forward_s": fwd_s, "adjoint_sweep_s": bwd_s,
+                   "adjoint_sweep_over_forward": bwd_s / plain_s, "backward_cg_iters": st["backward_cg_iters"],
+                   "loss": loss},
+        "gradient_check": {"direction": "random (seed 7), relative to theta", "adjoint": adj,
+                           "central_fd_h1e-6": fd, "rel_err": abs(adj - fd) / max(abs(fd), 1e-300)},
+    }
+    if args.ddm:
+        from tests import _oracle
+        o = _oracle.oracle_model(3, 6)
+        meshes.define_transient_frame(o, fr, theta, ag, dt, tol=1e-10, maxit=40)
+        o.setup()
+        t0 = time.perf_counter()
+        L, dL = _oracle.ddm_transient(o, N, dt, d[:, :3], d[:, 3], roof, 1)
+        ddm_s = time.perf_counter() - t0
+        out["gradient_check"].update({"oracle_ddm": dL, "rel_err_ddm": abs(adj - dL) / abs(dL),
+                                      "oracle_loss": L})
+        out["cpu_baseline"] = {"value": ddm_s, "unit": "s per directional derivative", "cores": 1,
+                               "kind": "port",
+                               "sample": f"oracle forward-mode (DDM) transient along one direction, {N} steps; "
+                                         f"the full gradient needs {4 * ne} such runs"}
+    prob.close()
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -18,7 +18,7 @@
 //   K_eff mu = utilde   (K_eff = K_T + (c3 + alphaM c2) M at the converged
 //                        state, the Newton tangent; symmetric),
 //   hbar_f <- J_h^T (hbar_f - w_f e_sig),  thetabar += J_theta^T (same),
-//            w_f = wt A_f eps_f(mu)  (mu^T F_int = sum_f w_f sigma_f),
+//            w_f = wt L A_f eps_f(mu)  (mu^T F_int = sum_f w_f sigma_f),
 //   ubar <- (c3 + alphaM c2) M mu - c2 vbar - c3 abar (+ loss gradient),
 //   vbar <- -(a3 + alphaM a1) M mu + a1 vbar + a3 abar,
 //   abar <- -(a4 + alphaM a2) M mu + a2 vbar + a4 abar.

@@ -6,7 +6,7 @@
 //       the element resisting-force kernel turns it into the nodal vector
 //       sum_f (d eps_f / dU)^T J_eps^T hbar;
 //   k_adj_history -- with the adjoint solution mu: the fiber's virtual
-//       strain eps(mu), the weight w = wt A eps(mu) that mu^T F_int puts on
+//       strain eps(mu), the weight w = wt L A eps(mu) that mu^T F_int puts on
 //       its stress, and the pull-back of hbar - w e_sigma to the committed
 //       history (written in place) and to the material parameters
 //       (accumulated per fiber);
@@ -68,12 +68,14 @@ __global__ __launch_bounds__(256) void k_adj_strain_adjoint(FrameDev f, const do
   r1 = wave_sum(r1);
   if (D3) r2 = wave_sum(r2);
   if (lane == 0) {
+    // the basic forces are q = L sum_ip wt B_ip^T s_ip (B_ip holding the
+    // 1/L of the section deformation), so s_hat = r / (wt L)
     const long long n = f.nsec;
-    const double wt = gl5_wt(ip) * 0.5;
-    shat[0 * n + sec] = r0 / wt;
-    shat[1 * n + sec] = r1 / wt;
+    const double wL = gl5_wt(ip) * 0.5 * f.geo[(long long)e * 10];
+    shat[0 * n + sec] = r0 / wL;
+    shat[1 * n + sec] = r1 / wL;
     if (D3) {
-      shat[2 * n + sec] = r2 / wt;
+      shat[2 * n + sec] = r2 / wL;
       shat[3 * n + sec] = 0.0;
     }
   }
@@ -91,7 +93,8 @@ __global__ __launch_bounds__(256) void k_adj_history(FrameDev f, const double* _
   double d[4], dm[4];
   section_deformation<D3>(f, sec, U, d);
   section_deformation<D3>(f, sec, mu, dm);
-  const double wt = gl5_wt(ip) * 0.5;
+  // mu^T F_int = sum_ip wt L d_ip(mu) . s_ip (q = L sum_ip wt B_ip^T s_ip)
+  const double wL = gl5_wt(ip) * 0.5 * f.geo[(long long)e * 10];
   const int def = f.secdef[e];
   const int f0 = f.def_off[def], nf = f.def_off[def + 1] - f0;
   const long long sb0 = f.steel_base[sec], cb0 = f.conc_base[sec];
@@ -103,7 +106,7 @@ __global__ __launch_bounds__(256) void k_adj_history(FrameDev f, const double* _
     const double z = D3 ? f.fz[fi] : 0.0;
     const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
     const double vstrain = D3 ? dm[0] - y * dm[1] + z * dm[2] : dm[0] - y * dm[1];
-    const double w = wt * f.fA[fi] * vstrain;
+    const double w = wL * f.fA[fi] * vstrain;
     double eb;
     if (kind == 1) {
       const long long n = f.nsteel, idx = sb0 + f.flocal[fi];
