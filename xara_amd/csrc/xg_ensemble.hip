@@ -10,7 +10,8 @@
 //   k_ens_iterate  -- one NewtonRaphson iteration (NewtonRaphson.cpp:160-215)
 //                     for every model still iterating: element tangents,
 //                     banded assembly of c1 K + c2 C + c3 M in element order,
-//                     banded Cholesky, solve, Newmark::update, fiber/section
+//                     banded Cholesky and solve (one wavefront for bands up to
+//                     24), Newmark::update, fiber/section
 //                     update, unbalance, CTestNormDispIncr / NormUnbalance.
 //   k_ens_commit   -- Domain::commit of converged models; revert of models
 //                     whose Newton failed (they stop, as analyze returns < 0).
@@ -73,6 +74,87 @@ __device__ void model_update_and_unbalance(const EnsDev& E, int m, double ag) {
   __syncthreads();
 }
 
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Banded Cholesky L L^T = A and solve, by one 64-lane wavefront (lanes of
+// the calling wave only, no barriers).  A is symmetric positive definite
+// with half bandwidth bw <= BW, stored by rows: A(r, c) = Lb[r*ld + c - r + bw]
+// for r - bw <= c <= r.  Right-looking: at pivot j, lane i holds row j + i
+// of the active window in registers (w[c] = A(j+i, j+c), c <= i) and the
+// forward-substituted right-hand side y; the pivot column is broadcast with
+// readlane, the trailing window updated, and the window shifted one row
+// down the wave.  Column j of L (diagonal first) overwrites band row j,
+// which has left the window by then.  x holds z = L^{-1} b, then the
+// solution.  Returns nonzero if a pivot is not positive.
+template <int BW>
+__device__ int wave_band_solve(double* __restrict__ Lb, int ld, int bw, int n, const double* __restrict__ b,
+                               double* __restrict__ x) {
+  const int lane = threadIdx.x & 63;
+  const bool act = lane <= bw;
+  double w[BW + 1];
+#pragma unroll
+  for (int c = 0; c <= BW; c++) w[c] = 0.0;
+  double y = 0.0;
+  if (act && lane < n) {
+    const double* row = Lb + (long long)lane * ld + (bw - lane);
+#pragma unroll
+    for (int c = 0; c <= BW; c++)
+      if (c <= lane) w[c] = row[c];
+    y = b[lane];
+  }
+  int bad = 0;
+  for (int j = 0; j < n; j++) {
+    // row j + 1 + bw enters the window after this pivot (into lane bw):
+    // lane c loads its entry c now, so the loads overlap the pivot work
+    const int r = j + 1 + bw;
+    const double nrow = (act && r < n) ? Lb[(long long)r * ld + lane] : 0.0;
+    const double ny = r < n ? b[r] : 0.0;
+    const double a00 = readlane_d(w[0], 0);
+    if (!(a00 > 0.0)) bad = 1;
+    const double d = sqrt(a00 > 0.0 ? a00 : 1.0);
+    const double l = lane == 0 ? d : (act ? w[0] / d : 0.0);
+    const double xj = readlane_d(y, 0) / d;
+    if (act && j + lane < n) Lb[(long long)j * ld + lane] = l;
+    if (lane == 0) x[j] = xj;
+    y -= lane == 0 ? 0.0 : l * xj;
+    // trailing update of the window: w[c] -= l_i l_c (entries c > i unused)
+#pragma unroll
+    for (int c = 1; c <= BW; c++) w[c] -= l * readlane_d(l, c);
+    // shift: row j+1+i moves to lane i, columns by one
+#pragma unroll
+    for (int c = 0; c < BW; c++) w[c] = __shfl_down(w[c + 1], 1, 64);
+    w[BW] = 0.0;
+    y = __shfl_down(y, 1, 64);
+#pragma unroll
+    for (int c = 0; c <= BW; c++) {
+      const double v = readlane_d(nrow, c);
+      if (lane == bw) w[c] = v;
+    }
+    if (lane == bw) y = ny;
+  }
+  // backward substitution: x_i = (z_i - sum_k L(i+k, i) x_{i+k}) / L(i, i),
+  // lane k holding x_{i+k}
+  double xw = 0.0;
+  double lk = (act && n - 1 + lane < n) ? Lb[(long long)(n - 1) * ld + lane] : 0.0;
+  for (int i = n - 1; i >= 0; i--) {
+    const double lnext = (i > 0 && act && i - 1 + lane < n) ? Lb[(long long)(i - 1) * ld + lane] : 0.0;
+    double part = lane == 0 ? 0.0 : lk * xw;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    const double xi = (x[i] - part) / readlane_d(lk, 0);
+    xw = __shfl_up(xw, 1, 64);
+    if (lane == 1) xw = xi;
+    if (lane == 0) x[i] = xi;
+    lk = lnext;
+  }
+  return bad;
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void k_ens_predict(EnsDev E) {
   for (int k = blockIdx.x; k < E.nmodel; k += gridDim.x) {
@@ -131,49 +213,61 @@ __global__ __launch_bounds__(NT) void k_ens_iterate(EnsDev E) {
       if (mm != 0.0) L[(long long)a * ld + bw] += (c3 + c2 * alphaM) * mm;
     }
     __syncthreads();
-    // ---- banded Cholesky, left-looking by columns ----
-    for (int j = 0; j < n; j++) {
-      if (tid == 0) {
-        double s = L[(long long)j * ld + bw];
-        const int k0 = j - bw > 0 ? j - bw : 0;
-        for (int kk = k0; kk < j; kk++) {
-          const double v = L[(long long)j * ld + (kk - j + bw)];
-          s -= v * v;
-        }
-        if (!(s > 0.0)) s_bad = 1;
-        L[(long long)j * ld + bw] = sqrt(s > 0.0 ? s : 1.0);
-      }
-      __syncthreads();
-      const double djj = L[(long long)j * ld + bw];
-      for (int r = tid; r < bw; r += NT) {
-        const int i = j + 1 + r;
-        if (i >= n) break;
-        double s = L[(long long)i * ld + (j - i + bw)];
-        const int k0 = i - bw > 0 ? i - bw : 0;
-        for (int kk = k0; kk < j; kk++)
-          s -= L[(long long)i * ld + (kk - i + bw)] * L[(long long)j * ld + (kk - j + bw)];
-        L[(long long)i * ld + (j - i + bw)] = s / djj;
-      }
-      __syncthreads();
-    }
-    // ---- solve L L^T x = B ----
     double* x = E.X + eq0;
     const double* rhs = E.B + eq0;
-    if (tid == 0) {
-      for (int i = 0; i < n; i++) {
-        double s = rhs[i];
-        const int k0 = i - bw > 0 ? i - bw : 0;
-        for (int kk = k0; kk < i; kk++) s -= L[(long long)i * ld + (kk - i + bw)] * x[kk];
-        x[i] = s / L[(long long)i * ld + bw];
+    if (bw <= 24) {
+      // ---- one wavefront: banded Cholesky + forward substitution, then
+      //      backward substitution (the other waves wait) ----
+      if (tid < 64) {
+        int bad;
+        if (bw <= 8) bad = wave_band_solve<8>(L, ld, bw, n, rhs, x);
+        else if (bw <= 16) bad = wave_band_solve<16>(L, ld, bw, n, rhs, x);
+        else bad = wave_band_solve<24>(L, ld, bw, n, rhs, x);
+        if (tid == 0 && bad) s_bad = 1;
       }
-      for (int i = n - 1; i >= 0; i--) {
-        double s = x[i];
-        const int k1 = i + bw < n - 1 ? i + bw : n - 1;
-        for (int kk = i + 1; kk <= k1; kk++) s -= L[(long long)kk * ld + (i - kk + bw)] * x[kk];
-        x[i] = s / L[(long long)i * ld + bw];
+      __syncthreads();
+    } else {
+      // ---- wide bands: left-looking banded Cholesky by columns ----
+      for (int j = 0; j < n; j++) {
+        if (tid == 0) {
+          double s = L[(long long)j * ld + bw];
+          const int k0 = j - bw > 0 ? j - bw : 0;
+          for (int kk = k0; kk < j; kk++) {
+            const double v = L[(long long)j * ld + (kk - j + bw)];
+            s -= v * v;
+          }
+          if (!(s > 0.0)) s_bad = 1;
+          L[(long long)j * ld + bw] = sqrt(s > 0.0 ? s : 1.0);
+        }
+        __syncthreads();
+        const double djj = L[(long long)j * ld + bw];
+        for (int r = tid; r < bw; r += NT) {
+          const int i = j + 1 + r;
+          if (i >= n) break;
+          double s = L[(long long)i * ld + (j - i + bw)];
+          const int k0 = i - bw > 0 ? i - bw : 0;
+          for (int kk = k0; kk < j; kk++)
+            s -= L[(long long)i * ld + (kk - i + bw)] * L[(long long)j * ld + (kk - j + bw)];
+          L[(long long)i * ld + (j - i + bw)] = s / djj;
+        }
+        __syncthreads();
       }
+      if (tid == 0) {
+        for (int i = 0; i < n; i++) {
+          double s = rhs[i];
+          const int k0 = i - bw > 0 ? i - bw : 0;
+          for (int kk = k0; kk < i; kk++) s -= L[(long long)i * ld + (kk - i + bw)] * x[kk];
+          x[i] = s / L[(long long)i * ld + bw];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+          double s = x[i];
+          const int k1 = i + bw < n - 1 ? i + bw : n - 1;
+          for (int kk = i + 1; kk <= k1; kk++) s -= L[(long long)kk * ld + (i - kk + bw)] * x[kk];
+          x[i] = s / L[(long long)i * ld + bw];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
     if (s_bad) {
       if (tid == 0) E.status[m] = -1;  // BadLinearSolve
       __syncthreads();

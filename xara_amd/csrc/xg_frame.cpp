@@ -4,6 +4,7 @@
 // of xg_frame.hip.
 #include "xg_frame.h"
 
+#include <algorithm>
 #include <cmath>
 
 #include "xg_frame_dev.h"
@@ -16,7 +17,7 @@ struct FrameGroup : Group {
   std::vector<double> rho_h;
   FrameDev f;
   // device storage
-  DVec<int> dconn, dsecdef, ddef_off, dfkind, dfmat, dflocal;
+  DVec<int> dconn, dsecdef, ddef_off, dfkind, dfmat, dflocal, drun_off, drun_beg, drun_kind, drun_mat, ddef_last;
   DVec<double> dgeo, dfy, dfz, dfA, dGJ, de, ds, dks, delasE;
   DVec<long long> dsb, dcb, deb;
   DVec<Steel02Param> dsteelp;
@@ -68,6 +69,7 @@ struct FrameGroup : Group {
     // ---- section definitions ----
     std::map<int, int> def_of;
     std::vector<int> def_off{0}, fkind, fmat, flocal, secdef(ne);
+    std::vector<int> run_off{0}, run_beg, run_kind, run_mat, def_last;
     std::vector<double> fy, fz, fA, GJ;
     std::vector<int> nsteel_def, nconc_def, nelas_def;
     std::map<int, int> steel_idx, conc_idx, elas_idx;
@@ -102,23 +104,47 @@ struct FrameGroup : Group {
             yBar = QzBar / ABar;
             zBar = QyBar / ABar;
           }
+        // pool slot of every fiber in definition order
+        const int nfd = (int)sd.A.size();
         int ns = 0, nc = 0, nl = 0;
-        for (size_t i = 0; i < sd.A.size(); i++) {
+        std::vector<int> mat_of(nfd), local_of(nfd);
+        for (int i = 0; i < nfd; i++) {
+          if (sd.kind[i] == 1) {
+            mat_of[i] = steel_idx.at(sd.mat[i]);
+            local_of[i] = ns++;
+          } else if (sd.kind[i] == 2) {
+            mat_of[i] = conc_idx.at(sd.mat[i]);
+            local_of[i] = nc++;
+          } else {
+            mat_of[i] = elas_idx.at(sd.mat[i]);
+            local_of[i] = nl++;
+          }
+        }
+        // device order: runs of one (kind, material), definition order within
+        // a run, so a wavefront pass updates a single material
+        std::vector<int> order(nfd);
+        for (int i = 0; i < nfd; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+          return sd.kind[a] != sd.kind[b] ? sd.kind[a] < sd.kind[b] : mat_of[a] < mat_of[b];
+        });
+        const int base = (int)fA.size();
+        for (int k = 0; k < nfd; k++) {
+          const int i = order[k];
+          if (k == 0 || sd.kind[i] != sd.kind[order[k - 1]] || mat_of[i] != mat_of[order[k - 1]]) {
+            run_beg.push_back(base + k);
+            run_kind.push_back(sd.kind[i]);
+            run_mat.push_back(mat_of[i]);
+          }
+          if (i == nfd - 1) def_last.push_back(k);
           fy.push_back(sd.y[i] - yBar);
           fz.push_back(d3 ? sd.z[i] - zBar : 0.0);
           fA.push_back(sd.A[i]);
           fkind.push_back(sd.kind[i]);
-          if (sd.kind[i] == 1) {
-            fmat.push_back(steel_idx.at(sd.mat[i]));
-            flocal.push_back(ns++);
-          } else if (sd.kind[i] == 2) {
-            fmat.push_back(conc_idx.at(sd.mat[i]));
-            flocal.push_back(nc++);
-          } else {
-            fmat.push_back(elas_idx.at(sd.mat[i]));
-            flocal.push_back(nl++);
-          }
+          fmat.push_back(mat_of[i]);
+          flocal.push_back(local_of[i]);
         }
+        if (nfd == 0) def_last.push_back(-1);
+        run_off.push_back((int)run_beg.size());
         nsteel_def.push_back(ns);
         nconc_def.push_back(nc);
         nelas_def.push_back(nl);
@@ -184,6 +210,12 @@ struct FrameGroup : Group {
     dfkind.upload(fkind, s);
     dfmat.upload(fmat, s);
     dflocal.upload(flocal, s);
+    run_beg.push_back((int)fA.size());
+    drun_off.upload(run_off, s);
+    drun_beg.upload(run_beg, s);
+    drun_kind.upload(run_kind, s);
+    drun_mat.upload(run_mat, s);
+    ddef_last.upload(def_last, s);
     dGJ.upload(GJ, s);
     dsb.upload(sb, s);
     dcb.upload(cb, s);
@@ -221,6 +253,11 @@ struct FrameGroup : Group {
     f.fkind = dfkind.p;
     f.fmat = dfmat.p;
     f.flocal = dflocal.p;
+    f.run_off = drun_off.p;
+    f.run_beg = drun_beg.p;
+    f.run_kind = drun_kind.p;
+    f.run_mat = drun_mat.p;
+    f.def_last = ddef_last.p;
     f.GJ = dGJ.p;
     f.steel_base = dsb.p;
     f.conc_base = dcb.p;

@@ -19,6 +19,14 @@ struct FrameDev {
   const int* fkind = nullptr;     // 1 steel02, 2 concrete02, 3 elastic
   const int* fmat = nullptr;      // index into the parameter table of its kind
   const int* flocal = nullptr;    // index within its kind in the section
+  // fibers of a definition are stored in runs of one (kind, material):
+  // definition d owns runs run_off[d] .. run_off[d+1]-1, run r the fibers
+  // run_beg[r] .. run_beg[r+1]-1
+  const int* run_off = nullptr;
+  const int* run_beg = nullptr;
+  const int* run_kind = nullptr;
+  const int* run_mat = nullptr;
+  const int* def_last = nullptr;  // position of the definition's last fiber (definition order)
   const double* GJ = nullptr;     // per definition, < 0 for none
   // per section instance
   const long long* steel_base = nullptr;

@@ -78,57 +78,54 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
-// One fiber: committed state -> trial state; returns stress and tangent.
-__device__ __forceinline__ void fiber_trial(const FrameDev& f, int kind, int mat, long long idx, double strain,
+// One Steel02 fiber: committed state -> trial state; returns stress and
+// tangent.
+__device__ __forceinline__ void steel_fiber(const FrameDev& f, const Steel02Param& p, long long idx, double strain,
                                             double& stress, double& tangent) {
-  if (kind == 1) {
-    const Steel02Param p = f.steelp[mat];
-    const long long n = f.nsteel;
-    Steel02State c, t;
-    c.epsmin = f.st_c[0 * n + idx];
-    c.epsmax = f.st_c[1 * n + idx];
-    c.epspl = f.st_c[2 * n + idx];
-    c.epss0 = f.st_c[3 * n + idx];
-    c.sigs0 = f.st_c[4 * n + idx];
-    c.epsr = f.st_c[5 * n + idx];
-    c.sigr = f.st_c[6 * n + idx];
-    c.eps = f.st_c[7 * n + idx];
-    c.sig = f.st_c[8 * n + idx];
-    c.e = f.st_c[9 * n + idx];
-    c.kon = f.kon_c[idx];
-    steel02_trial(p, c, strain, t);
-    f.st_t[0 * n + idx] = t.epsmin;
-    f.st_t[1 * n + idx] = t.epsmax;
-    f.st_t[2 * n + idx] = t.epspl;
-    f.st_t[3 * n + idx] = t.epss0;
-    f.st_t[4 * n + idx] = t.sigs0;
-    f.st_t[5 * n + idx] = t.epsr;
-    f.st_t[6 * n + idx] = t.sigr;
-    f.st_t[7 * n + idx] = t.eps;
-    f.st_t[8 * n + idx] = t.sig;
-    f.st_t[9 * n + idx] = t.e;
-    f.kon_t[idx] = t.kon;
-    stress = t.sig;
-    tangent = t.e;
-  } else if (kind == 2) {
-    const Concrete02Param p = f.concp[mat];
-    const long long n = f.nconc;
-    double ecmin, dept, eps, sig = f.cs_t[3 * n + idx], e = f.cs_t[4 * n + idx];
-    concrete02_trial(p, f.cs_c[0 * n + idx], f.cs_c[1 * n + idx], f.cs_c[2 * n + idx], f.cs_c[3 * n + idx],
-                     strain, ecmin, dept, eps, sig, e);
-    f.cs_t[0 * n + idx] = ecmin;
-    f.cs_t[1 * n + idx] = dept;
-    f.cs_t[2 * n + idx] = eps;
-    f.cs_t[3 * n + idx] = sig;
-    f.cs_t[4 * n + idx] = e;
-    stress = sig;
-    tangent = e;
-  } else {
-    const double E = f.elasE[mat];
-    f.el_t[idx] = strain;
-    stress = E * strain;
-    tangent = E;
-  }
+  const long long n = f.nsteel;
+  Steel02State c, t;
+  c.epsmin = f.st_c[0 * n + idx];
+  c.epsmax = f.st_c[1 * n + idx];
+  c.epspl = f.st_c[2 * n + idx];
+  c.epss0 = f.st_c[3 * n + idx];
+  c.sigs0 = f.st_c[4 * n + idx];
+  c.epsr = f.st_c[5 * n + idx];
+  c.sigr = f.st_c[6 * n + idx];
+  c.eps = f.st_c[7 * n + idx];
+  c.sig = f.st_c[8 * n + idx];
+  c.e = f.st_c[9 * n + idx];
+  c.kon = f.kon_c[idx];
+  steel02_trial(p, c, strain, t);
+  f.st_t[0 * n + idx] = t.epsmin;
+  f.st_t[1 * n + idx] = t.epsmax;
+  f.st_t[2 * n + idx] = t.epspl;
+  f.st_t[3 * n + idx] = t.epss0;
+  f.st_t[4 * n + idx] = t.sigs0;
+  f.st_t[5 * n + idx] = t.epsr;
+  f.st_t[6 * n + idx] = t.sigr;
+  f.st_t[7 * n + idx] = t.eps;
+  f.st_t[8 * n + idx] = t.sig;
+  f.st_t[9 * n + idx] = t.e;
+  f.kon_t[idx] = t.kon;
+  stress = t.sig;
+  tangent = t.e;
+}
+
+// One Concrete02 fiber (the previous trial sig / e are kept on the early
+// return of setTrialStrain).
+__device__ __forceinline__ void concrete_fiber(const FrameDev& f, const Concrete02Param& p, long long idx,
+                                               double strain, double& stress, double& tangent) {
+  const long long n = f.nconc;
+  double ecmin, dept, eps, sig = f.cs_t[3 * n + idx], e = f.cs_t[4 * n + idx];
+  concrete02_trial(p, f.cs_c[0 * n + idx], f.cs_c[1 * n + idx], f.cs_c[2 * n + idx], f.cs_c[3 * n + idx], strain,
+                   ecmin, dept, eps, sig, e);
+  f.cs_t[0 * n + idx] = ecmin;
+  f.cs_t[1 * n + idx] = dept;
+  f.cs_t[2 * n + idx] = eps;
+  f.cs_t[3 * n + idx] = sig;
+  f.cs_t[4 * n + idx] = e;
+  stress = sig;
+  tangent = e;
 }
 
 // Section deformation at the section's Gauss point from nodal
@@ -151,31 +148,21 @@ __device__ __forceinline__ void section_deformation(const FrameDev& f, long long
   }
 }
 
-// Section update, one wavefront per section instance.
-template <bool D3>
-__device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
-  const int e = (int)(sec / f.nip);
-  double d[4];
-  section_deformation<D3>(f, sec, U, d);
-  const int def = f.secdef[e];
-  const int f0 = f.def_off[def], nf = f.def_off[def + 1] - f0;
-  const long long sb = f.steel_base[sec], cb = f.conc_base[sec], eb = f.elas_base[sec];
+__device__ __forceinline__ int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double uniform_d(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)bits);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(bits >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Section resultant and tangent sums of one section (kData / sData terms of
+// FiberSection2d/3d::setTrialSectionDeformation)
+struct SecSums {
   double k0 = 0, k1 = 0, k2 = 0, k5 = 0, k6 = 0, k10 = 0, s0 = 0, s1 = 0, s2 = 0;
-  for (int i = lane; i < nf; i += 64) {
-    const int fi = f0 + i;
-    const double y = f.fy[fi], A = f.fA[fi];
-    const int kind = f.fkind[fi];
-    const long long base = kind == 1 ? sb : (kind == 2 ? cb : eb);
-    double strain;
+  template <bool D3>
+  __device__ __forceinline__ void add(double y, double z, double A, double stress, double tangent) {
     if (D3) {
-      const double z = f.fz[fi];
-      strain = d[0] - y * d[1] + z * d[2];
-    } else
-      strain = d[0] - y * d[1];
-    double stress, tangent;
-    fiber_trial(f, kind, f.fmat[fi], base + f.flocal[fi], strain, stress, tangent);
-    if (D3) {
-      const double z = f.fz[fi];
       const double EA = tangent * A;
       k0 += EA;
       k1 += -y * EA;
@@ -198,6 +185,63 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
       s1 += fs0 * -y;
     }
   }
+};
+
+// Section update, one wavefront per section instance (sec is the same for
+// every lane of the wave).  KINDS selects the fiber kinds updated (bit 0
+// Steel02, bit 1 Concrete02, bit 2 Elastic); with ACCUM the sums are added
+// to the resultant and tangent already stored for the section (a second
+// pass over the remaining kinds).
+constexpr int kKindSteel = 1, kKindConc = 2, kKindElas = 4, kKindAll = 7;
+template <bool D3, int KINDS = kKindAll, bool ACCUM = false>
+__device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
+  const int e = (int)(sec / f.nip);
+  double d[4];
+  section_deformation<D3>(f, sec, U, d);
+  const int def = uniform_i(f.secdef[e]);
+  const long long sb = f.steel_base[sec], cb = f.conc_base[sec], eb = f.elas_base[sec];
+  SecSums S;
+  // one run of a single (kind, material) at a time: wave-uniform
+  // parameters and no divergence between material kinds inside a pass
+  const int r0 = uniform_i(f.run_off[def]), r1 = uniform_i(f.run_off[def + 1]);
+  for (int r = r0; r < r1; r++) {
+    const int kind = uniform_i(f.run_kind[r]), mat = uniform_i(f.run_mat[r]);
+    const int b0 = uniform_i(f.run_beg[r]), b1 = uniform_i(f.run_beg[r + 1]);
+    if (!(KINDS & (kind == 1 ? kKindSteel : (kind == 2 ? kKindConc : kKindElas)))) continue;
+    if (kind == 1) {
+      const Steel02Param* q = f.steelp + mat;
+      const Steel02Param p{uniform_d(q->Fy), uniform_d(q->E0),  uniform_d(q->b),  uniform_d(q->R0),
+                           uniform_d(q->cR1), uniform_d(q->cR2), uniform_d(q->a1), uniform_d(q->a2),
+                           uniform_d(q->a3), uniform_d(q->a4),  uniform_d(q->sigini)};
+      for (int fi = b0 + lane; fi < b1; fi += 64) {
+        const double y = f.fy[fi], z = D3 ? f.fz[fi] : 0.0;
+        const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
+        double stress, tangent;
+        steel_fiber(f, p, sb + f.flocal[fi], strain, stress, tangent);
+        S.add<D3>(y, z, f.fA[fi], stress, tangent);
+      }
+    } else if (kind == 2) {
+      const Concrete02Param* q = f.concp + mat;
+      const Concrete02Param p{uniform_d(q->fc),    uniform_d(q->epsc0), uniform_d(q->fcu), uniform_d(q->epscu),
+                              uniform_d(q->rat),   uniform_d(q->ft),    uniform_d(q->Ets)};
+      for (int fi = b0 + lane; fi < b1; fi += 64) {
+        const double y = f.fy[fi], z = D3 ? f.fz[fi] : 0.0;
+        const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
+        double stress, tangent;
+        concrete_fiber(f, p, cb + f.flocal[fi], strain, stress, tangent);
+        S.add<D3>(y, z, f.fA[fi], stress, tangent);
+      }
+    } else {
+      const double E = uniform_d(f.elasE[mat]);
+      for (int fi = b0 + lane; fi < b1; fi += 64) {
+        const double y = f.fy[fi], z = D3 ? f.fz[fi] : 0.0;
+        const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
+        f.el_t[eb + f.flocal[fi]] = strain;
+        S.add<D3>(y, z, f.fA[fi], E * strain, E);
+      }
+    }
+  }
+  double k0 = S.k0, k1 = S.k1, k2 = S.k2, k5 = S.k5, k6 = S.k6, k10 = S.k10, s0 = S.s0, s1 = S.s1, s2 = S.s2;
   k0 = wave_sum(k0);
   k1 = wave_sum(k1);
   k5 = wave_sum(k5);
@@ -214,10 +258,17 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
     if (D3) {
       double kd[16] = {k0, k1, k2, 0, k1, k5, k6, 0, k2, k6, k10, 0, 0, 0, 0, 0};
       double sd[4] = {s0, s1, s2, 0.0};
-      const double GJ = f.GJ[def];
-      if (GJ >= 0.0) {
-        sd[3] = GJ * d[3];
-        kd[15] = GJ;
+      if (ACCUM) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) kd[c] += f.ks[c * n + sec];
+#pragma unroll
+        for (int c = 0; c < 4; c++) sd[c] += f.s[c * n + sec];
+      } else {
+        const double GJ = f.GJ[def];
+        if (GJ >= 0.0) {
+          sd[3] = GJ * d[3];
+          kd[15] = GJ;
+        }
       }
 #pragma unroll
       for (int c = 0; c < 16; c++) f.ks[c * n + sec] = kd[c];
@@ -227,6 +278,13 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
         f.e[c * n + sec] = d[c];
       }
     } else {
+      if (ACCUM) {
+        k0 += f.ks[0 * n + sec];
+        k1 += f.ks[1 * n + sec];
+        k5 += f.ks[3 * n + sec];
+        s0 += f.s[0 * n + sec];
+        s1 += f.s[1 * n + sec];
+      }
       f.ks[0 * n + sec] = k0;
       f.ks[1 * n + sec] = k1;
       f.ks[2 * n + sec] = k1;
@@ -299,7 +357,7 @@ __device__ __forceinline__ void section_revert(const FrameDev& f, long long sec,
       k0 += ks0;
       k1 += ks1;
       k5 += ks1 * -y;
-      if (i == nf - 1) {
+      if (i == f.def_last[def]) {  // the definition's last fiber
         last_s0 = stress * A;
         last_s1 = last_s0 * -y;
       }
