@@ -9,6 +9,7 @@
 //
 // Dot products reduce per block with a fixed LDS tree and then over blocks
 // in a single-block pass, so every PCG iteration is bitwise reproducible.
+#include <algorithm>
 #include "xg_common.h"
 #include "xg_kernels.h"
 
@@ -130,19 +131,24 @@ __global__ __launch_bounds__(256) void k_spmv_dot(int n, const int* __restrict__
   __shared__ double sh[256];
   if (stop && stop[0]) return;
   const int t = threadIdx.x;
-  const int row = blockIdx.x * (256 / L) + t / L;
   const int lane = t % L;
-  double sum = 0.0;
-  if (row < n) {
-    const int b = rowptr[row], e = rowptr[row + 1];
-    for (int k = b + lane; k < e; k += L) sum += vals[k] * x[colind[k]];
-  }
-#pragma unroll
-  for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
+  constexpr int R = 256 / L;  // rows per block and sweep
   double d = 0.0;
-  if (row < n && lane == 0) {
-    y[row] = sum;
-    d = sum * x[row];
+  // grid-stride over row groups: a fixed assignment, so the partial of each
+  // block (and the dot product) does not depend on timing
+  for (int row0 = blockIdx.x * R; row0 < n; row0 += gridDim.x * R) {
+    const int row = row0 + t / L;
+    double sum = 0.0;
+    if (row < n) {
+      const int b = rowptr[row], e = rowptr[row + 1];
+      for (int k = b + lane; k < e; k += L) sum += vals[k] * x[colind[k]];
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
+    if (row < n && lane == 0) {
+      y[row] = sum;
+      d += sum * x[row];
+    }
   }
   if (partial) {
     const double bs = block_sum<256>(d, sh);
@@ -194,9 +200,8 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, const CGScal* __restri
   __shared__ double sh[256];
   if (s->converged) return;
   const double alpha = s->alpha;
-  const int i = blockIdx.x * 256 + threadIdx.x;
   double rz = 0.0, rr = 0.0;
-  if (i < n) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     x[i] += alpha * p[i];
     const double ri = r_in[i] - alpha * q[i];
     r_out[i] = ri;
@@ -212,8 +217,8 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, const CGScal* __restri
       }
     }
     z[i] = zi;
-    rz = ri * zi;
-    rr = ri * ri;
+    rz += ri * zi;
+    rr += ri * ri;
   }
   const double s1 = block_sum<256>(rz, sh);
   const double s2 = block_sum<256>(rr, sh);
@@ -246,9 +251,8 @@ __global__ __launch_bounds__(256) void k_cg_beta(int nb, const double* __restric
 __global__ void k_cg_newp(int n, const CGScal* __restrict__ s, const double* __restrict__ z,
                           double* __restrict__ p) {
   if (s->converged) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  p[i] = z[i] + s->beta * p[i];
+  const double beta = s->beta;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
 }
 
 // Start: x = 0, r = b, z = M^{-1} b, p = z; partials of r.z, b.b
@@ -452,7 +456,11 @@ void launch_add_diag(hipStream_t s, int n, const long long* pos, const double* m
   if (n <= 0) return;
   hipLaunchKernelGGL(k_add_diag, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, pos, m, coef, vals);
 }
-int spmv_blocks(int n, int lanes) { return ceil_div(n, 256 / lanes); }
+// grid of the CG kernels: enough workgroups to fill the chip, few enough
+// that the single-workgroup reductions of their partials stay short
+constexpr int kCgMaxBlocks = 2048;
+int spmv_blocks(int n, int lanes) { return std::min(ceil_div(n, 256 / lanes), kCgMaxBlocks); }
+static int vec_blocks(int n) { return std::min(ceil_div(n, 256), kCgMaxBlocks); }
 void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
                  const double* x, double* y, double* partial, const int* stop) {
   const dim3 g(spmv_blocks(n, lanes)), b(256);
@@ -506,7 +514,7 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
                    const unsigned char* bsize, double* part, void* scal) {
   CGScal* S = (CGScal*)scal;
   const int nbs = spmv_blocks(n, lanes);
-  const int nbv = ceil_div(n, 256);
+  const int nbv = vec_blocks(n);
   const int* stop = &S->converged;
   for (int it = 0; it < iters; it++) {
     launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop);
