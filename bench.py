@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run the 256 MiB copy kernel (profiler counter calibration)")
-    ap.add_argument("--mode", choices=["update", "newton", "frame", "ensemble", "adjoint"], default="update")
+    ap.add_argument("--mode", choices=["update", "newton", "quad", "frame", "ensemble", "adjoint"], default="update")
     ap.add_argument("--adj-frame", default="5,5,10", help="bays x, bays y, stories of the adjoint mode")
     ap.add_argument("--adj-steps", type=int, default=500)
     ap.add_argument("--ckpt", type=int, default=25)
@@ -99,6 +99,8 @@ def main():
         return newton_main(args)
     if args.mode == "frame":
         return frame_main(args)
+    if args.mode == "quad":
+        return quad_main(args)
     if args.mode == "ensemble":
         return ensemble_main(args)
     if args.mode == "adjoint":
@@ -227,6 +229,57 @@ def newton_main(args):
                    "cg_iters": c["cg_iters"] - c0["cg_iters"], "solves": c["solves"] - c0["solves"],
                    "plastic_fraction": round(plastic, 3), "setup_s": round(setup_s, 2),
                    "p_y": p_y, "final_load_factor": m.load_factor},
+    }
+    print(json.dumps(out))
+
+
+def quad_main(args):
+    """Plane-strain FourNodeQuad J2 strip 64x16 (20 x 5), clamped at x = 0,
+    unit vertical reference load at the free-end top node,
+    DisplacementControl of that node (dof 2, 0.001 per step, 50 steps),
+    Newton + NormUnbalance 1e-8.  Newton iterations/s of the GPU path and of
+    the CPU oracle (one core, direct solve) on the same pushover, and the
+    largest relative difference of their converged displacements."""
+    from tests import _oracle
+    from xara_amd import meshes
+    from xara_amd.model import Model
+
+    mesh = meshes.quad_strip(64, 16)
+    nsteps = 50
+
+    def define(m):
+        meshes.build_quad_strip(m, mesh)
+        m.load(mesh["tip"], 0.0, 1.0)
+        m.test("NormUnbalance", 1e-8, 25)
+        m.integrator("DisplacementControl", mesh["tip"], 2, 0.001)
+
+    res = {}
+    for name, m in (("gpu", Model(2, 2)), ("cpu", _oracle.oracle_model(2, 2))):
+        define(m)
+        if name == "gpu":
+            m.system(args.system, 1e-12, 100000)
+        m.setup()
+        c0 = m.counters()
+        t0 = time.perf_counter()
+        traj = []
+        for _ in range(nsteps):
+            if m.analyze(1) != 0:
+                break
+            traj.append(m.trial_disp().copy())
+        wall = time.perf_counter() - t0
+        c = m.counters()
+        res[name] = (wall, c["newton_iters"] - c0["newton_iters"], np.array(traj))
+        print(f"bench: quad {name} {wall:.2f}s newton={res[name][1]}", file=sys.stderr, flush=True)
+    (gw, gi, gt), (cw, ci, ct) = res["gpu"], res["cpu"]
+    rel = float(np.max(np.abs(gt - ct)) / np.max(np.abs(ct))) if len(gt) == len(ct) else None
+    out = {
+        "metric": "Newton iterations/s", "value": gi / gw, "unit": "iterations/s", "n_gpus": 1,
+        "steps": nsteps, "warmup": 0, "ms_per_step": 1e3 * gw / nsteps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "FourNodeQuad J2 plane-strain strip 64x16, DisplacementControl pushover to 0.05",
+                   "newton_iters": gi, "system": args.system, "max_rel_diff_vs_oracle": rel},
+        "cpu_baseline": {"value": ci / cw, "unit": "iterations/s", "cores": 1, "kind": "port",
+                         "sample": "the whole pushover, oracle direct solve"},
     }
     print(json.dumps(out))
 
