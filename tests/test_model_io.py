@@ -1,0 +1,161 @@
+"""Models read from the reference's ``print -json`` format define the same
+model as the direct commands: identical numbering, CSR pattern and state
+determination (checked on the CPU oracle, which shares the model API).
+The documents are written here by the rules of the reference's Print
+methods (field names, quoted or bare names, 2D fibers as [y, 0.0],
+LinearCrdTransf3d's local z axis as vecInLocXZPlane)."""
+import json
+
+import numpy as np
+import pytest
+
+from tests import _oracle
+from xara_amd import meshes, model_io
+from xara_amd.model import XaraError
+
+
+def node_json(tags, crds, ndf, mass=None):
+    out = []
+    for k, (t, c) in enumerate(zip(tags, crds)):
+        n = {"name": int(t), "ndf": ndf, "crd": [float(x) for x in c]}
+        if mass is not None and k in mass:
+            n["mass"] = mass[k]
+        out.append(n)
+    return out
+
+
+def sp_json(fixed, ndf):
+    out, k = [], 1
+    for t in fixed:
+        for dof in range(1, ndf + 1):
+            out.append({"name": k, "node": int(t), "dof": dof, "ref_value": 0})
+            k += 1
+    return out
+
+
+def j2_json(tag, mat):
+    return {"name": tag, "type": "J2Plasticity", "G": mat["G"], "K": mat["K"], "Fy": mat["sig0"],
+            "Fs": mat["sigInf"], "Hsat": mat["delta"], "Hiso": mat["H"], "eta": 0.0, "density": 0.0}
+
+
+def steel_json(tag, fy, E, b, R0, cR1, cR2, a1=0.0, a2=1.0, a3=0.0, a4=1.0):
+    return {"name": tag, "type": "Steel02", "E": E, "fy": fy, "b": b, "R0": R0, "cR1": cR1, "cR2": cR2,
+            "a1": a1, "a2": a2, "a3": a3, "a4": a4, "sigini": 0}
+
+
+def conc_json(tag, fc, epsc0, fcu, epscu, rat, ft, Ets):
+    return {"name": str(tag), "type": "Concrete02", "Ec": 2 * fc / epsc0, "fc": fc, "epsc": epsc0, "fcu": fcu,
+            "epscu": epscu, "ratio": rat, "ft": ft, "Ets": Ets}
+
+
+def fiber_section_json(tag, rows, three_d, GJ=None):
+    s = {"name": str(tag), "type": "FiberSection3d" if three_d else "FiberSection2d"}
+    if three_d and GJ is not None:
+        s["torsion"] = GJ
+    s["fibers"] = [{"coord": [y, z if three_d else 0.0], "area": A, "material": int(m)} for y, z, A, m in rows]
+    return s
+
+
+def doc(props, geom):
+    return json.dumps({"StructuralAnalysisModel": {"properties": props, "geometry": geom}})
+
+
+def same_model(a, b, npts, U, what="stress"):
+    assert np.array_equal(a.node_eqns(), b.node_eqns())
+    ra, ca = a.csr_pattern()
+    rb, cb = b.csr_pattern()
+    assert np.array_equal(ra, rb) and np.array_equal(ca, cb)
+    for m in (a, b):
+        m.set_trial_disp(U)
+        m.update()
+    assert np.array_equal(a.point_data(what, npts), b.point_data(what, npts))
+    assert np.array_equal(a.form_unbalance(), b.form_unbalance())
+
+
+def test_brick_cube_from_json():
+    mesh = meshes.brick_cube(3)
+    direct = _oracle.oracle_model(3, 3)
+    meshes.build_brick_cube(direct, mesh, pressure=0.0)
+    direct.setup()
+    d = doc({"nDMaterials": [j2_json(1, meshes.J2_STEEL)], "uniaxialMaterials": [], "sections": [],
+             "crdTransformations": []},
+            {"nodes": node_json(mesh["node_tags"], mesh["crds"], 3),
+             "elements": [{"name": int(t), "type": "Brick", "nodes": [int(x) for x in c], "bodyForces": [0, 0, 0],
+                           "material": [1]} for t, c in zip(mesh["ele_tags"], mesh["conn"])],
+             "constraints": sp_json(mesh["fixed_nodes"], 3)})
+    loaded = model_io.load_json(_oracle.oracle_model(3, 3), d)
+    loaded.setup()
+    U = 2e-3 * np.sin(np.arange(3 * len(mesh["node_tags"])).reshape(-1, 3))
+    same_model(loaded, direct, 8 * 27, U)
+
+
+def test_quad_strip_from_json():
+    mesh = meshes.quad_strip(8, 2)
+    direct = _oracle.oracle_model(2, 2)
+    meshes.build_quad_strip(direct, mesh)
+    direct.setup()
+    d = doc({"nDMaterials": [j2_json(1, meshes.J2_STEEL)]},
+            {"nodes": node_json(mesh["node_tags"], mesh["crds"], 2),
+             "elements": [{"name": int(t), "type": "FourNodeQuad", "nodes": [int(x) for x in c], "thickness": 1.0,
+                           "surfacePressure": 0, "density": 0, "bodyForces": [0, 0], "materials": [1, 1, 1, 1]}
+                          for t, c in zip(mesh["ele_tags"], mesh["conn"])],
+             "constraints": sp_json(mesh["fixed_nodes"], 2)})
+    loaded = model_io.load_json(_oracle.oracle_model(2, 2), d)
+    loaded.setup()
+    U = 1e-2 * np.cos(np.arange(2 * len(mesh["node_tags"])).reshape(-1, 2))
+    same_model(loaded, direct, 4 * 16, U)
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+def test_rc_frame_from_json(three_d):
+    fr = meshes.rc_frame_3d(1, 1, 2) if three_d else meshes.rc_frame_2d(2, 2)
+    ndm, ndf = (3, 6) if three_d else (2, 3)
+    conn, transf = meshes.frame_elements(fr)
+    theta = meshes.default_theta(len(conn))
+    theta[:, 0] *= np.linspace(0.9, 1.1, len(conn))
+    direct = _oracle.oracle_model(ndm, ndf)
+    meshes.build_rc_frame_per_element(direct, fr, theta)
+    floor = fr["load_tags"] if three_d else fr["floor_nodes"]
+    m = np.zeros((len(floor), ndf))
+    m[:, 0] = 60.0
+    direct.masses(floor, m)
+    direct.setup()
+    index = {int(t): k for k, t in enumerate(fr["node_tags"])}
+    mass = {index[int(t)]: list(row) for t, row in zip(floor, m)}
+    uni, secs = [], []
+    for e in range(len(conn)):
+        tag = e + 1
+        fy, E, b, fc = theta[e]
+        uni.append(steel_json(meshes.STEEL_TAG0 + tag, fy, E, b, *meshes.STEEL02[3:]))
+        uni.append(conc_json(meshes.CONC_TAG0 + tag, fc, *meshes.CONCRETE02[1:]))
+        rows = meshes.rc_section_fibers(conc_tag=meshes.CONC_TAG0 + tag, steel_tag=meshes.STEEL_TAG0 + tag)
+        secs.append(fiber_section_json(tag, rows, three_d, 1.1e14))
+    if three_d:
+        # LinearCrdTransf3d prints its local z axis: columns (axis z, vecxz x) -> (1, 0, 0) ... beams (0, 0, 1)
+        transfs = [{"name": "1", "type": "LinearCrdTransf3d", "vecInLocXZPlane": [1.0, 0.0, 0.0]},
+                   {"name": "2", "type": "LinearCrdTransf3d", "vecInLocXZPlane": [0.0, 0.0, 1.0]}]
+    else:
+        transfs = [{"name": "1", "type": "LinearCrdTransf2d"}]
+    kind = "DispBeamColumn3d" if three_d else "DispBeamColumn2d"
+    eles = [{"name": e + 1, "type": kind, "nodes": [int(conn[e, 0]), int(conn[e, 1])],
+             "sections": [str(e + 1)] * 5, "integration": {"type": "Legendre"}, "massperlength": 0,
+             "crdTransformation": str(int(transf[e]))} for e in range(len(conn))]
+    d = doc({"sections": secs, "nDMaterials": [], "uniaxialMaterials": uni, "crdTransformations": transfs},
+            {"nodes": node_json(fr["node_tags"], fr["crds"], ndf, mass), "elements": eles,
+             "constraints": sp_json(fr["fixed_nodes"], ndf)})
+    loaded = model_io.load_json(_oracle.oracle_model(ndm, ndf), d)
+    loaded.setup()
+    rng = np.random.default_rng(1)
+    U = rng.normal(size=(len(fr["node_tags"]), ndf)) * 5.0
+    U[:, 3 if three_d else 2:] *= 1e-3
+    same_model(loaded, direct, 5 * len(conn), U, "tangent")
+
+
+def test_unsupported_objects_are_refused():
+    base = {"nodes": [{"name": 1, "ndf": 2, "crd": [0.0, 0.0]}]}
+    for props, geom in [({"nDMaterials": [{"name": 1, "type": "ElasticIsotropic"}]}, {}),
+                        ({"uniaxialMaterials": [{"name": 1, "type": "Steel01"}]}, {}),
+                        ({}, {"elements": [{"name": 1, "type": "Truss", "nodes": [1, 1]}]}),
+                        ({}, {"constraints": [{"name": 1, "node": 1, "dof": 1, "ref_value": 0.5}]})]:
+        with pytest.raises(XaraError):
+            model_io.load_json(_oracle.oracle_model(2, 2), doc(props, dict(base, **geom)))

@@ -89,8 +89,9 @@ class Model:
     def nodes(self, tags, crds):
         tags = _i32(tags)
         crds = _f64(crds).reshape(len(tags), self.ndm)
-        self._node_tags = None
         self._check(self._b.nodes(self._h, len(tags), _i(tags), _d(crds)))
+        known = self._node_tags if self._node_tags is not None else np.zeros(0, np.int32)
+        self._node_tags = np.union1d(known, tags)  # ascending: the order of nodal results
 
     def fix(self, tag, *flags):
         self.fix_many([tag], [flags])
@@ -243,7 +244,6 @@ class Model:
     # -- setup and numbering ---------------------------------------------
     def setup(self):
         self._check(self._b.setup(self._h))
-        self._node_tags = None
 
     @property
     def neq(self):
