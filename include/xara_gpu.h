@@ -245,6 +245,33 @@ int  xg_ensemble_model_disp(xg_ensemble* e, int model, double* U, int* nnode);
 /* per-member status (0/1 running, -2 stopped) and Newton iteration totals */
 int  xg_ensemble_status(xg_ensemble* e, int* status, long long* newton_iters);
 
+/* ---- Committed state (checkpoint / restart, hand-over between models) -----
+   The committed state in the reference's sendSelf layouts, which define it
+   (FE_Datastore save/restore of MovableObjects):
+     XG_STATE_NODES       per node (ascending tag): committed displacement,
+                          velocity, acceleration (ndf each; Node.cpp:767-791)
+     XG_STATE_J2          per Gauss point (elements ascending, points in
+                          element order): xi_n, epsilon_p_n row by row (9)
+                          (J2Plasticity::sendSelf, J2Plasticity.cpp:597-614)
+     XG_STATE_STEEL02     per Steel02 fiber (elements ascending, sections,
+                          fibers in definition order): epsminP, epsmaxP,
+                          epsplP, epss0P, sigs0P, epssrP, sigsrP, konP, epsP,
+                          sigP, eP (Steel02::sendSelf, Steel02.cpp:287-308)
+     XG_STATE_CONCRETE02  per Concrete02 fiber: ecminP, deptP, epsP, sigP, eP
+                          (Concrete02::sendSelf, Concrete02.cpp:339-354)
+     XG_STATE_DOMAIN      committed time and load factor (2 values)
+   xg_set_state writes the committed state and sets the trial state equal
+   to it, as after revertToLastCommit; stresses and tangents follow at the
+   next update. */
+#define XG_STATE_NODES      1
+#define XG_STATE_J2         2
+#define XG_STATE_STEEL02    3
+#define XG_STATE_CONCRETE02 4
+#define XG_STATE_DOMAIN     5
+long long xg_state_size(xg_domain* d, int kind);   /* doubles; < 0 on error */
+int  xg_get_state(xg_domain* d, int kind, double* out);
+int  xg_set_state(xg_domain* d, int kind, const double* in);
+
 /* ---- Adjoint sensitivity of a frame transient ----------------------------
    Gradients of
        loss = sum over the steps n of a forward run of sum_i U_n(node_i, dof)^2

@@ -1565,4 +1565,141 @@ int orc_material_jvp(int kind, const double* prm, const double* h, int kon, doub
   return fail(XG_ERR_ARG, "material_jvp: kind must be 1 (Steel02) or 2 (Concrete02)");
 }
 
+
+// ---- committed state in the reference's sendSelf layouts (see xara_gpu.h) ----
+static void j2_points(Dom& d, std::vector<J2Point*>& pts) {
+  for (auto* e : d.eles) {
+    if (auto* b = dynamic_cast<OBrick*>(e))
+      for (auto& p : b->pts) pts.push_back(&p);
+    else if (auto* q = dynamic_cast<OQuad*>(e))
+      for (auto& p : q->pts) pts.push_back(&p);
+  }
+}
+static void fibers(Dom& d, int kind, std::vector<FiberState*>& out) {
+  for (auto* e : d.eles)
+    if (auto* be = dynamic_cast<OBeam*>(e))
+      for (auto& sec : be->fe.secs)
+        for (size_t f = 0; f < sec.fs.size(); f++)
+          if (sec.def->fibers[f].kind == kind) out.push_back(&sec.fs[f]);
+}
+
+long long orc_state_size(xg_domain* d, int kind) {
+  if (!d->ready && setup(*d) < 0) return XG_ERR_STATE;
+  if (kind == XG_STATE_NODES) return (long long)d->nodes.size() * d->ndf * 3;
+  if (kind == XG_STATE_DOMAIN) return 2;
+  if (kind == XG_STATE_J2) {
+    std::vector<J2Point*> p;
+    j2_points(*d, p);
+    return (long long)p.size() * 10;
+  }
+  if (kind == XG_STATE_STEEL02 || kind == XG_STATE_CONCRETE02) {
+    std::vector<FiberState*> f;
+    fibers(*d, kind == XG_STATE_STEEL02 ? 1 : 2, f);
+    return (long long)f.size() * (kind == XG_STATE_STEEL02 ? 11 : 5);
+  }
+  return fail(XG_ERR_ARG, "state: unknown kind");
+}
+
+int orc_get_state(xg_domain* d, int kind, double* out) {
+  const long long n = orc_state_size(d, kind);
+  if (n < 0) return (int)n;
+  if (kind == XG_STATE_DOMAIN) {
+    out[0] = d->time;
+    out[1] = d->lambda;
+  } else if (kind == XG_STATE_NODES) {
+    const int ndf = d->ndf;
+    for (size_t k = 0; k < d->nodes.size(); k++)
+      for (int j = 0; j < ndf; j++) {
+        out[k * 3 * ndf + j] = d->nodes[k].Uc[j];
+        out[k * 3 * ndf + ndf + j] = d->nodes[k].Vc[j];
+        out[k * 3 * ndf + 2 * ndf + j] = d->nodes[k].Ac[j];
+      }
+  } else if (kind == XG_STATE_J2) {
+    std::vector<J2Point*> p;
+    j2_points(*d, p);
+    for (size_t k = 0; k < p.size(); k++) {
+      out[k * 10] = p[k]->xi_n;
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) out[k * 10 + 1 + 3 * i + j] = p[k]->epsp_n[i][j];
+    }
+  } else if (kind == XG_STATE_STEEL02) {
+    std::vector<FiberState*> f;
+    fibers(*d, 1, f);
+    for (size_t k = 0; k < f.size(); k++) {
+      const Steel02State& s = f[k]->st;
+      const double v[11] = {s.epsminP, s.epsmaxP, s.epsplP, s.epss0P, s.sigs0P, s.epssrP,
+                            s.sigsrP,  (double)s.konP, s.epsP, s.sigP, s.eP};
+      for (int c = 0; c < 11; c++) out[k * 11 + c] = v[c];
+    }
+  } else {
+    std::vector<FiberState*> f;
+    fibers(*d, 2, f);
+    for (size_t k = 0; k < f.size(); k++) {
+      const Concrete02State& s = f[k]->cs;
+      const double v[5] = {s.ecminP, s.deptP, s.epsP, s.sigP, s.eP};
+      for (int c = 0; c < 5; c++) out[k * 5 + c] = v[c];
+    }
+  }
+  return 0;
+}
+
+int orc_set_state(xg_domain* d, int kind, const double* in) {
+  const long long n = orc_state_size(d, kind);
+  if (n < 0) return (int)n;
+  if (kind == XG_STATE_DOMAIN) {
+    d->time = in[0];
+    d->lambda = in[1];
+  } else if (kind == XG_STATE_NODES) {
+    const int ndf = d->ndf;
+    for (size_t k = 0; k < d->nodes.size(); k++)
+      for (int j = 0; j < ndf; j++) {
+        NodeRec& r = d->nodes[k];
+        r.Uc[j] = r.U[j] = in[k * 3 * ndf + j];
+        r.Vc[j] = r.V[j] = in[k * 3 * ndf + ndf + j];
+        r.Ac[j] = r.A[j] = in[k * 3 * ndf + 2 * ndf + j];
+      }
+  } else if (kind == XG_STATE_J2) {
+    std::vector<J2Point*> p;
+    j2_points(*d, p);
+    for (size_t k = 0; k < p.size(); k++) {
+      p[k]->xi_n = p[k]->xi_np1 = in[k * 10];
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) p[k]->epsp_n[i][j] = p[k]->epsp_np1[i][j] = in[k * 10 + 1 + 3 * i + j];
+    }
+  } else if (kind == XG_STATE_STEEL02) {
+    std::vector<FiberState*> f;
+    fibers(*d, 1, f);
+    for (size_t k = 0; k < f.size(); k++) {
+      Steel02State& s = f[k]->st;
+      const double* v = in + k * 11;
+      s.epsminP = v[0];
+      s.epsmaxP = v[1];
+      s.epsplP = v[2];
+      s.epss0P = v[3];
+      s.sigs0P = v[4];
+      s.epssrP = v[5];
+      s.sigsrP = v[6];
+      s.konP = (int)v[7];
+      s.epsP = v[8];
+      s.sigP = v[9];
+      s.eP = v[10];
+      steel02_revert(s);
+    }
+  } else {
+    std::vector<FiberState*> f;
+    fibers(*d, 2, f);
+    for (size_t k = 0; k < f.size(); k++) {
+      Concrete02State& s = f[k]->cs;
+      const double* v = in + k * 5;
+      s.ecminP = v[0];
+      s.deptP = v[1];
+      s.epsP = v[2];
+      s.sigP = v[3];
+      s.eP = v[4];
+      concrete02_revert(s);
+    }
+  }
+  return 0;
+}
+
 }  // extern "C"

@@ -86,6 +86,9 @@ SIGNATURES = {
     "element_resisting_force": (C.c_int, [VP, C.c_int, P_D, P_I]),
     "get_counters": (C.c_int, [VP, C.POINTER(Counters)]),
     "time_op": (C.c_int, [VP, C.c_int, C.c_int, P_D]),
+    "state_size": (C.c_longlong, [VP, C.c_int]),
+    "get_state": (C.c_int, [VP, C.c_int, P_D]),
+    "set_state": (C.c_int, [VP, C.c_int, P_D]),
     # batched ensemble (GPU library only)
     "ensemble_create": (C.c_int, [C.c_int, C.POINTER(VP)]),
     "ensemble_destroy": (C.c_int, [VP]),

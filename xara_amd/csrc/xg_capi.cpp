@@ -771,6 +771,32 @@ int xg_ensemble_status(xg_ensemble* h, int* status, long long* newton_iters) {
 }
 
 
+// ---- committed state --------------------------------------------------------
+long long xg_state_size(xg_domain* h, int kind) {
+  long long n = 0;
+  const int rc = guard([&] {
+    n = D(h).state_size(kind);
+    return 0;
+  });
+  return rc < 0 ? rc : n;
+}
+
+int xg_get_state(xg_domain* h, int kind, double* out) {
+  return guard([&] {
+    if (!out) throw xg::Error(XG_ERR_ARG, "null output");
+    D(h).get_state(kind, out);
+    return 0;
+  });
+}
+
+int xg_set_state(xg_domain* h, int kind, const double* in) {
+  return guard([&] {
+    if (!in) throw xg::Error(XG_ERR_ARG, "null input");
+    D(h).set_state(kind, in);
+    return 0;
+  });
+}
+
 // ---- adjoint ----------------------------------------------------------------
 static xg::Adjoint& ADJ(xg_domain* h) {
   if (!h->adj) h->adj = std::make_unique<xg::Adjoint>();

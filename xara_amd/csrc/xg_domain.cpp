@@ -134,6 +134,39 @@ struct J2Group : Group {
   size_t device_bytes() const override {
     return (epn.n + ept.n + xin.n + xit.n + sig.n + D.n) * 8 + dconn.n * 4;
   }
+  // J2Plasticity::sendSelf state part (J2Plasticity.cpp:597-614): xi_n, then
+  // epsilon_p_n(i, j) row by row; points in element order
+  long long state_size(int kind) const override { return kind == XG_STATE_J2 ? np() * 10 : 0; }
+  void get_state(Domain& d, int kind, double* out) override {
+    if (kind != XG_STATE_J2) return;
+    const long long n = np();
+    std::vector<double> e = epn.download(d.stream), x = xin.download(d.stream);
+    for (long long p = 0; p < n; p++) {
+      double t[6] = {0, 0, 0, 0, 0, 0};  // 00 11 22 01 12 20
+      for (int c = 0; c < ncomp(); c++) t[c] = e[c * n + p];
+      const double m[9] = {t[0], t[3], t[5], t[3], t[1], t[4], t[5], t[4], t[2]};
+      out[p * 10] = x[p];
+      for (int k = 0; k < 9; k++) out[p * 10 + 1 + k] = m[k];
+    }
+  }
+  void set_state(Domain& d, int kind, const double* in) override {
+    if (kind != XG_STATE_J2) return;
+    const long long n = np();
+    std::vector<double> e((size_t)ncomp() * n), x(n);
+    for (long long p = 0; p < n; p++) {
+      const double* m = in + p * 10 + 1;
+      x[p] = in[p * 10];
+      const double t[6] = {m[0], m[4], m[8], m[1], m[5], m[2]};
+      if (!brick && (t[4] != 0.0 || t[5] != 0.0))
+        throw Error(XG_ERR_ARG, "set_state: plane-strain points have no 12/20 plastic strain");
+      for (int c = 0; c < ncomp(); c++) e[(size_t)c * n + p] = t[c];
+    }
+    epn.upload(e, d.stream);
+    ept.upload(e, d.stream);
+    xin.upload(x, d.stream);
+    xit.upload(x, d.stream);
+    XG_HIP(hipStreamSynchronize(d.stream));
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -687,6 +720,73 @@ int Domain::analyze(int nsteps, double dtv) {
     commit();
   }
   return 0;
+}
+
+// ---- committed state in the reference's sendSelf layouts --------------------
+long long Domain::state_size(int kind) {
+  if (!ready) setup();
+  if (kind == XG_STATE_NODES) return (long long)nnode * ndf * 3;
+  if (kind == XG_STATE_DOMAIN) return 2;
+  if (kind != XG_STATE_J2 && kind != XG_STATE_STEEL02 && kind != XG_STATE_CONCRETE02)
+    throw Error(XG_ERR_ARG, "state: unknown kind");
+  long long n = 0;
+  for (auto& g : groups) n += g->state_size(kind);
+  return n;
+}
+
+// committed displacement, velocity, acceleration per node (Node::sendSelf,
+// Node.cpp:767-791), nodes in ascending tag order
+void Domain::get_state(int kind, double* out) {
+  const long long size = state_size(kind);
+  if (kind == XG_STATE_NODES) {
+    const std::vector<double> u = Uc.download(stream), v = Vc.download(stream), a = Ac.download(stream);
+    for (int n = 0; n < nnode; n++)
+      for (int j = 0; j < ndf; j++) {
+        out[(size_t)n * 3 * ndf + j] = u[(size_t)n * ndf + j];
+        out[(size_t)n * 3 * ndf + ndf + j] = v[(size_t)n * ndf + j];
+        out[(size_t)n * 3 * ndf + 2 * ndf + j] = a[(size_t)n * ndf + j];
+      }
+    return;
+  }
+  if (kind == XG_STATE_DOMAIN) {
+    out[0] = time;
+    out[1] = lambda;
+    return;
+  }
+  long long off = 0;
+  for (auto& g : groups) {
+    g->get_state(*this, kind, out + off);
+    off += g->state_size(kind);
+  }
+  (void)size;
+}
+
+void Domain::set_state(int kind, const double* in) {
+  (void)state_size(kind);
+  if (kind == XG_STATE_NODES) {
+    std::vector<double> u((size_t)nnode * ndf), v(u.size()), a(u.size());
+    for (int n = 0; n < nnode; n++)
+      for (int j = 0; j < ndf; j++) {
+        u[(size_t)n * ndf + j] = in[(size_t)n * 3 * ndf + j];
+        v[(size_t)n * ndf + j] = in[(size_t)n * 3 * ndf + ndf + j];
+        a[(size_t)n * ndf + j] = in[(size_t)n * 3 * ndf + 2 * ndf + j];
+      }
+    for (DVec<double>* t : {&U, &Uc}) t->upload(u, stream);
+    for (DVec<double>* t : {&V, &Vc}) t->upload(v, stream);
+    for (DVec<double>* t : {&A, &Ac}) t->upload(a, stream);
+    XG_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+  if (kind == XG_STATE_DOMAIN) {
+    time = in[0];
+    lambda = in[1];
+    return;
+  }
+  long long off = 0;
+  for (auto& g : groups) {
+    g->set_state(*this, kind, in + off);
+    off += g->state_size(kind);
+  }
 }
 
 }  // namespace xg

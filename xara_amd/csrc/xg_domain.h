@@ -36,6 +36,11 @@ struct Group {
   // per-point outputs for element i (local index) appended to out; returns width
   virtual int point_data(Domain& d, int what, std::vector<double>& out) = 0;
   virtual size_t device_bytes() const { return 0; }
+  // committed material state in the reference's sendSelf layout (XG_STATE_*
+  // kinds of the C-ABI): number of doubles, read, write (committed and trial)
+  virtual long long state_size(int) const { return 0; }
+  virtual void get_state(Domain&, int, double*) {}
+  virtual void set_state(Domain&, int, const double*) {}
 };
 
 struct NodeDef {
@@ -148,6 +153,9 @@ struct Domain {
   double dot_dev(const double* a, const double* b, int n);
   double read_eq(const double* v, int a);
   int analyze(int nsteps, double dtv);
+  long long state_size(int kind);
+  void get_state(int kind, double* out);
+  void set_state(int kind, const double* in);
   int newton();
   int new_step();
   int integ_update(const double* x);

@@ -327,6 +327,60 @@ struct FrameGroup : Group {
         for (int j = 0; j < ntr; j++) out[(size_t)e * nd + a * per + j] = m;
     }
   }
+  // Steel02::sendSelf (Steel02.cpp:287-308) state part: epsminP, epsmaxP,
+  // epsplP, epss0P, sigs0P, epssrP, sigsrP, konP, epsP, sigP, eP;
+  // Concrete02::sendSelf (Concrete02.cpp:339-354): ecminP, deptP, epsP,
+  // sigP, eP.  Fibers by element (ascending tag), section, then definition
+  // order: the pool order.
+  long long state_size(int kind) const override {
+    if (kind == XG_STATE_STEEL02) return f.nsteel * 11;
+    if (kind == XG_STATE_CONCRETE02) return f.nconc * 5;
+    return 0;
+  }
+  void get_state(Domain& d, int kind, double* out) override {
+    if (kind == XG_STATE_STEEL02) {
+      const long long n = f.nsteel;
+      const std::vector<double> h = st_c.download(d.stream);
+      const std::vector<int> k = kon_c.download(d.stream);
+      for (long long i = 0; i < n; i++) {
+        double* o = out + i * 11;
+        for (int c = 0; c < 7; c++) o[c] = h[(size_t)c * n + i];
+        o[7] = k[i];
+        for (int c = 7; c < 10; c++) o[c + 1] = h[(size_t)c * n + i];
+      }
+    } else if (kind == XG_STATE_CONCRETE02) {
+      const long long n = f.nconc;
+      const std::vector<double> h = cs_c.download(d.stream);
+      for (long long i = 0; i < n; i++)
+        for (int c = 0; c < 5; c++) out[i * 5 + c] = h[(size_t)c * n + i];
+    }
+  }
+  void set_state(Domain& d, int kind, const double* in) override {
+    if (kind == XG_STATE_STEEL02) {
+      const long long n = f.nsteel;
+      std::vector<double> h((size_t)kSteelHist * n);
+      std::vector<int> k(n);
+      for (long long i = 0; i < n; i++) {
+        const double* o = in + i * 11;
+        for (int c = 0; c < 7; c++) h[(size_t)c * n + i] = o[c];
+        k[i] = (int)o[7];
+        for (int c = 7; c < 10; c++) h[(size_t)c * n + i] = o[c + 1];
+      }
+      st_c.upload(h, d.stream);
+      st_t.upload(h, d.stream);
+      kon_c.upload(k, d.stream);
+      kon_t.upload(k, d.stream);
+    } else if (kind == XG_STATE_CONCRETE02) {
+      const long long n = f.nconc;
+      std::vector<double> h((size_t)kConcHist * n);
+      for (long long i = 0; i < n; i++)
+        for (int c = 0; c < 5; c++) h[(size_t)c * n + i] = in[i * 5 + c];
+      cs_c.upload(h, d.stream);
+      cs_t.upload(h, d.stream);
+    }
+    XG_HIP(hipStreamSynchronize(d.stream));
+  }
+
   int point_data(Domain& d, int what, std::vector<double>& out) override {
     const long long nsec = f.nsec;
     const int o = ord();

@@ -138,6 +138,36 @@ class Model:
         else:
             raise XaraError(-1, f"uniaxialMaterial {kind} is not on the GPU path")
 
+    STATE_KINDS = {"nodes": 1, "J2": 2, "Steel02": 3, "Concrete02": 4, "domain": 5}
+
+    def get_state(self, kind):
+        """Committed state of one kind in the reference's sendSelf layout
+        (see include/xara_gpu.h: XG_STATE_*), as a flat float64 array."""
+        k = self.STATE_KINDS[kind]
+        n = self._b.state_size(self._h, k)
+        self._check(n)
+        out = np.zeros(n)
+        self._check(self._b.get_state(self._h, k, _d(out)))
+        return out
+
+    def set_state(self, kind, values):
+        """Write a committed state (trial state set equal to it)."""
+        k = self.STATE_KINDS[kind]
+        v = _f64(values)
+        n = self._b.state_size(self._h, k)
+        self._check(n)
+        if v.size != n:
+            raise XaraError(-1, f"state {kind}: {v.size} values given, {n} expected")
+        self._check(self._b.set_state(self._h, k, _d(v)))
+
+    def save_state(self):
+        """All committed state kinds (checkpoint)."""
+        return {k: self.get_state(k) for k in self.STATE_KINDS}
+
+    def load_state(self, state):
+        for k, v in state.items():
+            self.set_state(k, v)
+
     def material_tags(self, kind):
         """Tags of the defined materials of one kind, ascending (the order of
         per-material results such as the adjoint gradients)."""
