@@ -5,15 +5,12 @@
 
 namespace xg {
 
-// Two passes over the sections: Concrete02 / Elastic fibers first (short
-// code, high occupancy), then Steel02 fibers adding to the section sums (the
-// Menegotto-Pinto powers need many more registers).
-template <bool D3, int KINDS, bool ACCUM>
+template <bool D3>
 __global__ __launch_bounds__(256) void k_frame_update(FrameDev f, const double* __restrict__ U) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long sec = (long long)blockIdx.x * 4 + wave;
   if (sec >= f.nsec) return;
-  section_update<D3, KINDS, ACCUM>(f, sec, lane, U);
+  section_update<D3>(f, sec, lane, U);
 }
 
 template <bool D3>
@@ -51,13 +48,8 @@ void launch_frame_update(hipStream_t s, const FrameDev& f, bool d3, const double
   (void)err;
   if (f.nsec <= 0) return;
   const dim3 g(ceil_div(f.nsec, 4)), b(256);
-  if (d3) {
-    hipLaunchKernelGGL((k_frame_update<true, kKindConc | kKindElas, false>), g, b, 0, s, f, U);
-    hipLaunchKernelGGL((k_frame_update<true, kKindSteel, true>), g, b, 0, s, f, U);
-  } else {
-    hipLaunchKernelGGL((k_frame_update<false, kKindConc | kKindElas, false>), g, b, 0, s, f, U);
-    hipLaunchKernelGGL((k_frame_update<false, kKindSteel, true>), g, b, 0, s, f, U);
-  }
+  if (d3) hipLaunchKernelGGL(k_frame_update<true>, g, b, 0, s, f, U);
+  else hipLaunchKernelGGL(k_frame_update<false>, g, b, 0, s, f, U);
 }
 void launch_frame_revert(hipStream_t s, const FrameDev& f, bool d3) {
   if (f.nsec <= 0) return;

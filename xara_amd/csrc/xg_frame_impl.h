@@ -111,23 +111,6 @@ __device__ __forceinline__ void steel_fiber(const FrameDev& f, const Steel02Para
   tangent = t.e;
 }
 
-// One Concrete02 fiber (the previous trial sig / e are kept on the early
-// return of setTrialStrain).
-__device__ __forceinline__ void concrete_fiber(const FrameDev& f, const Concrete02Param& p, long long idx,
-                                               double strain, double& stress, double& tangent) {
-  const long long n = f.nconc;
-  double ecmin, dept, eps, sig = f.cs_t[3 * n + idx], e = f.cs_t[4 * n + idx];
-  concrete02_trial(p, f.cs_c[0 * n + idx], f.cs_c[1 * n + idx], f.cs_c[2 * n + idx], f.cs_c[3 * n + idx], strain,
-                   ecmin, dept, eps, sig, e);
-  f.cs_t[0 * n + idx] = ecmin;
-  f.cs_t[1 * n + idx] = dept;
-  f.cs_t[2 * n + idx] = eps;
-  f.cs_t[3 * n + idx] = sig;
-  f.cs_t[4 * n + idx] = e;
-  stress = sig;
-  tangent = e;
-}
-
 // Section deformation at the section's Gauss point from nodal
 // displacements U (DispBeamColumn2d.cpp:235-284, DispBeamColumn3d.cpp:235-292).
 template <bool D3>
@@ -188,12 +171,10 @@ struct SecSums {
 };
 
 // Section update, one wavefront per section instance (sec is the same for
-// every lane of the wave).  KINDS selects the fiber kinds updated (bit 0
-// Steel02, bit 1 Concrete02, bit 2 Elastic); with ACCUM the sums are added
-// to the resultant and tangent already stored for the section (a second
-// pass over the remaining kinds).
-constexpr int kKindSteel = 1, kKindConc = 2, kKindElas = 4, kKindAll = 7;
-template <bool D3, int KINDS = kKindAll, bool ACCUM = false>
+// every lane of the wave).  Concrete02 runs, the bulk of an RC section, are
+// walked two fibers per lane per pass so that both fibers' history loads are
+// in flight together.
+template <bool D3>
 __device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
   const int e = (int)(sec / f.nip);
   double d[4];
@@ -207,7 +188,6 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
   for (int r = r0; r < r1; r++) {
     const int kind = uniform_i(f.run_kind[r]), mat = uniform_i(f.run_mat[r]);
     const int b0 = uniform_i(f.run_beg[r]), b1 = uniform_i(f.run_beg[r + 1]);
-    if (!(KINDS & (kind == 1 ? kKindSteel : (kind == 2 ? kKindConc : kKindElas)))) continue;
     if (kind == 1) {
       const Steel02Param* q = f.steelp + mat;
       const Steel02Param p{uniform_d(q->Fy), uniform_d(q->E0),  uniform_d(q->b),  uniform_d(q->R0),
@@ -224,12 +204,43 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
       const Concrete02Param* q = f.concp + mat;
       const Concrete02Param p{uniform_d(q->fc),    uniform_d(q->epsc0), uniform_d(q->fcu), uniform_d(q->epscu),
                               uniform_d(q->rat),   uniform_d(q->ft),    uniform_d(q->Ets)};
-      for (int fi = b0 + lane; fi < b1; fi += 64) {
-        const double y = f.fy[fi], z = D3 ? f.fz[fi] : 0.0;
-        const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
-        double stress, tangent;
-        concrete_fiber(f, p, cb + f.flocal[fi], strain, stress, tangent);
-        S.add<D3>(y, z, f.fA[fi], stress, tangent);
+      // Concrete02::setTrialStrain; the previous trial sig / e are kept on its
+      // early return, so they are inputs too
+      for (int fi = b0 + lane; fi < b1; fi += 128) {
+        const int fj = fi + 64;
+        const bool two = fj < b1;
+        const long long n = f.nconc;
+        const long long ia = cb + f.flocal[fi], ib = two ? cb + f.flocal[fj] : ia;
+        const double ya = f.fy[fi], za = D3 ? f.fz[fi] : 0.0;
+        const double yb = two ? f.fy[fj] : 0.0, zb = (D3 && two) ? f.fz[fj] : 0.0;
+        double ca[4], cb2[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          ca[k] = f.cs_c[k * n + ia];
+          cb2[k] = f.cs_c[k * n + ib];
+        }
+        double sa = f.cs_t[3 * n + ia], ea = f.cs_t[4 * n + ia];
+        double sb2 = f.cs_t[3 * n + ib], eb2 = f.cs_t[4 * n + ib];
+        const double Aa = f.fA[fi], Ab = two ? f.fA[fj] : 0.0;
+        const double ea_ = D3 ? d[0] - ya * d[1] + za * d[2] : d[0] - ya * d[1];
+        const double eb_ = D3 ? d[0] - yb * d[1] + zb * d[2] : d[0] - yb * d[1];
+        double m1, d1, e1, m2, d2, e2;
+        concrete02_trial(p, ca[0], ca[1], ca[2], ca[3], ea_, m1, d1, e1, sa, ea);
+        concrete02_trial(p, cb2[0], cb2[1], cb2[2], cb2[3], eb_, m2, d2, e2, sb2, eb2);
+        f.cs_t[0 * n + ia] = m1;
+        f.cs_t[1 * n + ia] = d1;
+        f.cs_t[2 * n + ia] = e1;
+        f.cs_t[3 * n + ia] = sa;
+        f.cs_t[4 * n + ia] = ea;
+        S.add<D3>(ya, za, Aa, sa, ea);
+        if (two) {
+          f.cs_t[0 * n + ib] = m2;
+          f.cs_t[1 * n + ib] = d2;
+          f.cs_t[2 * n + ib] = e2;
+          f.cs_t[3 * n + ib] = sb2;
+          f.cs_t[4 * n + ib] = eb2;
+          S.add<D3>(yb, zb, Ab, sb2, eb2);
+        }
       }
     } else {
       const double E = uniform_d(f.elasE[mat]);
@@ -258,17 +269,10 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
     if (D3) {
       double kd[16] = {k0, k1, k2, 0, k1, k5, k6, 0, k2, k6, k10, 0, 0, 0, 0, 0};
       double sd[4] = {s0, s1, s2, 0.0};
-      if (ACCUM) {
-#pragma unroll
-        for (int c = 0; c < 16; c++) kd[c] += f.ks[c * n + sec];
-#pragma unroll
-        for (int c = 0; c < 4; c++) sd[c] += f.s[c * n + sec];
-      } else {
-        const double GJ = f.GJ[def];
-        if (GJ >= 0.0) {
-          sd[3] = GJ * d[3];
-          kd[15] = GJ;
-        }
+      const double GJ = f.GJ[def];
+      if (GJ >= 0.0) {
+        sd[3] = GJ * d[3];
+        kd[15] = GJ;
       }
 #pragma unroll
       for (int c = 0; c < 16; c++) f.ks[c * n + sec] = kd[c];
@@ -278,13 +282,6 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
         f.e[c * n + sec] = d[c];
       }
     } else {
-      if (ACCUM) {
-        k0 += f.ks[0 * n + sec];
-        k1 += f.ks[1 * n + sec];
-        k5 += f.ks[3 * n + sec];
-        s0 += f.s[0 * n + sec];
-        s1 += f.s[1 * n + sec];
-      }
       f.ks[0 * n + sec] = k0;
       f.ks[1 * n + sec] = k1;
       f.ks[2 * n + sec] = k1;
