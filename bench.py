@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--frame-steps", type=int, default=0, help="DisplacementControl steps in frame mode")
     ap.add_argument("--newton-steps", type=int, default=40)
     ap.add_argument("--system", default="GpuBlockPCG")
+    ap.add_argument("--spmv", choices=["auto", "csr"], default="auto", help="SpMV layout of the PCG")
+    ap.add_argument("--lanes", type=int, default=0, help="SpMV threads per row (0 = from the row length)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -184,6 +186,7 @@ def newton_main(args):
     m = Model(3, 3, device=0)
     meshes.build_brick_cube(m, mesh, pressure=1.0)
     m.system(args.system, 1e-10, 50000)
+    m.spmv_config(args.spmv, args.lanes)
     t0 = time.time()
     m.setup()
     setup_s = time.time() - t0
@@ -193,6 +196,7 @@ def newton_main(args):
     B = m.form_unbalance()
     m.form_tangent(values=False)
     u = m.solve(B)
+    spmv_ms = m.time_op("spmv", 20)  # on the assembled elastic tangent
     m.incr_disp(u)
     m.update()
     sig = m.point_data("stress", npts)
@@ -227,6 +231,7 @@ def newton_main(args):
         "config": {"workload": f"stdBrick J2 cube {n}^3 pushover to 1.5 p_y, LoadControl, "
                                f"{args.system} rtol 1e-10", "newton_iters": iters,
                    "cg_iters": c["cg_iters"] - c0["cg_iters"], "solves": c["solves"] - c0["solves"],
+                   "spmv": args.spmv, "spmv_lanes": args.lanes, "spmv_ms": round(spmv_ms, 4),
                    "plastic_fraction": round(plastic, 3), "setup_s": round(setup_s, 2),
                    "p_y": p_y, "final_load_factor": m.load_factor},
     }
@@ -514,7 +519,7 @@ def adjoint_main(args):
     ag = ground_motion.kanai_tajimi(0, N + 5, dt, pga)
 
     def build(m, theta):
-        meshes.define_transient_frame(m, fr, theta, ag, dt, tol=1e-10, maxit=40)
+        meshes.define_transient_frame(m, fr, theta, ag, dt, tol=1e-12, maxit=40)
         m.system(args.system, 1e-12, 200000)
 
     roof = meshes.roof_nodes(fr)
@@ -566,7 +571,7 @@ def adjoint_main(args):
     if args.ddm:
         from tests import _oracle
         o = _oracle.oracle_model(3, 6)
-        meshes.define_transient_frame(o, fr, theta, ag, dt, tol=1e-10, maxit=40)
+        meshes.define_transient_frame(o, fr, theta, ag, dt, tol=1e-12, maxit=40)
         o.setup()
         t0 = time.perf_counter()
         L, dL = _oracle.ddm_transient(o, N, dt, d[:, :3], d[:, 3], roof, 1)

@@ -144,6 +144,15 @@ int  xg_rayleigh(xg_domain* d, double alphaM, double betaK, double betaKinit,
 /* ---- analysis objects (BasicAnalysisBuilder::set, BasicAnalysisBuilder.cpp:781-822) */
 /* `system ...` (solver.hpp:113 soe_table)                                    */
 int  xg_system(xg_domain* d, int kind, double rtol, int maxit);
+/* SpMV layout of the PCG systems.  No counterpart in the reference (its
+   sparse solvers take the CSR arrays as they are): a tuning knob.
+   XG_SPMV_AUTO uses node-blocked column indices (one index per neighbour
+   node instead of one per nonzero, same values and pattern) when every node
+   has all its DOFs free or none, and plain CSR otherwise; lanes = threads per
+   matrix row (4, 8, 16, 32 or 64; 0 = chosen from the mean row length).     */
+#define XG_SPMV_AUTO 0
+#define XG_SPMV_CSR  1
+int  xg_spmv_config(xg_domain* d, int format, int lanes);
 /* `numberer RCM|Plain`                                                       */
 int  xg_numberer(xg_domain* d, int kind);
 /* `test NormUnbalance tol maxIter` etc.                                      */

@@ -137,3 +137,55 @@ def test_quad_strip_displacement_control():
     for (ug, lg), (uo, lo) in zip(*res):
         assert relerr(ug, uo) <= RTOL
         assert abs(lg - lo) <= RTOL * abs(lo)
+
+
+def _spmv_models():
+    mesh = meshes.brick_cube(4)
+    m = Model(3, 3)
+    meshes.build_brick_cube(m, mesh, pressure=1.0)
+    yield "brick", m
+    mesh = meshes.quad_strip(12, 3)
+    m = Model(2, 2)
+    meshes.build_quad_strip(m, mesh)
+    yield "quad", m
+    fr = meshes.rc_frame_3d(2, 2, 3)
+    conn, _ = meshes.frame_elements(fr)
+    m = Model(3, 6)
+    meshes.build_rc_frame_per_element(m, fr, meshes.default_theta(len(conn)))
+    yield "frame3d", m
+
+
+@pytest.mark.parametrize("lanes", [0, 4, 32])
+def test_node_blocked_spmv_matches_csr(lanes):
+    """The node-blocked column indices address the same entries in the same
+    order as the CSR column indices: y = K x is bitwise the same."""
+    for name, m in _spmv_models():
+        print(f"case: spmv {name} lanes {lanes}", flush=True)
+        m.setup()
+        rng = np.random.default_rng(7)
+        m.set_trial_disp(1e-4 * rng.normal(size=(m.num_nodes, m.ndf)))
+        m.update()
+        m.form_tangent(values=False)
+        x = rng.normal(size=m.neq)
+        m.spmv_config("csr", lanes)
+        y_csr = m.spmv(x)
+        m.spmv_config("auto", lanes)
+        y_nb = m.spmv(x)
+        assert np.array_equal(y_nb, y_csr)
+        K = m.form_tangent()
+        rp, ci = m.csr_pattern()
+        rows = np.repeat(np.arange(m.neq), np.diff(rp))
+        y_ref = np.zeros(m.neq)
+        np.add.at(y_ref, rows, K * x[ci])
+        assert relerr(y_nb, y_ref) <= 1e-13
+        m.close()
+
+
+def test_pcg_solve_csr_layout():
+    print("case: pcg csr layout", flush=True)
+    mesh, (g, o) = cube_pair(5, system="GpuBlockPCG")
+    g.spmv_config("csr")
+    for m in (g, o):
+        m.form_tangent(values=False)
+    b = np.random.default_rng(4).normal(size=g.neq)
+    assert relerr(g.solve(b), o.solve(b)) <= 1e-9

@@ -351,6 +351,18 @@ int xg_system(xg_domain* h, int kind, double rtol, int maxit) {
   });
 }
 
+int xg_spmv_config(xg_domain* h, int format, int lanes) {
+  return guard([&] {
+    auto& d = D(h);
+    if (format != XG_SPMV_AUTO && format != XG_SPMV_CSR) throw xg::Error(XG_ERR_ARG, "spmv format");
+    if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+      throw xg::Error(XG_ERR_ARG, "spmv lanes must be 0, 4, 8, 16, 32 or 64");
+    d.spmv_format = format;
+    d.spmv_lanes = lanes;
+    return 0;
+  });
+}
+
 int xg_numberer(xg_domain* h, int kind) {
   return guard([&] {
     auto& d = D(h);
@@ -514,7 +526,7 @@ int xg_spmv(xg_domain* h, const double* x, double* y) {
     auto& d = R(h);
     d.tmpv.upload(x, d.num.neq, d.stream);
     xg::launch_spmv(d.stream, d.lanes(), d.num.neq, d.rowptr.p, d.colind.p, d.vals.p, d.tmpv.p, d.q.p, nullptr,
-                nullptr);
+                    nullptr, d.spmv_nbm(), d.nbcol.p);
     d.q.download(y, d.num.neq, d.stream);
     return 0;
   });
@@ -636,7 +648,7 @@ int xg_time_op(xg_domain* h, int op, int reps, double* ms) {
         d.form_unbalance();
       } else if (op == XG_OP_SPMV) {
         xg::launch_spmv(d.stream, d.lanes(), d.num.neq, d.rowptr.p, d.colind.p, d.vals.p, d.p.p, d.q.p, nullptr,
-                    nullptr);
+                        nullptr, d.spmv_nbm(), d.nbcol.p);
       } else if (op == XG_OP_COPY) {
         xg::launch_copy(d.stream, ncopy, cx.p, cy.p);
       } else

@@ -182,9 +182,55 @@ Domain::~Domain() {
 int Domain::lanes() {
   if (spmv_lanes > 0) return spmv_lanes;
   const double mean = num.neq ? (double)num.colind.size() / num.neq : 1.0;
-  if (mean > 48) return 16;
-  if (mean > 20) return 8;
+  if (mean > 128) return 16;
+  if (mean > 24) return 8;
   return 4;
+}
+
+// Node-blocked view of the CSR pattern for k_spmv_nb_dot: with M = ndf, it
+// exists when every node has all M DOFs free or none, its equations being
+// M*J .. M*J + M-1, and every row of block J lists the same neighbour blocks
+// as ascending runs of M columns.  Returns one block column per (block row,
+// neighbour) and sets nb_m = M, or returns nothing and sets nb_m = 0.
+std::vector<int> Domain::node_block_columns() {
+  nb_m = 0;
+  const int M = ndf;
+  std::vector<int> out;
+  if (M != 2 && M != 3 && M != 6) return out;
+  for (int n = 0; n < nnode; n++) {
+    const int* e = &num.eq[(size_t)n * M];
+    if (e[0] < 0) {
+      for (int j = 1; j < M; j++)
+        if (e[j] >= 0) return out;
+      continue;
+    }
+    if (e[0] % M) return out;
+    for (int j = 1; j < M; j++)
+      if (e[j] != e[0] + j) return out;
+  }
+  const int neq = num.neq;
+  if (neq % M) return out;
+  const std::vector<int>& rp = num.rowptr;
+  const std::vector<int>& ci = num.colind;
+  for (int J = 0; J < neq / M; J++) {
+    const int r0 = M * J, b0 = rp[r0], len = rp[r0 + 1] - b0;
+    if (len % M || (long long)b0 != (long long)M * M * (long long)out.size()) return {};
+    for (int k = 0; k < len; k += M) {
+      const int c0 = ci[b0 + k];
+      if (c0 % M) return {};
+      for (int c = 1; c < M; c++)
+        if (ci[b0 + k + c] != c0 + c) return {};
+      out.push_back(c0 / M);
+    }
+    for (int r = 1; r < M; r++) {
+      const int b = rp[r0 + r];
+      if (rp[r0 + r + 1] - b != len) return {};
+      for (int k = 0; k < len; k++)
+        if (ci[b + k] != ci[b0 + k]) return {};
+    }
+  }
+  nb_m = M;
+  return out;
 }
 
 void Domain::setup() {
@@ -354,6 +400,7 @@ void Domain::setup() {
     }
   }
   nblk = (int)blkf.size();
+  std::vector<int> nbc = node_block_columns();
 
   // ---- device upload ----
   if (!stream) XG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
@@ -378,6 +425,7 @@ void Domain::setup() {
   eqmap.upload(num.eq, s);
   rowptr.upload(num.rowptr, s);
   colind.upload(num.colind, s);
+  nbcol.upload(nbc, s);
   vals.resize(nnz);
   vals.zero(s);
   Ke.resize(ke_total);
@@ -511,7 +559,7 @@ int Domain::solve(const double* b, double* x) {
     if (issued >= maxit) break;
     const int chunk = std::min(cg_chunk, maxit - issued);
     cg_iterations(stream, chunk, issued, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, q.p, bw,
-                  binv.p, bfirst.p, bsize.p, part.p, scal.p);
+                  binv.p, bfirst.p, bsize.p, part.p, scal.p, spmv_nbm(), nbcol.p);
     issued += chunk;
   }
   counters.solves++;

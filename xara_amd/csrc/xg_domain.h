@@ -97,6 +97,10 @@ struct Domain {
   DVec<double> crd, U, Uc, V, Vc, A, Ac;
   DVec<int> eqmap;
   DVec<int> rowptr, colind;
+  // node-blocked column indices of the CSR pattern (k_spmv_nb_dot), when
+  // every node has all ndf DOFs free or none: nb_m = ndf, else 0
+  DVec<int> nbcol;
+  int nb_m = 0;
   DVec<double> vals, Ke, Re, B, X, Peq;
   DVec<int> asm_ptr, asm_idx, rhs_ptr, rhs_idx;
   DVec<int> nfail, pcbad;
@@ -112,6 +116,7 @@ struct Domain {
   double rtol = 1e-10;
   int maxit = 20000;
   int spmv_lanes = 0;  // 0 = choose from the mean row length
+  int spmv_format = XG_SPMV_AUTO;
   int cg_chunk = 16;
   int test_kind = XG_TEST_NORM_UNBALANCE;
   double test_tol = 1e-6;
@@ -162,6 +167,8 @@ struct Domain {
   int run_test(int& iter, std::vector<double>& norms, int& numIncr);
   void precond_setup();
   int lanes();
+  std::vector<int> node_block_columns();
+  int spmv_nbm() const { return spmv_format == XG_SPMV_CSR ? 0 : nb_m; }
 };
 
 }  // namespace xg

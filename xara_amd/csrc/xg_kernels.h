@@ -42,8 +42,11 @@ void launch_newmark_predict(hipStream_t s, long long ndofs, const int* eqmap, co
 void launch_newmark_correct(hipStream_t s, long long ndofs, const int* eqmap, const double* dU, double c2, double c3,
                             double* U, double* V, double* A);
 int spmv_blocks(int n, int lanes);
+// nbm = 2, 3 or 6 with nbcol: node-blocked column indices (k_spmv_nb_dot);
+// nbm = 0: plain CSR
 void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
-                 const double* x, double* y, double* partial, const int* stop);
+                 const double* x, double* y, double* partial, const int* stop, int nbm = 0,
+                 const int* nbcol = nullptr);
 void launch_block_inverse(hipStream_t s, int nblk, const int* first, const unsigned char* size,
                           const int* rowptr, const int* colind, const double* vals, int bw, double* binv,
                           int* bad);
@@ -60,7 +63,7 @@ void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, doubl
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
                    double* p, double* q, int bw, const double* binv, const int* bfirst,
-                   const unsigned char* bsize, double* part, void* scal);
+                   const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol);
 void cg_read_scal(hipStream_t s, const void* scal, int* converged, int* iter, double* rr, double* bb,
                   int* breakdown);
 

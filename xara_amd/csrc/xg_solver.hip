@@ -121,6 +121,35 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // ---------------------------------------------------------------------------
 // CSR SpMV y = A x with L lanes per row; per-block partial of x.y
 // ---------------------------------------------------------------------------
+// One lane's share of a row: sum over k = b + lane, b + lane + L, ... < e of
+// vals[k] * x[col(raw(k), k)], in that order.  raw(k) is the stored column
+// index of entry k and col() forms the column from it.  The value and index
+// loads of U entries are issued together, then their x loads, then the
+// products, so a wave waits for memory twice per U entries rather than twice
+// per entry.  Entries past e load entry e - 1 again (in bounds, cached) and
+// add nothing.
+template <int L, int U = 4, class Raw, class Col>
+__device__ __forceinline__ double row_dot(int b, int e, int lane, const double* __restrict__ vals,
+                                          const double* __restrict__ x, Raw raw, Col col) {
+  double sum = 0.0;
+  for (int k0 = b + lane; k0 < e; k0 += U * L) {
+    double v[U], xv[U];
+    int r[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = min(k0 + u * L, e - 1);
+      v[u] = vals[k];
+      r[u] = raw(k);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = x[col(r[u], min(k0 + u * L, e - 1))];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u * L < e) sum += v[u] * xv[u];
+  }
+  return sum;
+}
+
 template <int L>
 __global__ __launch_bounds__(256) void k_spmv_dot(int n, const int* __restrict__ rowptr,
                                                   const int* __restrict__ colind,
@@ -141,7 +170,56 @@ __global__ __launch_bounds__(256) void k_spmv_dot(int n, const int* __restrict__
     double sum = 0.0;
     if (row < n) {
       const int b = rowptr[row], e = rowptr[row + 1];
-      for (int k = b + lane; k < e; k += L) sum += vals[k] * x[colind[k]];
+      sum = row_dot<L>(
+          b, e, lane, vals, x, [&](int k) { return colind[k]; }, [](int c, int) { return c; });
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
+    if (row < n && lane == 0) {
+      y[row] = sum;
+      d += sum * x[row];
+    }
+  }
+  if (partial) {
+    const double bs = block_sum<256>(d, sh);
+    if (t == 0) partial[blockIdx.x] = bs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Node-blocked SpMV.  When every node owns M consecutive equations (all its
+// DOFs free, M = ndf) or none, equation M*J + r belongs to node block J and
+// its CSR row lists the neighbouring blocks' equations as ascending runs of
+// M, the same runs in each of the M rows of a block.  The columns then follow
+// from one block column per neighbour (nbcol, 4 bytes per M*M nonzeros
+// instead of 4 per nonzero) and rowptr[M*J] / (M*M) is block J's first entry
+// in nbcol; the values are the CSR values themselves.  Same lane layout,
+// reductions and dot-product partials as k_spmv_dot.
+// ---------------------------------------------------------------------------
+template <int L, int M>
+__global__ __launch_bounds__(256) void k_spmv_nb_dot(int n, const int* __restrict__ rowptr,
+                                                     const int* __restrict__ nbcol,
+                                                     const double* __restrict__ vals,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     double* __restrict__ partial,
+                                                     const int* __restrict__ stop) {
+  __shared__ double sh[256];
+  if (stop && stop[0]) return;
+  const int t = threadIdx.x;
+  const int lane = t % L;
+  constexpr int R = 256 / L;
+  double d = 0.0;
+  for (int row0 = blockIdx.x * R; row0 < n; row0 += gridDim.x * R) {
+    const int row = row0 + t / L;
+    double sum = 0.0;
+    if (row < n) {
+      const int b = rowptr[row], e = rowptr[row + 1];
+      // the M rows of a block have the same length, so the block's first
+      // entry is b - (row % M) * (e - b)
+      const int* nc = nbcol + (b - (row % M) * (e - b)) / (M * M);
+      sum = row_dot<L>(
+          b, e, lane, vals, x, [&](int k) { return nc[(k - b) / M]; },
+          [&](int c, int k) { return c * M + (k - b) % M; });
     }
 #pragma unroll
     for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
@@ -169,8 +247,17 @@ __global__ __launch_bounds__(256) void k_cg_alpha(int nb, const double* __restri
                                                   CGScal* __restrict__ s) {
   __shared__ double sh[256];
   if (s->converged) return;
+  // a thread's partials in ascending order, eight loads in flight at a time
   double v = 0.0;
-  for (int i = threadIdx.x; i < nb; i += 256) v += part[i];
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < nb; i += 8 * 256) {
+    double a[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) a[u] = part[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v += a[u];
+  }
+  for (; i < nb; i += 256) v += part[i];
   const double pq = block_sum<256>(v, sh);
   if (threadIdx.x == 0) {
     s->pq = pq;
@@ -457,12 +544,40 @@ void launch_add_diag(hipStream_t s, int n, const long long* pos, const double* m
   hipLaunchKernelGGL(k_add_diag, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, pos, m, coef, vals);
 }
 // grid of the CG kernels: enough workgroups to fill the chip, few enough
-// that the single-workgroup reductions of their partials stay short
+// that the single-workgroup reductions of their partials stay short.  The
+// SpMV, whose rows vary in length, balances better over more workgroups.
 constexpr int kCgMaxBlocks = 2048;
-int spmv_blocks(int n, int lanes) { return std::min(ceil_div(n, 256 / lanes), kCgMaxBlocks); }
+constexpr int kSpmvMaxBlocks = 8192;
+int spmv_blocks(int n, int lanes) { return std::min(ceil_div(n, 256 / lanes), kSpmvMaxBlocks); }
 static int vec_blocks(int n) { return std::min(ceil_div(n, 256), kCgMaxBlocks); }
+template <int M>
+static void launch_spmv_nb(hipStream_t s, int lanes, int n, const int* rowptr, const int* nbcol,
+                           const double* vals, const double* x, double* y, double* partial, const int* stop) {
+  const dim3 g(spmv_blocks(n, lanes)), b(256);
+  switch (lanes) {
+    case 4: hipLaunchKernelGGL((k_spmv_nb_dot<4, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop); break;
+    case 8: hipLaunchKernelGGL((k_spmv_nb_dot<8, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop); break;
+    case 16:
+      hipLaunchKernelGGL((k_spmv_nb_dot<16, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop);
+      break;
+    case 32:
+      hipLaunchKernelGGL((k_spmv_nb_dot<32, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop);
+      break;
+    default:
+      hipLaunchKernelGGL((k_spmv_nb_dot<64, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop);
+      break;
+  }
+}
+
 void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
-                 const double* x, double* y, double* partial, const int* stop) {
+                 const double* x, double* y, double* partial, const int* stop, int nbm, const int* nbcol) {
+  if (n <= 0) return;
+  switch (nbm) {
+    case 2: return launch_spmv_nb<2>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop);
+    case 3: return launch_spmv_nb<3>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop);
+    case 6: return launch_spmv_nb<6>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop);
+    default: break;
+  }
   const dim3 g(spmv_blocks(n, lanes)), b(256);
   switch (lanes) {
     case 4: hipLaunchKernelGGL(k_spmv_dot<4>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
@@ -511,13 +626,13 @@ void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, doubl
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
                    double* p, double* q, int bw, const double* binv, const int* bfirst,
-                   const unsigned char* bsize, double* part, void* scal) {
+                   const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol) {
   CGScal* S = (CGScal*)scal;
   const int nbs = spmv_blocks(n, lanes);
   const int nbv = vec_blocks(n);
   const int* stop = &S->converged;
   for (int it = 0; it < iters; it++) {
-    launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop);
+    launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop, nbm, nbcol);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, nbs, part, S);
     const bool even = ((it0 + it) & 1) == 0;
     hipLaunchKernelGGL(k_cg_update, dim3(nbv), dim3(256), 0, s, n, S, p, q, x, even ? r0 : r1,

@@ -252,6 +252,12 @@ class Model:
     def system(self, kind, rtol=0.0, maxit=0):
         self._check(self._b.system(self._h, SYSTEMS[kind.lower()], rtol, maxit))
 
+    def spmv_config(self, fmt="auto", lanes=0):
+        """SpMV layout of the PCG: "auto" (node-blocked column indices when
+        every node has all DOFs free or none) or "csr"; lanes per row (0 =
+        from the mean row length)."""
+        self._check(self._b.spmv_config(self._h, {"auto": 0, "csr": 1}[fmt], int(lanes)))
+
     def numberer(self, kind):
         self._check(self._b.numberer(self._h, NUMBERERS[kind.lower()]))
 
