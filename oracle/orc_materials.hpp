@@ -8,6 +8,7 @@
 #pragma once
 #include <cmath>
 #include <cfloat>
+#include <cstddef>
 #include <cstring>
 
 namespace orc {
@@ -44,9 +45,10 @@ struct J2Point {
   double xi_n, xi_np1;
   double strain[3][3], stress[3][3];
   double tangent[3][3][3][3];
-  void zero() {  // J2Plasticity::zero, J2Plasticity.cpp:213-224
-    std::memset(this, 0, sizeof(*this));
-  }
+  // J2Plasticity::zero, J2Plasticity.cpp:213-224: plastic state, stress and
+  // strain; the tangent of the last plastic_integrator call stays (the
+  // constructor and revertToStart both leave it to plastic_integrator)
+  void zero() { std::memset(this, 0, offsetof(J2Point, tangent)); }
 };
 
 // q(xi), qprime(xi): J2Plasticity.cpp:425-437

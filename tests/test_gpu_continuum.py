@@ -189,3 +189,29 @@ def test_pcg_solve_csr_layout():
         m.form_tangent(values=False)
     b = np.random.default_rng(4).normal(size=g.neq)
     assert relerr(g.solve(b), o.solve(b)) <= 1e-9
+
+
+def test_revert_to_start_keeps_last_tangent():
+    """revertToStart after a plastic trial: zero state and stress, the last
+    material tangent kept (J2Plasticity.cpp:213-224, :532-543), on the GPU as
+    in the oracle; the next Newton correction is solved with it."""
+    print("case: brick revert to start after a plastic trial", flush=True)
+    n = 3
+    mesh, (g, o) = cube_pair(n, system="GpuBlockPCG")
+    npts = 8 * n ** 3
+    D0 = o.point_data("tangent", npts).copy()
+    U = np.zeros((len(mesh["crds"]), 3))
+    U[:, 2] = -2.2e-3 * mesh["crds"][:, 2]
+    for m in (g, o):
+        m.set_trial_disp(U)
+        m.update()
+        m.revert_to_start()
+    assert np.abs(o.point_data("tangent", npts) - D0).max() > 1e-3 * np.abs(D0).max()  # plastic tangent kept
+    for what in ("tangent", "stress", "plastic"):
+        assert relerr(g.point_data(what, npts), o.point_data(what, npts)) <= RTOL, what
+    assert relerr(g.form_tangent(), o.form_tangent()) <= RTOL
+    for m in (g, o):
+        m.set_load_factor(40.0)
+    B = o.form_unbalance()
+    assert relerr(g.form_unbalance(), B) <= RTOL
+    assert relerr(g.solve(B), o.solve(B)) <= RTOL

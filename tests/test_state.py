@@ -58,3 +58,32 @@ def test_brick_pushover_state_handover():
         assert a.analyze(1) == 0 and b.analyze(1) == 0
     Ua, Ub = a.trial_disp(), b.trial_disp()
     assert np.max(np.abs(Ua - Ub)) <= 1e-12 * np.max(np.abs(Ua))
+
+
+def test_j2_revert_to_start_keeps_last_tangent():
+    """J2Plasticity::revertToStart only zeroes the plastic state, stress and
+    strain (J2Plasticity.cpp:213-224, :532-543): the tangent of the last
+    setTrialStrain stays.  After a plastic trial and revertToStart the
+    material tangent is that plastic tangent (not elastic, not zero) while
+    stress and plastic state are zero; the next formTangent uses it."""
+    mesh = meshes.brick_cube(3)
+    npts = 8 * 27
+    m = _oracle.oracle_model(3, 3)
+    meshes.build_brick_cube(m, mesh, pressure=1.0)
+    m.setup()
+    D0 = m.point_data("tangent", npts).copy()
+    crd = mesh["crds"]
+    U = np.zeros((len(crd), 3))
+    U[:, 2] = -4e-3 * crd[:, 2]  # compression well past first yield
+    m.set_trial_disp(U)
+    m.update()
+    assert (m.point_data("plastic", npts)[:, 6] > 0).all()
+    Dp = m.point_data("tangent", npts).copy()
+    Kp = m.form_tangent()
+    assert np.abs(Dp - D0).max() > 1e-3 * np.abs(D0).max()
+    m.revert_to_start()
+    assert np.array_equal(m.point_data("tangent", npts), Dp)
+    assert not m.point_data("stress", npts).any() and not m.point_data("plastic", npts).any()
+    assert not m.trial_disp().any()
+    # the system tangent of the next Newton step is assembled from it
+    assert np.array_equal(m.form_tangent(), Kp)

@@ -49,7 +49,7 @@ struct J2Group : Group {
     xit.resize(n);
     sig.resize(nsig() * n);
     D.resize(ntan() * n);
-    revert_to_start(d);
+    initial_state(d);
   }
   void update(Domain& d) override {
     if (brick)
@@ -70,8 +70,20 @@ struct J2Group : Group {
   void commit(Domain& d) override { launch_j2_commit(d.stream, np(), ncomp(), ept.p, xit.p, epn.p, xin.p); }
   void revert(Domain&) override {}  // J2Plasticity::revertToLastCommit is a no-op (J2Plasticity.cpp:524)
   void revert_to_start(Domain& d) override {
-    // J2Plasticity::zero + the constructor's plastic_integrator at zero strain:
-    // zero state and stress, elastic tangent (filled by an update at U = 0).
+    // J2Plasticity::revertToStart -> zero (J2Plasticity.cpp:213-224, :532-543):
+    // plastic state, stress and strain; the tangent of the last
+    // plastic_integrator call stays.
+    epn.zero(d.stream);
+    ept.zero(d.stream);
+    xin.zero(d.stream);
+    xit.zero(d.stream);
+    sig.zero(d.stream);
+    XG_HIP(hipStreamSynchronize(d.stream));
+  }
+  void initial_state(Domain& d) {
+    // J2Plasticity::zero + the constructor's plastic_integrator at zero strain
+    // (J2Plasticity.cpp:78-80): zero state and stress, elastic tangent
+    // (filled by an update at U = 0).
     epn.zero(d.stream);
     ept.zero(d.stream);
     xin.zero(d.stream);
