@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "xg_frame.h"
+#include "xg_j2.h"
 #include "xg_kernels.h"
 
 namespace xg {
@@ -25,10 +26,18 @@ struct J2Group : Group {
   DVec<int> dconn, dmat;
   DVec<J2Param> dprm;
   DVec<double> db, dthk;
-  DVec<double> epn, xin, ept, xit, sig, D;
+  DVec<double> epn, xin, ept, xit, sig;
+  // brick: the tangent coefficients c2, c3 of every point (the tangent is
+  // re-formed from them and the stress, xg_j2.h); quad: the upper triangle
+  // of the plane-strain tangent
+  DVec<double> D;
+  // brick after revertToStart: the stress the kept tangent goes with
+  DVec<double> sigtan;
+  bool tan_saved = false;
   int ncomp() const { return brick ? 6 : 4; }
   int nsig() const { return brick ? 6 : 3; }
-  int ntan() const { return brick ? 21 : 6; }
+  int ntan() const { return brick ? 2 : 6; }
+  const double* tan_sig() const { return tan_saved ? sigtan.p : sig.p; }
   int npts() const override { return brick ? 8 : 4; }
   long long np() const { return (long long)nele() * npts(); }
 
@@ -52,6 +61,7 @@ struct J2Group : Group {
     initial_state(d);
   }
   void update(Domain& d) override {
+    tan_saved = false;
     if (brick)
       launch_brick_update(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, d.U.p, d.ndf, d.dt, np(),
                           epn.p, xin.p, ept.p, xit.p, sig.p, D.p, d.nfail.p);
@@ -64,7 +74,7 @@ struct J2Group : Group {
     else launch_quad_resid(d.stream, nele(), dconn.p, d.crd.p, dthk.p, has_b ? db.p : nullptr, np(), sig.p, Re);
   }
   void tangent(Domain& d, double* Ke) override {
-    if (brick) launch_brick_tangent(d.stream, nele(), dconn.p, d.crd.p, np(), D.p, Ke);
+    if (brick) launch_brick_tangent(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, np(), tan_sig(), D.p, Ke);
     else launch_quad_tangent(d.stream, nele(), dconn.p, d.crd.p, dthk.p, np(), D.p, Ke);
   }
   void commit(Domain& d) override { launch_j2_commit(d.stream, np(), ncomp(), ept.p, xit.p, epn.p, xin.p); }
@@ -72,7 +82,12 @@ struct J2Group : Group {
   void revert_to_start(Domain& d) override {
     // J2Plasticity::revertToStart -> zero (J2Plasticity.cpp:213-224, :532-543):
     // plastic state, stress and strain; the tangent of the last
-    // plastic_integrator call stays.
+    // plastic_integrator call stays (a brick keeps the stress it is formed
+    // with until the next update).
+    if (brick && !tan_saved) {
+      sigtan.copy_from(sig, d.stream);
+      tan_saved = true;
+    }
     epn.zero(d.stream);
     ept.zero(d.stream);
     xin.zero(d.stream);
@@ -84,6 +99,7 @@ struct J2Group : Group {
     // J2Plasticity::zero + the constructor's plastic_integrator at zero strain
     // (J2Plasticity.cpp:78-80): zero state and stress, elastic tangent
     // (filled by an update at U = 0).
+    tan_saved = false;
     epn.zero(d.stream);
     ept.zero(d.stream);
     xin.zero(d.stream);
@@ -118,13 +134,29 @@ struct J2Group : Group {
       auto [h, nt] = soa(D, ntan());
       const int m = brick ? 6 : 3;
       out.resize((size_t)n * m * m);
-      for (long long p = 0; p < n; p++)
+      std::vector<double> st, tab_d;
+      std::vector<J2Param> tab;
+      if (brick) {
+        st.resize((size_t)n * 6);
+        XG_HIP(hipMemcpyAsync(st.data(), tan_sig(), st.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              d.stream));
+        XG_HIP(hipStreamSynchronize(d.stream));
+        for (auto& kv : d.j2defs) tab.push_back(kv.second);
+      }
+      for (long long p = 0; p < n; p++) {
+        double Dp[21];
+        if (brick) {
+          double s6[6];
+          for (int c = 0; c < 6; c++) s6[c] = st[c * n + p];
+          j2_tangent_from_stress(tab[mat_h[p / 8]], s6, h[p], h[n + p], Dp);
+        }
         for (int a = 0; a < m; a++)
           for (int b = a; b < m; b++) {
-            const double v = h[(long long)tri_index(m, a, b) * n + p];
+            const double v = brick ? Dp[tri_index(m, a, b)] : h[(long long)tri_index(m, a, b) * n + p];
             out[p * m * m + a * m + b] = v;
             out[p * m * m + b * m + a] = v;
           }
+      }
       (void)nt;
       return m * m;
     }

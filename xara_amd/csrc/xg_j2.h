@@ -7,6 +7,14 @@
 // components in the order 00,11,22,01,12,20 (J2ThreeDimensional.cpp:189-203);
 // the consistent tangent is the Voigt 6x6 matrix of J2ThreeDimensional::
 // getTangent (:205-234) -- symmetric, so only its upper triangle is formed.
+//
+// The tangent of plastic_integrator (J2Plasticity.cpp:361-391) is
+//   K I(x)I + 2G Idev + c2 n(x)n + c3 (Idev - n(x)n)
+// with two scalars c2, c3 (zero at an elastic point) and the unit normal n
+// of the trial deviatoric stress.  The radial return keeps the deviatoric
+// stress parallel to n, so a point stores c2 and c3 only (16 bytes instead
+// of the 168 of the upper triangle) and the tangent is re-formed from the
+// stress where it is used.
 #pragma once
 #include "xg_common.h"
 
@@ -16,8 +24,52 @@ struct J2Out {
   double ep[6];   // plastic strain at n+1
   double xi;      // hardening variable at n+1
   double sig[6];  // stress
-  double D[21];   // upper triangle of the 6x6 tangent, by rows
+  double c2, c3;  // tangent coefficients
+  double n[6];    // unit normal of the trial deviatoric stress (0 if none)
 };
+
+// Upper triangle (by rows) of the 6x6 tangent from its coefficients, in the
+// summation order of J2Plasticity.cpp:377-384.
+__host__ __device__ __forceinline__ void j2_tangent_upper(const J2Param& p, double c2, double c3,
+                                                          const double n[6], double D[21]) {
+  const double k_twoG = 2.0 * p.G;
+  int idx = 0;
+#pragma unroll
+  for (int a = 0; a < 6; a++) {
+#pragma unroll
+    for (int b = a; b < 6; b++) {
+      const bool nn = (a < 3) && (b < 3);
+      const bool ss = (a >= 3) && (b >= 3);
+      const double IbunI = nn ? 1.0 : 0.0;
+      const double IIdev = nn ? ((a == b) ? 2. / 3. : -1. / 3.) : (ss ? ((a == b) ? 0.5 : 0.0) : 0.0);
+      const double NbunN = n[a] * n[b];
+      double t = p.K * IbunI;
+      t += k_twoG * IIdev;
+      t += c2 * NbunN;
+      t += c3 * (IIdev - NbunN);
+      D[idx++] = t;
+    }
+  }
+}
+
+// Unit normal of the deviatoric part of a stress (tensor components; the
+// norm counts the off-diagonals twice), 0 for a hydrostatic stress.
+__host__ __device__ __forceinline__ void j2_stress_normal(const double sig[6], double n[6]) {
+  const double m = (sig[0] + sig[1] + sig[2]) * (1.0 / 3.0);
+  const double s[6] = {sig[0] - m, sig[1] - m, sig[2] - m, sig[3], sig[4], sig[5]};
+  const double nt = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+  const double inv = nt > 0.0 ? 1.0 / sqrt(nt) : 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) n[k] = s[k] * inv;
+}
+
+// The tangent re-formed from a point's stress and coefficients.
+__host__ __device__ __forceinline__ void j2_tangent_from_stress(const J2Param& p, const double sig[6], double c2,
+                                                                double c3, double D[21]) {
+  double n[6];
+  j2_stress_normal(sig, n);
+  j2_tangent_upper(p, c2, c3, n, D);
+}
 
 __device__ __forceinline__ double j2_q(const J2Param& p, double xi) {
   return p.H * xi + p.sigInf + (p.sig0 - p.sigInf) * exp(-p.delta * xi);
@@ -103,26 +155,10 @@ __device__ __forceinline__ bool j2_return_map(const J2Param& p, double dt, const
   o.sig[2] += p.K * trace;
 
   const double c1 = -4.0 * p.G * p.G;
-  const double c2 = c1 * theta_inv;
-  const double c3 = c1 * gamma * inv_norm_tau;
-  const double k_twoG = twoG;
-  int idx = 0;
+  o.c2 = c1 * theta_inv;
+  o.c3 = c1 * gamma * inv_norm_tau;
 #pragma unroll
-  for (int a = 0; a < 6; a++) {
-#pragma unroll
-    for (int b = a; b < 6; b++) {
-      const bool nn = (a < 3) && (b < 3);
-      const bool ss = (a >= 3) && (b >= 3);
-      const double IbunI = nn ? 1.0 : 0.0;
-      const double IIdev = nn ? ((a == b) ? 2. / 3. : -1. / 3.) : (ss ? ((a == b) ? 0.5 : 0.0) : 0.0);
-      const double NbunN = n[a] * n[b];
-      double t = p.K * IbunI;
-      t += k_twoG * IIdev;
-      t += c2 * NbunN;
-      t += c3 * (IIdev - NbunN);
-      o.D[idx++] = t;
-    }
-  }
+  for (int k = 0; k < 6; k++) o.n[k] = n[k];
   return true;
 }
 

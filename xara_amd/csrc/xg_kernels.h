@@ -8,11 +8,11 @@ namespace xg {
 void launch_brick_update(hipStream_t s, int nele, const int* conn, const int* mat, const J2Param* prm,
                          const double* crd, const double* U, int ndf, double dt, long long npts,
                          const double* epn, const double* xin, double* ept, double* xit, double* sig,
-                         double* D, int* nfail);
+                         double* tc, int* nfail);
 void launch_brick_resid(hipStream_t s, int nele, const int* conn, const double* crd,
                         const double* bforce, long long npts, const double* sig, double* Re);
-void launch_brick_tangent(hipStream_t s, int nele, const int* conn, const double* crd,
-                          long long npts, const double* D, double* Ke);
+void launch_brick_tangent(hipStream_t s, int nele, const int* conn, const int* mat, const J2Param* prm,
+                          const double* crd, long long npts, const double* sig, const double* tc, double* Ke);
 void launch_j2_commit(hipStream_t s, long long npts, int ncomp, const double* ept, const double* xit,
                       double* epn, double* xin);
 

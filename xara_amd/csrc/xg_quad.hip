@@ -106,12 +106,14 @@ __global__ __launch_bounds__(256) void k_quad_update(
   sig[p] = o.sig[0];
   sig[npts + p] = o.sig[1];
   sig[2 * npts + p] = o.sig[3];
-  D[p] = o.D[tri_index(6, 0, 0)];
-  D[npts + p] = o.D[tri_index(6, 0, 1)];
-  D[2 * npts + p] = o.D[tri_index(6, 0, 3)];
-  D[3 * npts + p] = o.D[tri_index(6, 1, 1)];
-  D[4 * npts + p] = o.D[tri_index(6, 1, 3)];
-  D[5 * npts + p] = o.D[tri_index(6, 3, 3)];
+  double D6[21];
+  j2_tangent_upper(prm, o.c2, o.c3, o.n, D6);
+  D[p] = D6[tri_index(6, 0, 0)];
+  D[npts + p] = D6[tri_index(6, 0, 1)];
+  D[2 * npts + p] = D6[tri_index(6, 0, 3)];
+  D[3 * npts + p] = D6[tri_index(6, 1, 1)];
+  D[4 * npts + p] = D6[tri_index(6, 1, 3)];
+  D[5 * npts + p] = D6[tri_index(6, 3, 3)];
 }
 
 __device__ __forceinline__ void quad_load_coords(long long e, const int* conn, const double* crd,
