@@ -215,3 +215,66 @@ def test_revert_to_start_keeps_last_tangent():
     B = o.form_unbalance()
     assert relerr(g.form_unbalance(), B) <= RTOL
     assert relerr(g.solve(B), o.solve(B)) <= RTOL
+
+
+def _pcg_iterations(K, b, blocks, rtol):
+    """PCG iteration count of the GPU solver's recurrence (x0 = 0, stop when
+    r.r <= rtol^2 b.b) with an exact block-Jacobi preconditioner."""
+    Minv = np.zeros_like(K)
+    for eqs in blocks:
+        Minv[np.ix_(eqs, eqs)] = np.linalg.inv(K[np.ix_(eqs, eqs)])
+    x, r = np.zeros_like(b), b.copy()
+    z = Minv @ r
+    p, rz, bb = z.copy(), r @ z, b @ b
+    for it in range(1, 10 * len(b)):
+        q = K @ p
+        alpha = rz / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = Minv @ r
+        rz_new = r @ z
+        if r @ r <= rtol * rtol * bb:
+            return it
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return -1
+
+
+@pytest.mark.parametrize("system", ["GpuPCG", "GpuBlockPCG"])
+def test_block_jacobi_with_partly_fixed_nodes(system):
+    """Roller base (z fixed, x and y free) plus two pinned corners: nodal
+    blocks of 3 and of 2 free DOFs.  The GPU PCG takes the iterations of an
+    exact (block-)Jacobi PCG run in numpy on the same matrix, and both
+    preconditioners give the direct solution."""
+    print(f"case: partly fixed nodes {system}", flush=True)
+    n = 4
+    mesh = meshes.brick_cube(n)
+    base = mesh["fixed_nodes"]
+    flags = np.zeros((len(base), 3), np.int32)
+    flags[:, 2] = 1
+    flags[0, :] = 1
+    flags[-1, :] = 1
+    rtol = 1e-10
+    g, o = Model(3, 3), _oracle.oracle_model(3, 3)
+    for m in (g, o):
+        meshes.build_brick_cube(m, dict(mesh, fixed_nodes=base[:0]), pressure=1.0)
+        m.fix_many(base, flags)
+        m.system(system, rtol, 20000)
+        m.setup()
+        m.form_tangent(values=False)
+    K = np.zeros((g.neq, g.neq))
+    rp, ci = g.csr_pattern()
+    K[np.repeat(np.arange(g.neq), np.diff(rp)), ci] = g.form_tangent()
+    eqs = g.node_eqns()
+    if system == "GpuBlockPCG":
+        blocks = [row[row >= 0] for row in eqs if (row >= 0).any()]
+        assert {len(bl) for bl in blocks} == {2, 3}
+    else:
+        blocks = [[a] for a in range(g.neq)]
+    b = np.random.default_rng(5).normal(size=g.neq)
+    c0 = g.counters()["cg_iters"]
+    x = g.solve(b)
+    its = g.counters()["cg_iters"] - c0
+    ref = _pcg_iterations(K, b, blocks, rtol)
+    assert abs(its - ref) <= 2, (its, ref)
+    assert relerr(x, o.solve(b)) <= 1e-9

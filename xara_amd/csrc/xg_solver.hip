@@ -10,6 +10,7 @@
 // Dot products reduce per block with a fixed LDS tree and then over blocks
 // in a single-block pass, so every PCG iteration is bitwise reproducible.
 #include <algorithm>
+#include "../../include/xara_gpu.h"
 #include "xg_common.h"
 #include "xg_kernels.h"
 
@@ -400,17 +401,22 @@ __global__ __launch_bounds__(256) void k_cg_init_scal(int nb, const double* __re
 }
 
 // Jacobi / nodal block-Jacobi inverse from the assembled CSR matrix.
-// One thread per node block: gathers the m x m diagonal block (m <= 6),
-// inverts it by Gauss-Jordan with partial pivoting and writes its rows.
-__global__ void k_block_inverse(int nblk, const int* __restrict__ blk_first,
+// One thread per node block: gathers the m x m diagonal block (m <= M, the
+// nodal DOF count), inverts it by Gauss-Jordan with partial pivoting and
+// writes its rows.  The block is padded to M x M with identity rows so that
+// every index is a compile-time constant (the matrix stays in registers);
+// the padding never mixes with the real rows, whose arithmetic is that of
+// the unpadded m x m elimination.
+template <int M>
+__global__ __launch_bounds__(128) void k_block_inverse(int nblk, const int* __restrict__ blk_first,
                                 const unsigned char* __restrict__ blk_size,
                                 const int* __restrict__ rowptr, const int* __restrict__ colind,
-                                const double* __restrict__ vals, int bw, double* __restrict__ binv,
+                                const double* __restrict__ vals, double* __restrict__ binv,
                                 int* __restrict__ bad) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nblk) return;
   const int f = blk_first[k], m = blk_size[k];
-  if (bw == 1) {
+  if (M == 1) {
     // Jacobi: each equation its own block
     for (int j = 0; j < m; j++) {
       const int row = f + j;
@@ -422,40 +428,63 @@ __global__ void k_block_inverse(int nblk, const int* __restrict__ blk_first,
     }
     return;
   }
-  double a[6][12];
-  for (int i = 0; i < m; i++) {
-    for (int j = 0; j < m; j++) a[i][j] = 0.0;
-    for (int j = 0; j < m; j++) a[i][m + j] = (i == j) ? 1.0 : 0.0;
-    const int row = f + i;
-    for (int q = rowptr[row]; q < rowptr[row + 1]; q++) {
-      const int c = colind[q] - f;
-      if (c >= 0 && c < m) a[i][c] = vals[q];
+  double a[M][2 * M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+#pragma unroll
+    for (int j = 0; j < 2 * M; j++) a[i][j] = (j - M == i || (i >= m && j == i)) ? 1.0 : 0.0;
+    if (i < m) {
+      // the block's columns f .. f+m-1 follow each other in the sorted row
+      const int row = f + i, e = rowptr[row + 1];
+      int q = rowptr[row];
+      while (q < e && colind[q] < f) q++;
+#pragma unroll
+      for (int j = 0; j < M; j++)
+        if (j < m && q < e && colind[q] == f + j) a[i][j] = vals[q++];
     }
   }
-  for (int c = 0; c < m; c++) {
+#pragma unroll
+  for (int c = 0; c < M; c++) {
     int piv = c;
-    for (int i = c + 1; i < m; i++)
-      if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
-    if (a[piv][c] == 0.0) {
+    double best = fabs(a[c][c]);
+#pragma unroll
+    for (int i = c + 1; i < M; i++) {
+      const double v = fabs(a[i][c]);
+      if (v > best) {
+        best = v;
+        piv = i;
+      }
+    }
+    if (best == 0.0) {
       bad[0] = 1;
       return;
     }
-    if (piv != c)
-      for (int j = 0; j < 2 * m; j++) {
-        double tmp = a[c][j];
-        a[c][j] = a[piv][j];
-        a[piv][j] = tmp;
+#pragma unroll
+    for (int i = c + 1; i < M; i++) {
+      const bool sw = (piv == i);
+#pragma unroll
+      for (int j = 0; j < 2 * M; j++) {
+        const double tc = a[c][j], ti = a[i][j];
+        a[c][j] = sw ? ti : tc;
+        a[i][j] = sw ? tc : ti;
       }
+    }
     const double inv = 1.0 / a[c][c];
-    for (int j = 0; j < 2 * m; j++) a[c][j] *= inv;
-    for (int i = 0; i < m; i++)
+#pragma unroll
+    for (int j = 0; j < 2 * M; j++) a[c][j] *= inv;
+#pragma unroll
+    for (int i = 0; i < M; i++)
       if (i != c) {
         const double fct = a[i][c];
-        for (int j = 0; j < 2 * m; j++) a[i][j] -= fct * a[c][j];
+#pragma unroll
+        for (int j = 0; j < 2 * M; j++) a[i][j] -= fct * a[c][j];
       }
   }
-  for (int i = 0; i < m; i++)
-    for (int j = 0; j < bw; j++) binv[(long long)(f + i) * bw + j] = (j < m) ? a[i][m + j] : 0.0;
+#pragma unroll
+  for (int i = 0; i < M; i++)
+    if (i < m)
+#pragma unroll
+      for (int j = 0; j < M; j++) binv[(long long)(f + i) * M + j] = (j < m) ? a[i][M + j] : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -591,8 +620,16 @@ void launch_block_inverse(hipStream_t s, int nblk, const int* first, const unsig
                           const int* rowptr, const int* colind, const double* vals, int bw, double* binv,
                           int* bad) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(k_block_inverse, dim3(ceil_div(nblk, 128)), dim3(128), 0, s, nblk, first, size, rowptr,
-                     colind, vals, bw, binv, bad);
+  const dim3 g(ceil_div(nblk, 128)), b(128);
+  switch (bw) {
+    case 1: hipLaunchKernelGGL(k_block_inverse<1>, g, b, 0, s, nblk, first, size, rowptr, colind, vals, binv, bad); break;
+    case 2: hipLaunchKernelGGL(k_block_inverse<2>, g, b, 0, s, nblk, first, size, rowptr, colind, vals, binv, bad); break;
+    case 3: hipLaunchKernelGGL(k_block_inverse<3>, g, b, 0, s, nblk, first, size, rowptr, colind, vals, binv, bad); break;
+    case 4: hipLaunchKernelGGL(k_block_inverse<4>, g, b, 0, s, nblk, first, size, rowptr, colind, vals, binv, bad); break;
+    case 5: hipLaunchKernelGGL(k_block_inverse<5>, g, b, 0, s, nblk, first, size, rowptr, colind, vals, binv, bad); break;
+    case 6: hipLaunchKernelGGL(k_block_inverse<6>, g, b, 0, s, nblk, first, size, rowptr, colind, vals, binv, bad); break;
+    default: throw Error(XG_ERR_ARG, "block-Jacobi block size must be 1..6");
+  }
 }
 void launch_incr_disp(hipStream_t s, long long ndofs, const int* eqmap, const double* dU, double scale,
                       double* U) {
