@@ -78,6 +78,40 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+__device__ __forceinline__ double read_lane_d(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Wavefront totals of the first N of K per-lane values (K = 8 or 16) by a
+// transposed butterfly: in each of the first log2(K) exchange steps a lane
+// keeps one half of its values, receives its partner's share of that half
+// and sends the other half, so the K totals take K - 1 + log2(64/K)
+// shuffles instead of 6K.  The total of value v ends in lanes (64/K) v ..
+// (64/K) v + 64/K - 1 and is returned to every lane in x[v].
+template <int K, int N>
+__device__ __forceinline__ void wave_sums(double (&x)[K], int lane) {
+  constexpr int G = 64 / K;
+#pragma unroll
+  for (int c = K; c > 1; c >>= 1) {
+    const int o = G * c / 2;  // 32, 16, ... for c = K, K/2, ...
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < c / 2; i++) {
+      const double keep = up ? x[i + c / 2] : x[i];
+      const double send = up ? x[i] : x[i + c / 2];
+      x[i] = keep + __shfl_xor(send, o, 64);
+    }
+  }
+  double t = x[0];
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+#pragma unroll
+  for (int v = 0; v < N; v++) x[v] = read_lane_d(t, G * v);
+}
+
 // One Steel02 fiber: committed state -> trial state; returns stress and
 // tangent.
 __device__ __forceinline__ void steel_fiber(const FrameDev& f, const Steel02Param& p, long long idx, double strain,
@@ -255,17 +289,15 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
       }
     }
   }
-  double k0 = S.k0, k1 = S.k1, k2 = S.k2, k5 = S.k5, k6 = S.k6, k10 = S.k10, s0 = S.s0, s1 = S.s1, s2 = S.s2;
-  k0 = wave_sum(k0);
-  k1 = wave_sum(k1);
-  k5 = wave_sum(k5);
-  s0 = wave_sum(s0);
-  s1 = wave_sum(s1);
+  double k0, k1, k2 = 0.0, k5, k6 = 0.0, k10 = 0.0, s0, s1, s2 = 0.0;
   if (D3) {
-    k2 = wave_sum(k2);
-    k6 = wave_sum(k6);
-    k10 = wave_sum(k10);
-    s2 = wave_sum(s2);
+    double x[16] = {S.k0, S.k1, S.k2, S.k5, S.k6, S.k10, S.s0, S.s1, S.s2, 0, 0, 0, 0, 0, 0, 0};
+    wave_sums<16, 9>(x, lane);
+    k0 = x[0], k1 = x[1], k2 = x[2], k5 = x[3], k6 = x[4], k10 = x[5], s0 = x[6], s1 = x[7], s2 = x[8];
+  } else {
+    double x[8] = {S.k0, S.k1, S.k5, S.s0, S.s1, 0, 0, 0};
+    wave_sums<8, 5>(x, lane);
+    k0 = x[0], k1 = x[1], k5 = x[2], s0 = x[3], s1 = x[4];
   }
   if (lane == 0) {
     const long long n = f.nsec;
