@@ -17,8 +17,7 @@
 namespace xg {
 
 Ensemble::~Ensemble() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
+  drop_graph();
   frames.reset();
   if (merged.stream) {
     (void)hipStreamDestroy(merged.stream);
@@ -309,6 +308,7 @@ void Ensemble::setup() {
   model_ele.assign(ele_off.begin(), ele_off.end());
   node_off_h = node_off;
   E = EnsDev{};
+  parity = 0;
   E.nmodel = nm;
   E.f = frame_dev(G);
   E.ndf = 3;
@@ -367,11 +367,27 @@ void Ensemble::setup() {
 }
 
 void Ensemble::drop_graph() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  graph_exec = nullptr;
-  graph = nullptr;
+  for (int k = 0; k < 2; k++) {
+    if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
+    if (graph[k]) (void)hipGraphDestroy(graph[k]);
+    graph_exec[k] = nullptr;
+    graph[k] = nullptr;
+  }
   graph_dt = -1;
+}
+
+// Commit of the fiber pools: the trial pools become the committed ones and
+// the old committed pools the next step's trial pools.  Converged models'
+// trial state is their new committed state; a failed model's trial pools
+// were reset to its committed state by k_ens_commit, so both pools agree;
+// the first update of a step reads Concrete02's previous trial stress and
+// tangent from the committed pool (section_update<.., true>).
+void Ensemble::flip() {
+  std::swap(E.f.st_c, E.f.st_t);
+  std::swap(E.f.kon_c, E.f.kon_t);
+  std::swap(E.f.cs_c, E.f.cs_t);
+  std::swap(E.f.el_c, E.f.el_t);
+  parity ^= 1;
 }
 
 void Ensemble::set_order(int mode) {
@@ -400,15 +416,20 @@ int Ensemble::analyze(int nsteps, double dt) {
   hipStream_t s = merged.stream;
   if (E.dt != dt) drop_graph();
   E.dt = dt;
-  if (use_graph && !graph_exec) {
-    XG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    enqueue_step(s);
-    XG_HIP(hipStreamEndCapture(s, &graph));
-    XG_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+  if (use_graph && !graph_exec[parity]) {
+    // capture both parities (capture records, it runs nothing)
+    for (int k = 0; k < 2; k++) {
+      XG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      enqueue_step(s);
+      XG_HIP(hipStreamEndCapture(s, &graph[parity]));
+      XG_HIP(hipGraphInstantiate(&graph_exec[parity], graph[parity], nullptr, nullptr, 0));
+      flip();
+    }
   }
   for (int st = 0; st < nsteps; st++) {
-    if (use_graph) XG_HIP(hipGraphLaunch(graph_exec, s));
+    if (use_graph) XG_HIP(hipGraphLaunch(graph_exec[parity], s));
     else enqueue_step(s);
+    flip();
   }
   XG_HIP(hipStreamSynchronize(s));
   steps_done += nsteps;

@@ -13,8 +13,10 @@
 //                     banded Cholesky and solve (one wavefront for bands up to
 //                     24), Newmark::update, fiber/section
 //                     update, unbalance, CTestNormDispIncr / NormUnbalance.
-//   k_ens_commit   -- Domain::commit of converged models; revert of models
-//                     whose Newton failed (they stop, as analyze returns < 0).
+//   k_ens_commit   -- Domain::commit of converged models (nodal state; the
+//                     fiber pools commit by exchanging the committed and
+//                     trial pool pointers); revert of models whose Newton
+//                     failed (they stop, as analyze returns < 0).
 #include "xg_ensemble.h"
 #include "xg_frame_impl.h"
 
@@ -48,12 +50,13 @@ __device__ __forceinline__ double path_accel(const EnsDev& E, int m, double t) {
   return E.gm_fac[m] * (v1 + (v2 - v1) * (incr - i1));
 }
 
-// domain update + transient unbalance of model m (all threads of the block)
-template <int NT>
+// domain update + transient unbalance of model m (all threads of the block);
+// First: the step's first update (the predictor's)
+template <int NT, bool First>
 __device__ void model_update_and_unbalance(const EnsDev& E, int m, double ag) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long long s0 = (long long)E.ele_off[m] * E.f.nip, s1 = (long long)E.ele_off[m + 1] * E.f.nip;
-  for (long long sec = s0 + wave; sec < s1; sec += NT / 64) section_update<false>(E.f, sec, lane, E.U);
+  for (long long sec = s0 + wave; sec < s1; sec += NT / 64) section_update<false, First>(E.f, sec, lane, E.U);
   __syncthreads();
   for (int e = E.ele_off[m] + tid; e < E.ele_off[m + 1]; e += NT) element_resid<false>(E.f, e, E.Re);
   __syncthreads();
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(NT) void k_ens_predict(EnsDev E) {
     // (Newmark.cpp newStep: time += deltaT), formed the same way everywhere
     const double ag = path_accel(E, m, E.time[m] + h);
     __syncthreads();
-    model_update_and_unbalance<NT>(E, m, ag);
+    model_update_and_unbalance<NT, true>(E, m, ag);
     if (tid == 0) {
       E.iter[m] = 0;
       E.status[m] = 0;
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(NT) void k_ens_iterate(EnsDev E) {
     }
     __syncthreads();
     const double ag = path_accel(E, m, E.time[m] + h);
-    model_update_and_unbalance<NT>(E, m, ag);
+    model_update_and_unbalance<NT, false>(E, m, ag);
     // ---- convergence test ----
     double v = 0.0;
     const double* w = E.test_kind == 1 ? E.B : E.X;
@@ -320,16 +323,8 @@ __global__ __launch_bounds__(NT) void k_ens_commit(EnsDev E) {
         E.Vc[d] = E.V[d];
         E.Ac[d] = E.A[d];
       }
-      const FrameDev& f = E.f;
-      const long long sb0 = f.steel_base[s0], sb1 = s1 < f.nsec ? f.steel_base[s1] : f.nsteel;
-      const long long cb0 = f.conc_base[s0], cb1 = s1 < f.nsec ? f.conc_base[s1] : f.nconc;
-      const long long eb0 = f.elas_base[s0], eb1 = s1 < f.nsec ? f.elas_base[s1] : f.nelas;
-      for (int c = 0; c < kSteelHist; c++)
-        for (long long i = sb0 + tid; i < sb1; i += NT) f.st_c[c * f.nsteel + i] = f.st_t[c * f.nsteel + i];
-      for (long long i = sb0 + tid; i < sb1; i += NT) f.kon_c[i] = f.kon_t[i];
-      for (int c = 0; c < kConcHist; c++)
-        for (long long i = cb0 + tid; i < cb1; i += NT) f.cs_c[c * f.nconc + i] = f.cs_t[c * f.nconc + i];
-      for (long long i = eb0 + tid; i < eb1; i += NT) f.el_c[i] = f.el_t[i];
+      // the fiber state commits by the host exchanging the committed and
+      // trial pools after this kernel (Ensemble::flip)
     } else {
       // failed (or never converged within the launched iterations):
       // Domain::revertToLastCommit and stop this model

@@ -49,8 +49,12 @@ struct Ensemble {
   DVec<int> status, iter;
   DVec<double> time, norm;
   DVec<long long> newton_count;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  // one captured step per pool parity: the committed and trial fiber pools
+  // exchange roles at every commit (flip), so the two graphs differ only in
+  // which pool pointers their kernels receive
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  int parity = 0;
   double graph_dt = -1;
 
   ~Ensemble();
@@ -62,6 +66,7 @@ struct Ensemble {
   void set_order(int mode);
   void enqueue_step(hipStream_t s);
   void drop_graph();
+  void flip();
 };
 
 }  // namespace xg

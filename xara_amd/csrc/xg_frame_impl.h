@@ -210,8 +210,11 @@ struct SecSums {
 // Section update, one wavefront per section instance (sec is the same for
 // every lane of the wave).  Concrete02 runs, the bulk of an RC section, are
 // walked two fibers per lane per pass so that both fibers' history loads are
-// in flight together.
-template <bool D3>
+// in flight together.  PrevCommitted: the first update after a commit, when
+// Concrete02's previous trial stress and tangent (kept on its early return)
+// equal the committed ones and are read from there, so that the trial
+// arrays need not hold a copy of the committed state.
+template <bool D3, bool PrevCommitted = false>
 __device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
   const int e = (int)(sec / f.nip);
   double d[4];
@@ -256,8 +259,9 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
           ca[k] = f.cs_c[k * n + ia];
           cb2[k] = f.cs_c[k * n + ib];
         }
-        double sa = f.cs_t[3 * n + ia], ea = f.cs_t[4 * n + ia];
-        double sb2 = f.cs_t[3 * n + ib], eb2 = f.cs_t[4 * n + ib];
+        const double* prev = PrevCommitted ? f.cs_c : f.cs_t;
+        double sa = prev[3 * n + ia], ea = prev[4 * n + ia];
+        double sb2 = prev[3 * n + ib], eb2 = prev[4 * n + ib];
         const double Aa = f.fA[fi], Ab = two ? f.fA[fj] : 0.0;
         const double ea_ = D3 ? d[0] - ya * d[1] + za * d[2] : d[0] - ya * d[1];
         const double eb_ = D3 ? d[0] - yb * d[1] + zb * d[2] : d[0] - yb * d[1];
