@@ -600,6 +600,9 @@ static void commit_domain(Dom& d) {
   for (auto* e : d.eles) e->commit();
 }
 
+// Domain::revertToLastCommit (Domain.cpp:1925-1939): nodes and elements
+// revert, then the domain updates once more at the committed displacements
+// with dT = 0.
 static void revert_domain(Dom& d) {
   for (auto& n : d.nodes)
     for (int j = 0; j < d.ndf; j++) {
@@ -608,6 +611,10 @@ static void revert_domain(Dom& d) {
       n.A[j] = n.Ac[j];
     }
   for (auto* e : d.eles) e->revert();
+  const double dt_keep = d.dt;
+  d.dt = 0.0;
+  domain_update(d);
+  d.dt = dt_keep;
 }
 
 // ---- convergence tests ------------------------------------------------------
@@ -1328,12 +1335,20 @@ int orc_revert_to_last_commit(xg_domain* d) {
   revert_domain(*d);
   return 0;
 }
+// Domain::revertToStart (Domain.cpp:1941-1958): nodes and elements to their
+// initial state, then one update at zero displacement with dT = 0.
 int orc_revert_to_start(xg_domain* d) {
   for (auto& n : d->nodes)
     for (int j = 0; j < d->ndf; j++) n.U[j] = n.Uc[j] = n.V[j] = n.Vc[j] = n.A[j] = n.Ac[j] = 0.0;
   for (auto* e : d->eles) e->revert_to_start();
   d->lambda = 0;
   d->time = 0;
+  if (d->ready) {
+    const double dt_keep = d->dt;
+    d->dt = 0.0;
+    domain_update(*d);
+    d->dt = dt_keep;
+  }
   return 0;
 }
 int orc_set_load_factor(xg_domain* d, double lambda) {

@@ -191,11 +191,12 @@ def test_pcg_solve_csr_layout():
     assert relerr(g.solve(b), o.solve(b)) <= 1e-9
 
 
-def test_revert_to_start_keeps_last_tangent():
-    """revertToStart after a plastic trial: zero state and stress, the last
-    material tangent kept (J2Plasticity.cpp:213-224, :532-543), on the GPU as
-    in the oracle; the next Newton correction is solved with it."""
-    print("case: brick revert to start after a plastic trial", flush=True)
+def test_revert_to_start_and_to_last_commit():
+    """Domain::revertToStart after a plastic trial gives back the fresh model
+    (its closing update at zero displacement forms the elastic tangent,
+    Domain.cpp:1941-1958); Domain::revertToLastCommit updates at the
+    committed displacements (Domain.cpp:1925-1939).  GPU and oracle alike."""
+    print("case: brick revert to start / to last commit", flush=True)
     n = 3
     mesh, (g, o) = cube_pair(n, system="GpuBlockPCG")
     npts = 8 * n ** 3
@@ -205,16 +206,24 @@ def test_revert_to_start_keeps_last_tangent():
     for m in (g, o):
         m.set_trial_disp(U)
         m.update()
+    assert (o.point_data("plastic", npts)[:, 6] > 0).all()
+    for m in (g, o):
         m.revert_to_start()
-    assert np.abs(o.point_data("tangent", npts) - D0).max() > 1e-3 * np.abs(D0).max()  # plastic tangent kept
+    assert np.array_equal(o.point_data("tangent", npts), D0)
     for what in ("tangent", "stress", "plastic"):
         assert relerr(g.point_data(what, npts), o.point_data(what, npts)) <= RTOL, what
     assert relerr(g.form_tangent(), o.form_tangent()) <= RTOL
+    # commit a plastic state, try a larger trial, revert to the commit
     for m in (g, o):
-        m.set_load_factor(40.0)
-    B = o.form_unbalance()
-    assert relerr(g.form_unbalance(), B) <= RTOL
-    assert relerr(g.solve(B), o.solve(B)) <= RTOL
+        m.set_trial_disp(U)
+        m.update()
+        m.commit()
+        m.set_trial_disp(2 * U)
+        m.update()
+        m.revert()
+    for what in ("stress", "tangent", "plastic"):
+        assert relerr(g.point_data(what, npts), o.point_data(what, npts)) <= RTOL, what
+    assert relerr(g.form_unbalance(), o.form_unbalance()) <= RTOL
 
 
 def _pcg_iterations(K, b, blocks, rtol):

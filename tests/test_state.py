@@ -60,30 +60,52 @@ def test_brick_pushover_state_handover():
     assert np.max(np.abs(Ua - Ub)) <= 1e-12 * np.max(np.abs(Ua))
 
 
-def test_j2_revert_to_start_keeps_last_tangent():
-    """J2Plasticity::revertToStart only zeroes the plastic state, stress and
-    strain (J2Plasticity.cpp:213-224, :532-543): the tangent of the last
-    setTrialStrain stays.  After a plastic trial and revertToStart the
-    material tangent is that plastic tangent (not elastic, not zero) while
-    stress and plastic state are zero; the next formTangent uses it."""
+def test_revert_to_start_after_plastic_trial_is_a_fresh_model():
+    """J2Plasticity::revertToStart zeroes only the plastic state, stress and
+    strain (J2Plasticity.cpp:213-224) -- the material keeps its last tangent
+    -- but Domain::revertToStart ends with an update at zero displacement
+    (Domain.cpp:1941-1958), which forms the elastic tangent again: the model
+    is then the fresh model, and a pushover from there is the fresh one."""
+    mesh = meshes.brick_cube(3)
+    npts = 8 * 27
+    fresh, m = (_oracle.oracle_model(3, 3) for _ in range(2))
+    for x in (fresh, m):
+        meshes.build_brick_cube(x, mesh, pressure=1.0)
+        x.setup()
+    U = np.zeros((len(mesh["crds"]), 3))
+    U[:, 2] = -4e-3 * mesh["crds"][:, 2]  # compression well past first yield
+    m.set_trial_disp(U)
+    m.update()
+    assert (m.point_data("plastic", npts)[:, 6] > 0).all()
+    m.revert_to_start()
+    for what in ("stress", "tangent", "plastic", "strain"):
+        assert np.array_equal(m.point_data(what, npts), fresh.point_data(what, npts)), what
+    assert np.array_equal(m.form_tangent(), fresh.form_tangent())
+    for x in (fresh, m):
+        x.test("NormUnbalance", 1e-8, 30)
+        x.integrator("LoadControl", 40.0)
+        assert x.analyze(3) == 0
+    assert np.array_equal(m.trial_disp(), fresh.trial_disp())
+
+
+def test_revert_to_last_commit_updates_at_the_committed_state():
+    """Domain::revertToLastCommit ends with an update at the committed
+    displacements (Domain.cpp:1925-1939): the stresses are those of the
+    committed state again, not of the abandoned trial."""
     mesh = meshes.brick_cube(3)
     npts = 8 * 27
     m = _oracle.oracle_model(3, 3)
     meshes.build_brick_cube(m, mesh, pressure=1.0)
     m.setup()
-    D0 = m.point_data("tangent", npts).copy()
-    crd = mesh["crds"]
-    U = np.zeros((len(crd), 3))
-    U[:, 2] = -4e-3 * crd[:, 2]  # compression well past first yield
+    U = np.zeros((len(mesh["crds"]), 3))
+    U[:, 2] = -1.0e-3 * mesh["crds"][:, 2]
     m.set_trial_disp(U)
     m.update()
-    assert (m.point_data("plastic", npts)[:, 6] > 0).all()
-    Dp = m.point_data("tangent", npts).copy()
-    Kp = m.form_tangent()
-    assert np.abs(Dp - D0).max() > 1e-3 * np.abs(D0).max()
-    m.revert_to_start()
-    assert np.array_equal(m.point_data("tangent", npts), Dp)
-    assert not m.point_data("stress", npts).any() and not m.point_data("plastic", npts).any()
-    assert not m.trial_disp().any()
-    # the system tangent of the next Newton step is assembled from it
-    assert np.array_equal(m.form_tangent(), Kp)
+    m.commit()
+    s_commit = m.point_data("stress", npts).copy()
+    m.set_trial_disp(3 * U)
+    m.update()
+    assert not np.allclose(m.point_data("stress", npts), s_commit)
+    m.revert()
+    assert np.array_equal(m.point_data("stress", npts), s_commit)
+    assert np.array_equal(m.trial_disp(), U)

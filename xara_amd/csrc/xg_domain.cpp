@@ -31,13 +31,9 @@ struct J2Group : Group {
   // re-formed from them and the stress, xg_j2.h); quad: the upper triangle
   // of the plane-strain tangent
   DVec<double> D;
-  // brick after revertToStart: the stress the kept tangent goes with
-  DVec<double> sigtan;
-  bool tan_saved = false;
   int ncomp() const { return brick ? 6 : 4; }
   int nsig() const { return brick ? 6 : 3; }
   int ntan() const { return brick ? 2 : 6; }
-  const double* tan_sig() const { return tan_saved ? sigtan.p : sig.p; }
   int npts() const override { return brick ? 8 : 4; }
   long long np() const { return (long long)nele() * npts(); }
 
@@ -61,7 +57,6 @@ struct J2Group : Group {
     initial_state(d);
   }
   void update(Domain& d) override {
-    tan_saved = false;
     if (brick)
       launch_brick_update(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, d.U.p, d.ndf, d.dt, np(),
                           epn.p, xin.p, ept.p, xit.p, sig.p, D.p, d.nfail.p);
@@ -74,20 +69,15 @@ struct J2Group : Group {
     else launch_quad_resid(d.stream, nele(), dconn.p, d.crd.p, dthk.p, has_b ? db.p : nullptr, np(), sig.p, Re);
   }
   void tangent(Domain& d, double* Ke) override {
-    if (brick) launch_brick_tangent(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, np(), tan_sig(), D.p, Ke);
+    if (brick) launch_brick_tangent(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, np(), sig.p, D.p, Ke);
     else launch_quad_tangent(d.stream, nele(), dconn.p, d.crd.p, dthk.p, np(), D.p, Ke);
   }
   void commit(Domain& d) override { launch_j2_commit(d.stream, np(), ncomp(), ept.p, xit.p, epn.p, xin.p); }
   void revert(Domain&) override {}  // J2Plasticity::revertToLastCommit is a no-op (J2Plasticity.cpp:524)
   void revert_to_start(Domain& d) override {
     // J2Plasticity::revertToStart -> zero (J2Plasticity.cpp:213-224, :532-543):
-    // plastic state, stress and strain; the tangent of the last
-    // plastic_integrator call stays (a brick keeps the stress it is formed
-    // with until the next update).
-    if (brick && !tan_saved) {
-      sigtan.copy_from(sig, d.stream);
-      tan_saved = true;
-    }
+    // plastic state, stress and strain; the tangent is formed again by the
+    // update at zero displacement that ends Domain::revertToStart.
     epn.zero(d.stream);
     ept.zero(d.stream);
     xin.zero(d.stream);
@@ -99,7 +89,6 @@ struct J2Group : Group {
     // J2Plasticity::zero + the constructor's plastic_integrator at zero strain
     // (J2Plasticity.cpp:78-80): zero state and stress, elastic tangent
     // (filled by an update at U = 0).
-    tan_saved = false;
     epn.zero(d.stream);
     ept.zero(d.stream);
     xin.zero(d.stream);
@@ -138,8 +127,7 @@ struct J2Group : Group {
       std::vector<J2Param> tab;
       if (brick) {
         st.resize((size_t)n * 6);
-        XG_HIP(hipMemcpyAsync(st.data(), tan_sig(), st.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              d.stream));
+        XG_HIP(hipMemcpyAsync(st.data(), sig.p, st.size() * sizeof(double), hipMemcpyDeviceToHost, d.stream));
         XG_HIP(hipStreamSynchronize(d.stream));
         for (auto& kv : d.j2defs) tab.push_back(kv.second);
       }
@@ -626,14 +614,22 @@ void Domain::commit() {
   XG_HIP(hipStreamSynchronize(stream));
 }
 
+// Domain::revertToLastCommit (Domain.cpp:1925-1939): nodes and elements
+// revert, then one more update at the committed displacements with dT = 0.
 void Domain::revert() {
   U.copy_from(Uc, stream);
   V.copy_from(Vc, stream);
   A.copy_from(Ac, stream);
   for (auto& g : groups) g->revert(*this);
+  const double dt_keep = dt;
+  dt = 0.0;
+  if (ready) update();
+  dt = dt_keep;
   XG_HIP(hipStreamSynchronize(stream));
 }
 
+// Domain::revertToStart (Domain.cpp:1941-1958): nodes and elements to their
+// initial state, then one update at zero displacement with dT = 0.
 void Domain::revert_to_start() {
   U.zero(stream);
   Uc.zero(stream);
@@ -644,6 +640,10 @@ void Domain::revert_to_start() {
   for (auto& g : groups) g->revert_to_start(*this);
   lambda = 0;
   time = 0;
+  const double dt_keep = dt;
+  dt = 0.0;
+  if (ready) update();
+  dt = dt_keep;
   XG_HIP(hipStreamSynchronize(stream));
 }
 
