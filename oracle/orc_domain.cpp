@@ -10,6 +10,7 @@
 // with the prefix `orc_` instead of `xg_`.
 #include "orc_ddm.hpp"
 #include "orc_elements.hpp"
+#include "orc_force.hpp"
 #include "orc_frame.hpp"
 
 #include <algorithm>
@@ -253,6 +254,29 @@ struct OBeam : OElement {
   void revert_to_start() override { fe.revert_to_start(); }
   int npoints() const override { return fe.nsec; }
   int point_data(int what, double* out) const override { return fe.point_data(what, out); }
+};
+
+struct OForce : OElement {
+  ForceElement fb;
+  int ndof() const override { return fb.fe.ndof(); }
+  int update(Dom& d) override {
+    const NodeRec& a = d.nodes[nodes[0]];
+    const NodeRec& c = d.nodes[nodes[1]];
+    const int nd = fb.fe.is3d ? 6 : 3;
+    double ug[12];
+    for (int j = 0; j < nd; j++) {
+      ug[j] = a.U[j];
+      ug[nd + j] = c.U[j];
+    }
+    return fb.update(ug);
+  }
+  void tangent(Dom&, double* K) override { fb.tangent(K); }
+  void resist(Dom&, double* R) override { fb.resisting_force(R); }
+  void commit() override { fb.commit(); }
+  void revert() override { fb.revert(); }
+  void revert_to_start() override { fb.revert_to_start(); }
+  int npoints() const override { return fb.fe.nsec; }
+  int point_data(int what, double* out) const override { return fb.fe.point_data(what, out); }
 };
 
 // ---------------------------------------------------------------------------
@@ -1191,6 +1215,29 @@ int orc_elements_dispbeam(xg_domain* d, int n, const int* tags, const int* conn,
     int r = e->fe.init(is3d, ni->second.crd, nj->second.crd, d->transfs[transf_tag].data(),
                        d->sections[sec_tag], nip, rho);
     if (r < 0) return fail(XG_ERR_ARG, "beam init failed");
+    d->elemap[tags[i]] = std::move(e);
+  }
+  d->ready = false;
+  return 0;
+}
+int orc_elements_forcebeam(xg_domain* d, int n, const int* tags, const int* conn, int transf_tag, int sec_tag,
+                           int nip, int max_iter, double tol) {
+  if (!d->sections.count(sec_tag)) return fail(XG_ERR_ARG, "forceBeamColumn: unknown section");
+  if (!d->transfs.count(transf_tag)) return fail(XG_ERR_ARG, "forceBeamColumn: unknown transf");
+  bool is3d = d->ndm == 3;
+  if ((is3d && d->ndf != 6) || (!is3d && d->ndf != 3)) return fail(XG_ERR_ARG, "forceBeamColumn ndf");
+  if (nip < 2 || nip > 10) return fail(XG_ERR_ARG, "forceBeamColumn: 2 to 10 Lobatto points");
+  for (int i = 0; i < n; i++) {
+    auto e = std::make_unique<OForce>();
+    e->tag = tags[i];
+    e->nodes.assign(conn + 2 * i, conn + 2 * i + 2);
+    auto ni = d->nodemap.find(conn[2 * i]);
+    auto nj = d->nodemap.find(conn[2 * i + 1]);
+    if (ni == d->nodemap.end() || nj == d->nodemap.end()) return fail(XG_ERR_ARG, "beam node missing");
+    int r = e->fb.init(is3d, ni->second.crd, nj->second.crd, d->transfs[transf_tag].data(), d->sections[sec_tag],
+                       nip, max_iter, tol);
+    if (r == -4) return fail(XG_ERR_ARG, "forceBeamColumn (3D): the section needs a torsional stiffness");
+    if (r < 0) return fail(XG_ERR_ARG, "forceBeamColumn init failed");
     d->elemap[tags[i]] = std::move(e);
   }
   d->ready = false;

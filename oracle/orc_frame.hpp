@@ -430,6 +430,10 @@ struct FrameElement {
   void tangent(double* K) const {
     double kb[6][6];
     basic_stiff(kb);
+    global_stiff(kb, K);
+  }
+  // the transformation part of tangent(): K = T^T kb T
+  void global_stiff(const double kb[6][6], double* K) const {
     double oneOverL = 1.0 / L;
     if (!is3d) {
       double tmp[3][6];
@@ -497,6 +501,10 @@ struct FrameElement {
   void resisting_force(double* P) const {
     double qb[6];
     basic_force(qb);
+    global_force(qb, P);
+  }
+  // the transformation part of resisting_force(): P = T^T qb
+  void global_force(const double qb[6], double* P) const {
     double oneOverL = 1.0 / L;
     if (!is3d) {
       double q0 = qb[0], q1 = qb[1], q2 = qb[2];

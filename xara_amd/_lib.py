@@ -51,6 +51,7 @@ SIGNATURES = {
     "elements_quad": (C.c_int, [VP, C.c_int, P_I, P_I, C.c_double, C.c_int, C.c_double, C.c_double,
                                 C.c_double, C.c_double]),
     "elements_dispbeam": (C.c_int, [VP, C.c_int, P_I, P_I, C.c_int, C.c_int, C.c_int, C.c_double]),
+    "elements_forcebeam": (C.c_int, [VP, C.c_int, P_I, P_I, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
     "nodal_loads": (C.c_int, [VP, C.c_int, P_I, P_D]),
     "uniform_excitation": (C.c_int, [VP, C.c_int, C.c_int, C.c_double, P_D, C.c_double]),
     "rayleigh": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_double]),

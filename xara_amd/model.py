@@ -207,6 +207,11 @@ class Model:
             else:  # legacy: tag i j nip sec transf
                 nip, sec, transf = args[2], args[3], args[4]
             self.elements("dispBeamColumn", [tag], [(i, j)], transf, sec, nip)
+        elif kind == "forceBeamColumn":  # tag i j transf "Lobatto" sec nip <max_iter tol>
+            i, j, transf, integ, sec, nip = args[:6]
+            if integ != "Lobatto":
+                raise XaraError(-1, f"forceBeamColumn: {integ} integration is not on the GPU path")
+            self.elements("forceBeamColumn", [tag], [(i, j)], transf, sec, nip, *args[6:8])
         else:
             raise XaraError(-1, f"element {kind} is not on the GPU path")
 
@@ -228,6 +233,12 @@ class Model:
             transf, sec, nip = int(args[0]), int(args[1]), int(args[2])
             rho = float(args[3]) if len(args) > 3 else 0.0
             self._check(self._b.elements_dispbeam(self._h, n, _i(tags), _i(conn), transf, sec, nip, rho))
+        elif kind == "forceBeamColumn":
+            conn = _i32(conn).reshape(n, 2)
+            transf, sec, nip = int(args[0]), int(args[1]), int(args[2])
+            max_iter = int(args[3]) if len(args) > 3 else 10
+            tol = float(args[4]) if len(args) > 4 else 1e-12
+            self._check(self._b.elements_forcebeam(self._h, n, _i(tags), _i(conn), transf, sec, nip, max_iter, tol))
         else:
             raise XaraError(-1, f"element {kind} is not on the GPU path")
 
