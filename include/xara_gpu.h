@@ -129,6 +129,12 @@ int  xg_elements_quad(xg_domain* d, int n, const int* tags, const int* conn /* n
    (frames.cpp:449,906 -> DispBeamColumn2d.cpp:48 / DispBeamColumn3d.cpp) */
 int  xg_elements_dispbeam(xg_domain* d, int n, const int* tags, const int* conn /* n*2 */,
                           int transf_tag, int sec_tag, int nip, double rho);
+/* `element forceBeamColumn tag i j transf "Lobatto" sec nip <-iter maxIters tol>`
+   (frames.cpp:200-205,544-545,884-889 -> ForceBeamColumn2d.cpp:113 /
+   ForceBeamColumn3d.cpp; LobattoBeamIntegration, 2 <= nip <= 10, no mass,
+   no element loads; in 3D the section needs -GJ)                             */
+int  xg_elements_forcebeam(xg_domain* d, int n, const int* tags, const int* conn /* n*2 */,
+                           int transf_tag, int sec_tag, int nip, int max_iter, double tol);
 /* `pattern Plain 1 Linear { load tag P1..Pndf }` (NodalLoad; LoadPattern
    with a Linear series, applied as lambda*P by Domain::applyLoad,
    Domain.cpp:1765)                                                           */

@@ -1,0 +1,133 @@
+"""GPU parity of ForceBeamColumn2d/3d (element compatibility iteration over
+Steel02 / Concrete02 fiber sections, Lobatto sections) against the CPU
+oracle through the C-ABI: element states after nonlinear updates, pushover
+and transient trajectories, commit/revert, to 1e-9 relative."""
+import numpy as np
+import pytest
+
+from tests import _oracle
+from xara_amd import ground_motion, meshes
+from xara_amd.model import Model
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-9
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    s = np.max(np.abs(b))
+    return 0.0 if s == 0 else float(np.max(np.abs(a - b)) / s)
+
+
+def frame_pair(three_d, nip=5):
+    fr = meshes.rc_frame_3d(2, 1, 2) if three_d else meshes.rc_frame_2d(3, 2)
+    conn, _ = meshes.frame_elements(fr)
+    theta = meshes.default_theta(len(conn))
+    theta[:, 0] *= np.linspace(0.9, 1.1, len(conn))
+    ms = []
+    for m in (Model(3, 6) if three_d else Model(2, 3), _oracle.oracle_model(3 if three_d else 2, 6 if three_d else 3)):
+        meshes.build_rc_frame_per_element(m, fr, theta, element="forceBeamColumn", nip=nip)
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        ms.append(m)
+    return fr, ms
+
+
+@pytest.mark.parametrize("three_d,nip", [(False, 5), (True, 5), (False, 3), (True, 7)])
+def test_force_element_update_parity(three_d, nip):
+    """Sections, resisting forces and tangents after two nonlinear trials
+    (the second from the committed first)."""
+    print(f"case: force update 3d={three_d} nip={nip}", flush=True)
+    fr, (g, o) = frame_pair(three_d, nip)
+    for m in (g, o):
+        m.setup()
+    ndf = 6 if three_d else 3
+    h = fr["crds"][:, -1]
+    rng = np.random.default_rng(3)
+    for step, amp in enumerate((25.0, 60.0)):
+        U = np.zeros((len(h), ndf))
+        U[:, 0] = amp * h / h.max()
+        U[:, 1] = (0.3 if three_d else -2e-4) * amp * (h / h.max()) ** 2
+        U[:, :] += rng.normal(0, 0.02 * amp, U.shape) * (h[:, None] > 0)
+        U[:, (3 if three_d else 2):] *= 1e-4
+        for m in (g, o):
+            m.set_trial_disp(U)
+            m.update()
+        npts = len(meshes.frame_elements(fr)[0]) * nip
+        for what in ("stress", "strain", "tangent"):
+            assert relerr(g.point_data(what, npts), o.point_data(what, npts)) <= RTOL, (what, step)
+        for tag in (1, 4, 9):
+            assert relerr(g.element_resisting_force(tag), o.element_resisting_force(tag)) <= RTOL, (tag, step)
+            assert relerr(g.element_tangent(tag), o.element_tangent(tag)) <= RTOL, (tag, step)
+        assert relerr(g.form_unbalance(), o.form_unbalance()) <= RTOL
+        for m in (g, o):
+            m.commit()
+
+
+@pytest.mark.parametrize("three_d", [False, True])
+def test_force_frame_pushover_trajectory(three_d):
+    print(f"case: force pushover 3d={three_d}", flush=True)
+    fr, ms = frame_pair(three_d)
+    ndf = 6 if three_d else 3
+    traj = []
+    for m in ms:
+        nodes = fr["load_tags"] if three_d else fr["floor_nodes"]
+        P = np.zeros((len(nodes), ndf))
+        P[:, 0] = 1.0
+        m.loads(nodes, P)
+        m.test("NormUnbalance", 1e-5, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 6.0)
+        m.setup()
+        t = []
+        for step in range(12):
+            assert m.analyze(1) == 0
+            t.append((m.trial_disp().copy(), m.load_factor))
+        traj.append(t)
+    for step, ((ug, lg), (uo, lo)) in enumerate(zip(*traj)):
+        assert relerr(ug, uo) <= RTOL, step
+        assert abs(lg - lo) <= RTOL * abs(lo), step
+
+
+def test_force_frame_transient_trajectory():
+    print("case: force transient 2d", flush=True)
+    fr = meshes.rc_frame_2d(3, 2)
+    conn, _ = meshes.frame_elements(fr)
+    theta = meshes.default_theta(len(conn))
+    ag = ground_motion.kanai_tajimi(11, 200, 0.01, 0.8)
+    traj = []
+    for m in (Model(2, 3), _oracle.oracle_model(2, 3)):
+        meshes.define_transient_frame(m, fr, theta, ag, 0.01, tol=1e-12, maxit=40, element="forceBeamColumn")
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        m.setup()
+        t = []
+        for step in range(120):
+            assert m.analyze(1, 0.01) == 0
+            t.append(m.trial_disp().copy())
+        traj.append(np.array(t))
+    assert relerr(traj[0], traj[1]) <= RTOL
+
+
+def test_force_frame_revert_and_restart():
+    """revertToLastCommit (element q, kv, section deformations and the
+    domain's closing update) and revertToStart, GPU and oracle alike."""
+    print("case: force revert", flush=True)
+    fr, (g, o) = frame_pair(False)
+    for m in (g, o):
+        m.setup()
+    h = fr["crds"][:, -1]
+    U = np.zeros((len(h), 3))
+    U[:, 0] = 30.0 * h / h.max()
+    for m in (g, o):
+        m.set_trial_disp(U)
+        m.update()
+        m.commit()
+        m.set_trial_disp(2.5 * U)
+        m.update()
+        m.revert()
+    npts = len(meshes.frame_elements(fr)[0]) * 5
+    for what in ("stress", "tangent"):
+        assert relerr(g.point_data(what, npts), o.point_data(what, npts)) <= RTOL, what
+    assert relerr(g.form_unbalance(), o.form_unbalance()) <= RTOL
+    for m in (g, o):
+        m.revert_to_start()
+    assert relerr(g.form_tangent(), o.form_tangent()) <= RTOL
+    assert relerr(g.point_data("tangent", npts), o.point_data("tangent", npts)) <= RTOL

@@ -287,6 +287,32 @@ int xg_elements_dispbeam(xg_domain* h, int n, const int* tags, const int* conn, 
   });
 }
 
+int xg_elements_forcebeam(xg_domain* h, int n, const int* tags, const int* conn, int transf_tag, int sec_tag,
+                          int nip, int max_iter, double tol) {
+  return guard([&] {
+    auto& d = D(h);
+    if (!d.secdefs.count(sec_tag)) throw xg::Error(XG_ERR_ARG, "forceBeamColumn: unknown section");
+    if (!d.transfdefs.count(transf_tag)) throw xg::Error(XG_ERR_ARG, "forceBeamColumn: unknown geomTransf");
+    if ((d.ndm == 3 && d.ndf != 6) || (d.ndm == 2 && d.ndf != 3))
+      throw xg::Error(XG_ERR_ARG, "forceBeamColumn requires ndf = 3 (2D) or 6 (3D)");
+    if (nip < 2 || nip > 10) throw xg::Error(XG_ERR_ARG, "forceBeamColumn: 2 to 10 Lobatto points");
+    if (max_iter < 1 || !(tol > 0.0)) throw xg::Error(XG_ERR_ARG, "forceBeamColumn: maxIters >= 1 and tol > 0");
+    for (int i = 0; i < n; i++) {
+      xg::EleDef e;
+      e.type = 4;
+      e.nodes.assign(conn + 2 * (size_t)i, conn + 2 * (size_t)i + 2);
+      e.sec = sec_tag;
+      e.transf = transf_tag;
+      e.nip = nip;
+      e.maxiter = max_iter;
+      e.tol = tol;
+      d.eledefs[tags[i]] = std::move(e);
+    }
+    invalidate(d);
+    return 0;
+  });
+}
+
 int xg_nodal_loads(xg_domain* h, int n, const int* tags, const double* P) {
   return guard([&] {
     auto& d = D(h);

@@ -310,7 +310,7 @@ void Domain::setup() {
         jg->nd = jg->nen * ndf;
         g = std::move(jg);
       } else {
-        g = make_frame_group(*this);
+        g = make_frame_group(*this, ed.type == 4);
       }
       gi = (int)groups.size();
       groups.push_back(std::move(g));
@@ -550,6 +550,15 @@ void Domain::update() {
   for (auto& g : groups) counters.material_points += g->fibers();
 }
 
+// AnalysisModel::updateDomain returns Domain::update, the sum of the
+// elements' update codes: here, whether an element group flagged a failure.
+int Domain::update_checked() {
+  update();
+  bool bad = false;
+  for (auto& g : groups) bad = g->update_failed(*this) || bad;
+  return bad ? -1 : 0;
+}
+
 void Domain::form_unbalance() {
   for (auto& g : groups) g->resid(*this, Re.p + g->re_off);
   if (integ == 3)
@@ -695,7 +704,7 @@ int Domain::new_step() {
     launch_newmark_predict(stream, (long long)nnode * ndf, eqmap.p, Uc.p, Vc.p, Ac.p, a1, a2, a3, a4, U.p, V.p, A.p);
     time += h;
     ag = ground_accel(time);  // EarthquakePattern::applyLoad(time)
-    update();
+    if (update_checked() < 0) return XG_BAD_STEP;
     return 0;
   }
   if (integ == 1) {  // LoadControl::newStep, LoadControl.cpp:70-95
@@ -719,7 +728,7 @@ int Domain::new_step() {
   XG_HIP(hipMemsetAsync(dU.p, 0, n * sizeof(double), stream));
   launch_axpby(stream, n, dl, dUhat.p, 0.0, dU.p);
   incr_disp(dU.p);
-  update();
+  if (update_checked() < 0) return XG_BAD_STEP;
   dc_last = 0;
   return 0;
 }
@@ -728,12 +737,12 @@ int Domain::integ_update(const double* x) {
   const int n = num.neq;
   if (integ == 3) {  // Newmark::update, Newmark.cpp:411-458
     launch_newmark_correct(stream, (long long)nnode * ndf, eqmap.p, x, nm_c2, nm_c3, U.p, V.p, A.p);
-    update();
+    if (update_checked() < 0) return XG_BAD_STEP;
     return 0;
   }
   if (integ == 1) {  // LoadControl::update, LoadControl.cpp:102-120
     incr_disp(x);
-    update();
+    if (update_checked() < 0) return XG_BAD_STEP;
     lc_last++;
     return 0;
   }
@@ -748,7 +757,7 @@ int Domain::integ_update(const double* x) {
   launch_axpby(stream, n, dL, dUhat.p, 1.0, dU.p);
   lambda += dL;
   incr_disp(dU.p);
-  update();
+  if (update_checked() < 0) return XG_BAD_STEP;
   XG_HIP(hipMemcpyAsync(X.p, dU.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
   dc_last++;
   return 0;

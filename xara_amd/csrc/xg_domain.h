@@ -41,6 +41,8 @@ struct Group {
   virtual long long state_size(int) const { return 0; }
   virtual void get_state(Domain&, int, double*) {}
   virtual void set_state(Domain&, int, const double*) {}
+  // an element update that failed since the last call (Element::update < 0)
+  virtual bool update_failed(Domain&) { return false; }
 };
 
 struct NodeDef {
@@ -51,10 +53,10 @@ struct NodeDef {
 };
 
 struct EleDef {
-  int type = 0;  // 1 brick, 2 quad, 3 dispbeam
+  int type = 0;  // 1 brick, 2 quad, 3 dispbeam, 4 forcebeam
   std::vector<int> nodes;  // node tags
-  int mat = 0, sec = 0, transf = 0, nip = 0;
-  double thk = 1.0, rho = 0.0, b[3] = {0, 0, 0};
+  int mat = 0, sec = 0, transf = 0, nip = 0, maxiter = 10;
+  double thk = 1.0, rho = 0.0, b[3] = {0, 0, 0}, tol = 1e-12;
 };
 
 struct Steel02Def {
@@ -164,6 +166,7 @@ struct Domain {
   int newton();
   int new_step();
   int integ_update(const double* x);
+  int update_checked();  // update(); < 0 if an element update failed (AnalysisModel::updateDomain)
   int run_test(int& iter, std::vector<double>& norms, int& numIncr);
   void precond_setup();
   int lanes();

@@ -11,11 +11,81 @@
 
 namespace xg {
 
+// Gauss-Lobatto locations on [-1, 1] and weights for 2..10 points as
+// LobattoBeamIntegration::getSectionLocations / getSectionWeights tabulate
+// them (LobattoBeamIntegration.cpp:113-293); the elements use 0.5 (x + 1)
+// and 0.5 w.
+static bool lobatto_points(int n, double* xi, double* wt) {
+  static const double X[11][10] = {
+      {}, {},
+      {-1.0, 1.0},
+      {-1.0, 0.0, 1.0},
+      {-1.0, -0.44721360, 0.44721360, 1.0},
+      {-1.0, -0.65465367, 0.0, 0.65465367, 1.0},
+      {-1.0, -0.7650553239, -0.2852315164, 0.2852315164, 0.7650553239, 1.0},
+      {-1.0, -0.8302238962, -0.4688487934, 0.0, 0.4688487934, 0.8302238962, 1.0},
+      {-1.0, -0.8717401485, -0.5917001814, -0.2092992179, 0.2092992179, 0.5917001814, 0.8717401485, 1.0},
+      {-1.0, -0.8997579954, -0.6771862795, -0.3631174638, 0.0, 0.3631174638, 0.6771862795, 0.8997579954, 1.0},
+      {-1.0, -0.9195339082, -0.7387738651, -0.4779249498, -0.1652789577, 0.1652789577, 0.4779249498,
+       0.7387738651, 0.9195339082, 1.0}};
+  static const double W[11][10] = {
+      {}, {},
+      {1.0, 1.0},
+      {0.333333333333333, 1.333333333333333, 0.333333333333333},
+      {0.166666666666667, 0.833333333333333, 0.833333333333333, 0.166666666666667},
+      {0.1, 0.5444444444, 0.7111111111, 0.5444444444, 0.1},
+      {0.06666666667, 0.3784749562, 0.5548583770, 0.5548583770, 0.3784749562, 0.06666666667},
+      {0.04761904762, 0.2768260473, 0.4317453812, 0.4876190476, 0.4317453812, 0.2768260473, 0.04761904762},
+      {0.03571428571, 0.2107042271, 0.3411226924, 0.4124587946, 0.4124587946, 0.3411226924, 0.2107042271,
+       0.03571428571},
+      {0.02777777778, 0.1654953615, 0.2745387125, 0.3464285109, 0.3715192743, 0.3464285109, 0.2745387125,
+       0.1654953615, 0.02777777778},
+      {0.02222222222, 0.1333059908, 0.2248893421, 0.2920426836, 0.3275397611, 0.3275397611, 0.2920426836,
+       0.2248893421, 0.1333059908, 0.02222222222}};
+  if (n < 2 || n > 10) return false;
+  for (int i = 0; i < n; i++) {
+    xi[i] = 0.5 * (X[n][i] + 1.0);
+    wt[i] = W[n][i] * 0.5;
+  }
+  return true;
+}
+
+// host Gauss-Jordan inverse with partial pivoting (the device's inverse_n)
+static bool host_inverse(int n, const double* A, double* Ai) {
+  double a[6][12];
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < 2 * n; j++) a[i][j] = j < n ? A[i * n + j] : (j - n == i ? 1.0 : 0.0);
+  for (int c = 0; c < n; c++) {
+    int piv = c;
+    for (int i = c + 1; i < n; i++)
+      if (std::fabs(a[i][c]) > std::fabs(a[piv][c])) piv = i;
+    if (a[piv][c] == 0.0) return false;
+    if (piv != c)
+      for (int j = 0; j < 2 * n; j++) std::swap(a[c][j], a[piv][j]);
+    const double inv = 1.0 / a[c][c];
+    for (int j = 0; j < 2 * n; j++) a[c][j] *= inv;
+    for (int i = 0; i < n; i++)
+      if (i != c) {
+        const double fct = a[i][c];
+        for (int j = 0; j < 2 * n; j++) a[i][j] -= fct * a[c][j];
+      }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) Ai[i * n + j] = a[i][n + j];
+  return true;
+}
+
 struct FrameGroup : Group {
   bool d3 = false;
+  bool force = false;  // ForceBeamColumn2d/3d
+  int nip = 5, maxiter = 10;
+  double tol = 1e-12;
   std::vector<int> sec_h, transf_h;
   std::vector<double> rho_h;
   FrameDev f;
+  DVec<double> dlxi, dlwt, dfs0, dSe, dSe_c, dkv, dkv_c, dvprev, dvprev_c, dvs, dvs_c, dfsec, dSsr;
+  DVec<int> dinit, dferr;
+  int NB() const { return d3 ? 6 : 3; }
   // device storage
   DVec<int> dconn, dsecdef, ddef_off, dfkind, dfmat, dflocal, drun_off, drun_beg, drun_kind, drun_mat, ddef_last;
   DVec<double> dgeo, dfy, dfz, dfA, dGJ, de, ds, dks, delasE;
@@ -27,7 +97,7 @@ struct FrameGroup : Group {
   std::vector<double> geo_h;
   int ord() const { return d3 ? 4 : 2; }
   int ld() const { return d3 ? 4 : 2; }
-  int npts() const override { return 5; }
+  int npts() const override { return nip; }
   long long nfib = 0;
   long long fibers() const override { return nfib; }
 
@@ -155,13 +225,13 @@ struct FrameGroup : Group {
       secdef[e] = it->second;
     }
     // ---- pools ----
-    const long long nsec = (long long)ne * 5;
+    const long long nsec = (long long)ne * nip;
     std::vector<long long> sb(nsec), cb(nsec), eb(nsec);
     long long ns = 0, nc = 0, nl = 0;
     nfib = 0;
     for (int e = 0; e < ne; e++)
-      for (int i = 0; i < 5; i++) {
-        const long long sec = (long long)e * 5 + i;
+      for (int i = 0; i < nip; i++) {
+        const long long sec = (long long)e * nip + i;
         const int df = secdef[e];
         sb[sec] = ns;
         cb[sec] = nc;
@@ -177,8 +247,8 @@ struct FrameGroup : Group {
     {
       // steel fibers in pool order
       for (int e = 0; e < ne; e++)
-        for (int i = 0; i < 5; i++) {
-          const long long sec = (long long)e * 5 + i;
+        for (int i = 0; i < nip; i++) {
+          const long long sec = (long long)e * nip + i;
           const int df = secdef[e];
           for (int fi = def_off[df]; fi < def_off[df + 1]; fi++) {
             if (fkind[fi] == 1) {
@@ -241,7 +311,7 @@ struct FrameGroup : Group {
     dks.zero(s);
     f = FrameDev{};
     f.nele = ne;
-    f.nip = 5;
+    f.nip = nip;
     f.nsec = nsec;
     f.conn = dconn.p;
     f.geo = dgeo.p;
@@ -279,11 +349,111 @@ struct FrameGroup : Group {
     f.cs_t = cs_t.p;
     f.el_c = el_c.p;
     f.el_t = el_t.p;
+    if (force) setup_force(d, def_off, fkind, fmat, fy, fz, fA, GJ, steelp, concp, elasE);
     stc_init = std::move(stc);
     csc_init = std::move(csc);
     konc_init = std::move(konc);
-    // setDomain calls update() (DispBeamColumn2d.cpp:184): zero displacement
+    // Domain::addElement: setDomain + update() at zero displacement
+    // (DispBeamColumn2d.cpp:184; ForceBeamColumn2d's first update forms the
+    // initial basic stiffness)
     zero_update(d);
+  }
+
+  // Element and section state of the force-based elements, the Lobatto rule
+  // and each definition's initial section flexibility (the inverse of the
+  // fibers' initial tangents, SectionForceDeformation::getInitialFlexibility).
+  void setup_force(Domain& d, const std::vector<int>& def_off, const std::vector<int>& fkind,
+                   const std::vector<int>& fmat, const std::vector<double>& fy, const std::vector<double>& fz,
+                   const std::vector<double>& fA, const std::vector<double>& GJ,
+                   const std::vector<Steel02Param>& steelp, const std::vector<Concrete02Param>& concp,
+                   const std::vector<double>& elasE) {
+    hipStream_t s = d.stream;
+    const int ne = nele(), nb = NB(), ns = ord();
+    const long long nsec = (long long)ne * nip;
+    std::vector<double> xi(nip), wt(nip);
+    if (!lobatto_points(nip, xi.data(), wt.data()))
+      throw Error(XG_ERR_ARG, "forceBeamColumn: 2 to 10 Lobatto points");
+    const int ndef = (int)GJ.size();
+    std::vector<double> fs0((size_t)ndef * ns * ns, 0.0);
+    for (int df = 0; df < ndef; df++) {
+      double k[16] = {0};
+      for (int fi = def_off[df]; fi < def_off[df + 1]; fi++) {
+        const double E = fkind[fi] == 1 ? steelp[fmat[fi]].E0
+                                         : (fkind[fi] == 2 ? 2.0 * concp[fmat[fi]].fc / concp[fmat[fi]].epsc0
+                                                           : elasE[fmat[fi]]);
+        const double y = fy[fi], A = fA[fi];
+        if (!d3) {
+          const double ks0 = E * A, ks1 = ks0 * -y;
+          k[0] += ks0;
+          k[1] += ks1;
+          k[3] += ks1 * -y;
+        } else {
+          const double z = fz[fi], EA = E * A;
+          k[0] += EA;
+          k[1] += -y * EA;
+          k[2] += z * EA;
+          k[5] += y * y * EA;
+          k[6] += -y * z * EA;
+          k[10] += z * z * EA;
+        }
+      }
+      if (!d3) {
+        k[2] = k[1];
+      } else {
+        if (GJ[df] < 0.0) throw Error(XG_ERR_ARG, "forceBeamColumn (3D): the section needs a torsional stiffness (-GJ)");
+        k[4] = k[1];
+        k[8] = k[2];
+        k[9] = k[6];
+        k[15] = GJ[df];
+      }
+      double kk[16];
+      for (int a = 0; a < ns; a++)
+        for (int b = 0; b < ns; b++) kk[a * ns + b] = k[a * (d3 ? 4 : 2) + b];
+      if (!host_inverse(ns, kk, &fs0[(size_t)df * ns * ns]))
+        throw Error(XG_ERR_ARG, "forceBeamColumn: singular initial section stiffness");
+    }
+    dlxi.upload(xi, s);
+    dlwt.upload(wt, s);
+    dfs0.upload(fs0, s);
+    auto zeros = [&](DVec<double>& v, size_t n) {
+      v.resize(n);
+      v.zero(s);
+    };
+    zeros(dSe, (size_t)ne * nb);
+    zeros(dSe_c, (size_t)ne * nb);
+    zeros(dkv, (size_t)ne * nb * nb);
+    zeros(dkv_c, (size_t)ne * nb * nb);
+    zeros(dvprev, (size_t)ne * nb);
+    zeros(dvprev_c, (size_t)ne * nb);
+    zeros(dvs, (size_t)nsec * ns);
+    zeros(dvs_c, (size_t)nsec * ns);
+    zeros(dfsec, (size_t)nsec * ns * ns);
+    zeros(dSsr, (size_t)nsec * ns);
+    dinit.resize(ne);
+    dinit.zero(s);
+    dferr.resize(1);
+    dferr.zero(s);
+    f.force = 1;
+    f.maxiter = maxiter;
+    f.tol = tol;
+    f.lxi = dlxi.p;
+    f.lwt = dlwt.p;
+    f.fs0 = dfs0.p;
+    f.Se = dSe.p;
+    f.Se_c = dSe_c.p;
+    f.kv = dkv.p;
+    f.kv_c = dkv_c.p;
+    f.vprev = dvprev.p;
+    f.vprev_c = dvprev_c.p;
+    f.init = dinit.p;
+    f.vs = dvs.p;
+    f.vs_c = dvs_c.p;
+    f.fsec = dfsec.p;
+    f.Ssr = dSsr.p;
+  }
+  void force_zero_state(hipStream_t s) {
+    for (DVec<double>* v : {&dSe, &dSe_c, &dkv, &dkv_c, &dvprev, &dvprev_c, &dvs, &dvs_c, &dfsec, &dSsr}) v->zero(s);
+    dinit.zero(s);
   }
   std::vector<double> stc_init, csc_init;
   std::vector<int> konc_init;
@@ -292,19 +462,45 @@ struct FrameGroup : Group {
     DVec<double> zeroU;
     zeroU.resize((size_t)d.nnode * d.ndf);
     zeroU.zero(d.stream);
-    launch_frame_update(d.stream, f, d3, zeroU.p, d.nfail.p);
+    if (force) launch_force_update(d.stream, f, d3, zeroU.p, dferr.p);
+    else launch_frame_update(d.stream, f, d3, zeroU.p, d.nfail.p);
     XG_HIP(hipStreamSynchronize(d.stream));
   }
-  void update(Domain& d) override { launch_frame_update(d.stream, f, d3, d.U.p, d.nfail.p); }
-  void resid(Domain& d, double* Re) override { launch_frame_resid(d.stream, f, d3, Re); }
-  void tangent(Domain& d, double* Ke) override { launch_frame_tangent(d.stream, f, d3, Ke); }
+  void update(Domain& d) override {
+    if (force) launch_force_update(d.stream, f, d3, d.U.p, dferr.p);
+    else launch_frame_update(d.stream, f, d3, d.U.p, d.nfail.p);
+  }
+  bool update_failed(Domain& d) override {
+    if (!force) return false;
+    int h = 0;
+    dferr.download(&h, 1, d.stream);
+    if (h) dferr.zero(d.stream);
+    return h != 0;
+  }
+  void resid(Domain& d, double* Re) override {
+    if (force) launch_force_resid(d.stream, f, d3, Re);
+    else launch_frame_resid(d.stream, f, d3, Re);
+  }
+  void tangent(Domain& d, double* Ke) override {
+    if (force) launch_force_tangent(d.stream, f, d3, Ke);
+    else launch_frame_tangent(d.stream, f, d3, Ke);
+  }
   void commit(Domain& d) override {
     launch_copy_d(d.stream, (long long)st_t.n, st_t.p, st_c.p);
     launch_copy_i(d.stream, (long long)kon_t.n, kon_t.p, kon_c.p);
     launch_copy_d(d.stream, (long long)cs_t.n, cs_t.p, cs_c.p);
     launch_copy_d(d.stream, (long long)el_t.n, el_t.p, el_c.p);
+    if (force) {  // ForceBeamColumn2d::commitState (:276-308)
+      launch_copy_d(d.stream, (long long)dvs.n, dvs.p, dvs_c.p);
+      launch_copy_d(d.stream, (long long)dkv.n, dkv.p, dkv_c.p);
+      launch_copy_d(d.stream, (long long)dSe.n, dSe.p, dSe_c.p);
+      launch_copy_d(d.stream, (long long)dvprev.n, dvprev.p, dvprev_c.p);
+    }
   }
-  void revert(Domain& d) override { launch_frame_revert(d.stream, f, d3); }
+  void revert(Domain& d) override {
+    if (force) launch_force_revert(d.stream, f, d3);
+    else launch_frame_revert(d.stream, f, d3);
+  }
   void revert_to_start(Domain& d) override {
     hipStream_t s = d.stream;
     st_c.upload(stc_init, s);
@@ -315,7 +511,8 @@ struct FrameGroup : Group {
     cs_t.upload(csc_init, s);
     el_c.zero(s);
     el_t.zero(s);
-    zero_update(d);
+    if (force) force_zero_state(s);  // ForceBeamColumn2d::revertToStart (:344-376)
+    else zero_update(d);
   }
   void mass_diag(Domain& d, std::vector<double>& out) override {
     // lumped translational mass 0.5 rho L (DispBeamColumn2d::getMass, cMass = 0)
@@ -333,6 +530,8 @@ struct FrameGroup : Group {
   // sigP, eP.  Fibers by element (ascending tag), section, then definition
   // order: the pool order.
   long long state_size(int kind) const override {
+    if (force && (kind == XG_STATE_STEEL02 || kind == XG_STATE_CONCRETE02))
+      throw Error(XG_ERR_ARG, "committed-state export: forceBeamColumn element state is not exported");
     if (kind == XG_STATE_STEEL02) return f.nsteel * 11;
     if (kind == XG_STATE_CONCRETE02) return f.nconc * 5;
     return 0;
@@ -409,9 +608,10 @@ FrameDev frame_dev(Group& g) {
   return fg->f;
 }
 
-std::unique_ptr<Group> make_frame_group(Domain& d) {
+std::unique_ptr<Group> make_frame_group(Domain& d, bool force) {
   auto g = std::make_unique<FrameGroup>();
   g->d3 = d.ndm == 3;
+  g->force = force;
   g->nen = 2;
   g->nd = 2 * d.ndf;
   return g;
@@ -421,6 +621,14 @@ void frame_group_add(Domain& d, Group& g, int tag, const EleDef& ed) {
   (void)d;
   (void)tag;
   FrameGroup& fg = static_cast<FrameGroup&>(g);
+  if (fg.force) {
+    if (fg.tags.size() == 1) {  // the group's first element sets its rule and iteration limits
+      fg.nip = ed.nip;
+      fg.maxiter = ed.maxiter;
+      fg.tol = ed.tol;
+    } else if (ed.nip != fg.nip || ed.maxiter != fg.maxiter || ed.tol != fg.tol)
+      throw Error(XG_ERR_ARG, "forceBeamColumn: all elements must share nip, maxIters and tol");
+  }
   fg.sec_h.push_back(ed.sec);
   fg.transf_h.push_back(ed.transf);
   fg.rho_h.push_back(ed.rho);

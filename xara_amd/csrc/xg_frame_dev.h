@@ -1,4 +1,5 @@
-// Device view of a frame-element group (DispBeamColumn2d/3d + fiber sections).
+// Device view of a frame-element group (DispBeamColumn2d/3d or
+// ForceBeamColumn2d/3d over fiber sections).
 #pragma once
 #include "xg_common.h"
 #include "xg_uniaxial.h"
@@ -48,12 +49,35 @@ struct FrameDev {
   double* cs_t = nullptr;
   double* el_c = nullptr;  // elastic fiber strain
   double* el_t = nullptr;
+  // ---- force-based elements (ForceBeamColumn2d/3d), NB = 3 or 6 basic
+  // forces, NS = ord section forces; element arrays [nele][..], section
+  // arrays [nsec][..] ----
+  int force = 0, maxiter = 10;
+  double tol = 1e-12;
+  const double* lxi = nullptr;  // section locations on (0, 1), nip
+  const double* lwt = nullptr;  // section weights (sum 1), nip
+  const double* fs0 = nullptr;  // initial section flexibility per definition [ndef][NS*NS]
+  double* Se = nullptr;         // basic forces, trial / committed
+  double* Se_c = nullptr;
+  double* kv = nullptr;         // basic stiffness NB x NB, trial / committed
+  double* kv_c = nullptr;
+  double* vprev = nullptr;      // basic displacement at the last update / at commit
+  double* vprev_c = nullptr;
+  int* init = nullptr;          // initialFlag
+  double* vs = nullptr;         // section deformations, trial / committed
+  double* vs_c = nullptr;
+  double* fsec = nullptr;       // section flexibility NS x NS
+  double* Ssr = nullptr;        // section resisting forces
 };
 
 void launch_frame_update(hipStream_t s, const FrameDev& f, bool d3, const double* U, int* err);
 void launch_frame_revert(hipStream_t s, const FrameDev& f, bool d3);
 void launch_frame_resid(hipStream_t s, const FrameDev& f, bool d3, double* Re);
 void launch_frame_tangent(hipStream_t s, const FrameDev& f, bool d3, double* Ke);
+void launch_force_update(hipStream_t s, const FrameDev& f, bool d3, const double* U, int* err);
+void launch_force_resid(hipStream_t s, const FrameDev& f, bool d3, double* Re);
+void launch_force_tangent(hipStream_t s, const FrameDev& f, bool d3, double* Ke);
+void launch_force_revert(hipStream_t s, const FrameDev& f, bool d3);
 void launch_copy_d(hipStream_t s, long long n, const double* a, double* b);
 void launch_copy_i(hipStream_t s, long long n, const int* a, int* b);
 

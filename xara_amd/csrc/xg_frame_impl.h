@@ -214,11 +214,16 @@ struct SecSums {
 // Concrete02's previous trial stress and tangent (kept on its early return)
 // equal the committed ones and are read from there, so that the trial
 // arrays need not hold a copy of the committed state.
+//
+// section_fibers: the fiber updates and section sums at the wave-uniform
+// section deformation d (FiberSection2d/3d::setTrialSectionDeformation);
+// lane 0 stores e, s and ks, and every lane receives the stress resultant
+// sr[ord] and the tangent kt[ord x ord] (row-major; 3D with the torsion
+// entry GJ).
 template <bool D3, bool PrevCommitted = false>
-__device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
+__device__ __forceinline__ void section_fibers(const FrameDev& f, long long sec, int lane, const double (&d)[4],
+                                               double* sr, double* kt) {
   const int e = (int)(sec / f.nip);
-  double d[4];
-  section_deformation<D3>(f, sec, U, d);
   const int def = uniform_i(f.secdef[e]);
   const long long sb = f.steel_base[sec], cb = f.conc_base[sec], eb = f.elas_base[sec];
   SecSums S;
@@ -303,34 +308,42 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
     wave_sums<8, 5>(x, lane);
     k0 = x[0], k1 = x[1], k5 = x[2], s0 = x[3], s1 = x[4];
   }
+  if (D3) {
+    const double GJ = uniform_d(f.GJ[def]);
+    const double kd[16] = {k0, k1, k2, 0, k1, k5, k6, 0, k2, k6, k10, 0, 0, 0, 0, GJ >= 0.0 ? GJ : 0.0};
+#pragma unroll
+    for (int c = 0; c < 16; c++) kt[c] = kd[c];
+    sr[0] = s0;
+    sr[1] = s1;
+    sr[2] = s2;
+    sr[3] = GJ >= 0.0 ? GJ * d[3] : 0.0;
+  } else {
+    kt[0] = k0;
+    kt[1] = k1;
+    kt[2] = k1;
+    kt[3] = k5;
+    sr[0] = s0;
+    sr[1] = s1;
+  }
   if (lane == 0) {
     const long long n = f.nsec;
-    if (D3) {
-      double kd[16] = {k0, k1, k2, 0, k1, k5, k6, 0, k2, k6, k10, 0, 0, 0, 0, 0};
-      double sd[4] = {s0, s1, s2, 0.0};
-      const double GJ = f.GJ[def];
-      if (GJ >= 0.0) {
-        sd[3] = GJ * d[3];
-        kd[15] = GJ;
-      }
+    constexpr int ord = D3 ? 4 : 2;
 #pragma unroll
-      for (int c = 0; c < 16; c++) f.ks[c * n + sec] = kd[c];
+    for (int c = 0; c < ord * ord; c++) f.ks[c * n + sec] = kt[c];
 #pragma unroll
-      for (int c = 0; c < 4; c++) {
-        f.s[c * n + sec] = sd[c];
-        f.e[c * n + sec] = d[c];
-      }
-    } else {
-      f.ks[0 * n + sec] = k0;
-      f.ks[1 * n + sec] = k1;
-      f.ks[2 * n + sec] = k1;
-      f.ks[3 * n + sec] = k5;
-      f.s[0 * n + sec] = s0;
-      f.s[1 * n + sec] = s1;
-      f.e[0 * n + sec] = d[0];
-      f.e[1 * n + sec] = d[1];
+    for (int c = 0; c < ord; c++) {
+      f.s[c * n + sec] = sr[c];
+      f.e[c * n + sec] = d[c];
     }
   }
+}
+
+template <bool D3, bool PrevCommitted = false>
+__device__ __forceinline__ void section_update(const FrameDev& f, long long sec, int lane, const double* __restrict__ U) {
+  double d[4];
+  section_deformation<D3>(f, sec, U, d);
+  double sr[4], kt[16];
+  section_fibers<D3, PrevCommitted>(f, sec, lane, d, sr, kt);
 }
 
 // revertToLastCommit of every fiber followed by the section recomputation of
@@ -444,25 +457,10 @@ __device__ __forceinline__ void section_revert(const FrameDev& f, long long sec,
 // Element resisting force: q = sum_i B_i^T s_i wt_i, P = T^T q
 // (DispBeamColumn2d::getResistingForce :665-726, LinearCrdTransf2d::
 // getGlobalResistingForce :473-525; 3D :295-360 and :700).
+// P = T^T q: LinearCrdTransf2d::getGlobalResistingForce (:473-525),
+// LinearCrdTransf3d (:700)
 template <bool D3>
-__device__ __forceinline__ void element_resid(const FrameDev& f, int e, double* __restrict__ Re) {
-  const long long n = f.nsec;
-  double q[6] = {0, 0, 0, 0, 0, 0};
-  for (int i = 0; i < f.nip; i++) {
-    const long long sec = (long long)e * f.nip + i;
-    const double xi6 = 6.0 * (0.5 * (1.0 + gl5_pt(i)));
-    const double wt = gl5_wt(i) * 0.5;
-    q[0] += f.s[0 * n + sec] * wt;
-    double si = f.s[1 * n + sec] * wt;
-    q[1] += (xi6 - 4.0) * si;
-    q[2] += (xi6 - 2.0) * si;
-    if (D3) {
-      si = f.s[2 * n + sec] * wt;
-      q[3] += (xi6 - 4.0) * si;
-      q[4] += (xi6 - 2.0) * si;
-      q[5] += f.s[3 * n + sec] * wt;
-    }
-  }
+__device__ __forceinline__ void global_force(const FrameDev& f, int e, const double q[6], double* __restrict__ Re) {
   const double* g = f.geo + (long long)e * 10;
   const double oneOverL = 1.0 / g[0];
   if (!D3) {
@@ -500,85 +498,34 @@ __device__ __forceinline__ void element_resid(const FrameDev& f, int e, double* 
   }
 }
 
-// Element tangent: kb = sum_i B_i^T ks_i B_i wt_i / L (getBasicStiff, 2D
-// :287-355; 3D getTangentStiff :407-537), K = T^T kb T (LinearCrdTransf2d::
-// getGlobalStiffMatrix :610-733, LinearCrdTransf3d :768).  Row-major output.
 template <bool D3>
-__device__ __forceinline__ void element_tangent(const FrameDev& f, int e, double* __restrict__ Ke) {
-  constexpr int nb = D3 ? 6 : 3;
-  constexpr int ord = D3 ? 4 : 2;
-  constexpr int ld = D3 ? 4 : 2;
+__device__ __forceinline__ void element_resid(const FrameDev& f, int e, double* __restrict__ Re) {
   const long long n = f.nsec;
-  const double* g = f.geo + (long long)e * 10;
-  const double oneOverL = 1.0 / g[0];
-  double kb[6][6];
-#pragma unroll
-  for (int i = 0; i < 6; i++)
-#pragma unroll
-    for (int j = 0; j < 6; j++) kb[i][j] = 0.0;
+  double q[6] = {0, 0, 0, 0, 0, 0};
   for (int i = 0; i < f.nip; i++) {
     const long long sec = (long long)e * f.nip + i;
     const double xi6 = 6.0 * (0.5 * (1.0 + gl5_pt(i)));
-    const double wti = (gl5_wt(i) * 0.5) * oneOverL;
-    double ks[ord][ord];
-#pragma unroll
-    for (int a = 0; a < ord; a++)
-#pragma unroll
-      for (int b = 0; b < ord; b++) ks[a][b] = f.ks[(long long)(b * ld + a) * n + sec];
-    double ka[ord][6];
-#pragma unroll
-    for (int a = 0; a < ord; a++)
-#pragma unroll
-      for (int b = 0; b < 6; b++) ka[a][b] = 0.0;
-#pragma unroll
-    for (int j = 0; j < ord; j++) {
-      if (j == 0) {
-#pragma unroll
-        for (int k = 0; k < ord; k++) ka[k][0] += ks[k][j] * wti;
-      } else if (j == 1) {
-#pragma unroll
-        for (int k = 0; k < ord; k++) {
-          const double tmp = ks[k][j] * wti;
-          ka[k][1] += (xi6 - 4.0) * tmp;
-          ka[k][2] += (xi6 - 2.0) * tmp;
-        }
-      } else if (j == 2) {
-#pragma unroll
-        for (int k = 0; k < ord; k++) {
-          const double tmp = ks[k][j] * wti;
-          ka[k][3] += (xi6 - 4.0) * tmp;
-          ka[k][4] += (xi6 - 2.0) * tmp;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < ord; k++) ka[k][5] += ks[k][j] * wti;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < ord; j++) {
-      if (j == 0) {
-#pragma unroll
-        for (int k = 0; k < nb; k++) kb[0][k] += ka[j][k];
-      } else if (j == 1) {
-#pragma unroll
-        for (int k = 0; k < nb; k++) {
-          const double tmp = ka[j][k];
-          kb[1][k] += (xi6 - 4.0) * tmp;
-          kb[2][k] += (xi6 - 2.0) * tmp;
-        }
-      } else if (j == 2) {
-#pragma unroll
-        for (int k = 0; k < nb; k++) {
-          const double tmp = ka[j][k];
-          kb[3][k] += (xi6 - 4.0) * tmp;
-          kb[4][k] += (xi6 - 2.0) * tmp;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < nb; k++) kb[5][k] += ka[j][k];
-      }
+    const double wt = gl5_wt(i) * 0.5;
+    q[0] += f.s[0 * n + sec] * wt;
+    double si = f.s[1 * n + sec] * wt;
+    q[1] += (xi6 - 4.0) * si;
+    q[2] += (xi6 - 2.0) * si;
+    if (D3) {
+      si = f.s[2 * n + sec] * wt;
+      q[3] += (xi6 - 4.0) * si;
+      q[4] += (xi6 - 2.0) * si;
+      q[5] += f.s[3 * n + sec] * wt;
     }
   }
+  global_force<D3>(f, e, q, Re);
+}
+
+// K = T^T kb T: LinearCrdTransf2d::getGlobalStiffMatrix (:610-733),
+// LinearCrdTransf3d (:768); row-major output
+template <bool D3>
+__device__ __forceinline__ void global_stiff(const FrameDev& f, int e, const double (&kb)[6][6], double* __restrict__ Ke) {
+  const double* g = f.geo + (long long)e * 10;
+  const double oneOverL = 1.0 / g[0];
   if (!D3) {
     const double c = g[1], s = g[2];
     const double sl = s * oneOverL, cl = c * oneOverL;
@@ -667,6 +614,88 @@ __device__ __forceinline__ void element_tangent(const FrameDev& f, int e, double
         K[(3 * b + c) * 12 + m] =
             R[0 * 3 + c] * col[3 * b] + R[1 * 3 + c] * col[3 * b + 1] + R[2 * 3 + c] * col[3 * b + 2];
   }
+}
+
+// Element tangent: kb = sum_i B_i^T ks_i B_i wt_i / L (getBasicStiff, 2D
+// :287-355; 3D getTangentStiff :407-537), K = T^T kb T (LinearCrdTransf2d::
+// getGlobalStiffMatrix :610-733, LinearCrdTransf3d :768).  Row-major output.
+template <bool D3>
+__device__ __forceinline__ void element_tangent(const FrameDev& f, int e, double* __restrict__ Ke) {
+  constexpr int nb = D3 ? 6 : 3;
+  constexpr int ord = D3 ? 4 : 2;
+  constexpr int ld = D3 ? 4 : 2;
+  const long long n = f.nsec;
+  const double* g = f.geo + (long long)e * 10;
+  const double oneOverL = 1.0 / g[0];
+  double kb[6][6];
+#pragma unroll
+  for (int i = 0; i < 6; i++)
+#pragma unroll
+    for (int j = 0; j < 6; j++) kb[i][j] = 0.0;
+  for (int i = 0; i < f.nip; i++) {
+    const long long sec = (long long)e * f.nip + i;
+    const double xi6 = 6.0 * (0.5 * (1.0 + gl5_pt(i)));
+    const double wti = (gl5_wt(i) * 0.5) * oneOverL;
+    double ks[ord][ord];
+#pragma unroll
+    for (int a = 0; a < ord; a++)
+#pragma unroll
+      for (int b = 0; b < ord; b++) ks[a][b] = f.ks[(long long)(b * ld + a) * n + sec];
+    double ka[ord][6];
+#pragma unroll
+    for (int a = 0; a < ord; a++)
+#pragma unroll
+      for (int b = 0; b < 6; b++) ka[a][b] = 0.0;
+#pragma unroll
+    for (int j = 0; j < ord; j++) {
+      if (j == 0) {
+#pragma unroll
+        for (int k = 0; k < ord; k++) ka[k][0] += ks[k][j] * wti;
+      } else if (j == 1) {
+#pragma unroll
+        for (int k = 0; k < ord; k++) {
+          const double tmp = ks[k][j] * wti;
+          ka[k][1] += (xi6 - 4.0) * tmp;
+          ka[k][2] += (xi6 - 2.0) * tmp;
+        }
+      } else if (j == 2) {
+#pragma unroll
+        for (int k = 0; k < ord; k++) {
+          const double tmp = ks[k][j] * wti;
+          ka[k][3] += (xi6 - 4.0) * tmp;
+          ka[k][4] += (xi6 - 2.0) * tmp;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < ord; k++) ka[k][5] += ks[k][j] * wti;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < ord; j++) {
+      if (j == 0) {
+#pragma unroll
+        for (int k = 0; k < nb; k++) kb[0][k] += ka[j][k];
+      } else if (j == 1) {
+#pragma unroll
+        for (int k = 0; k < nb; k++) {
+          const double tmp = ka[j][k];
+          kb[1][k] += (xi6 - 4.0) * tmp;
+          kb[2][k] += (xi6 - 2.0) * tmp;
+        }
+      } else if (j == 2) {
+#pragma unroll
+        for (int k = 0; k < nb; k++) {
+          const double tmp = ka[j][k];
+          kb[3][k] += (xi6 - 4.0) * tmp;
+          kb[4][k] += (xi6 - 2.0) * tmp;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < nb; k++) kb[5][k] += ka[j][k];
+      }
+    }
+  }
+  global_stiff<D3>(f, e, kb, Ke);
 }
 
 }  // namespace xg

@@ -141,7 +141,7 @@ def rc_frame_3d(nbx=10, nby=10, nst=20, bay=6000.0, story=3500.0):
                 roof=roof, height=story * nst)
 
 
-def build_rc_frame_3d(model, fr, GJ=1.1e14, loads=True):
+def build_rc_frame_3d(model, fr, GJ=1.1e14, loads=True, element="dispBeamColumn", nip=5):
     model.nodes(fr["node_tags"], fr["crds"])
     fx = fr["fixed_nodes"]
     model.fix_many(fx, np.ones((len(fx), 6), np.int32))
@@ -151,9 +151,9 @@ def build_rc_frame_3d(model, fr, GJ=1.1e14, loads=True):
     model.geomTransf("Linear", 1, 1.0, 0.0, 0.0)  # columns (axis z)
     model.geomTransf("Linear", 2, 0.0, 0.0, 1.0)  # beams
     nc, nx_, ny_ = len(fr["columns"]), len(fr["beams_x"]), len(fr["beams_y"])
-    model.elements("dispBeamColumn", np.arange(1, nc + 1), fr["columns"], 1, 1, 5)
-    model.elements("dispBeamColumn", np.arange(nc + 1, nc + nx_ + 1), fr["beams_x"], 2, 1, 5)
-    model.elements("dispBeamColumn", np.arange(nc + nx_ + 1, nc + nx_ + ny_ + 1), fr["beams_y"], 2, 1, 5)
+    model.elements(element, np.arange(1, nc + 1), fr["columns"], 1, 1, nip)
+    model.elements(element, np.arange(nc + 1, nc + nx_ + 1), fr["beams_x"], 2, 1, nip)
+    model.elements(element, np.arange(nc + nx_ + 1, nc + nx_ + ny_ + 1), fr["beams_y"], 2, 1, nip)
     if loads:
         P = np.zeros((len(fr["load_tags"]), 6))
         P[:, 0] = fr["Px"]
@@ -235,7 +235,7 @@ def frame_elements(fr):
     return conn, transf
 
 
-def build_rc_frame_per_element(model, fr, theta, GJ=1.1e14):
+def build_rc_frame_per_element(model, fr, theta, GJ=1.1e14, element="dispBeamColumn", nip=5):
     """RC frame (rc_frame_2d or rc_frame_3d layout, no loads) where element
     e (tag e, 1-based) has its own Steel02 (tag STEEL_TAG0 + e), Concrete02
     (tag CONC_TAG0 + e) and fiber section (tag e) with the parameters of row
@@ -259,15 +259,16 @@ def build_rc_frame_per_element(model, fr, theta, GJ=1.1e14):
         model.uniaxialMaterial("Concrete02", CONC_TAG0 + tag, fc, *CONCRETE02[1:])
         model.section("Fiber", tag, rc_section_fibers(conc_tag=CONC_TAG0 + tag, steel_tag=STEEL_TAG0 + tag),
                       GJ=GJ if three_d else None)
-        model.elements("dispBeamColumn", [tag], conn[e:e + 1], int(transf[e]), tag, 5)
+        model.elements(element, [tag], conn[e:e + 1], int(transf[e]), tag, nip)
 
 
-def define_transient_frame(model, fr, theta, ag, dt, mass=60.0, alphaM=0.35, tol=1e-8, maxit=25):
+def define_transient_frame(model, fr, theta, ag, dt, mass=60.0, alphaM=0.35, tol=1e-8, maxit=25,
+                           element="dispBeamColumn"):
     """Per-element RC frame (2D or 3D) under a UniformExcitation record in x:
     lumped translational floor masses, mass-proportional damping, Newmark
     average acceleration, NormDispIncr test."""
     three_d = "beams_x" in fr
-    build_rc_frame_per_element(model, fr, theta)
+    build_rc_frame_per_element(model, fr, theta, element=element)
     floor = fr["floor_nodes"] if "floor_nodes" in fr else fr["load_tags"]
     ndf = 6 if three_d else 3
     m = np.zeros((len(floor), ndf))
