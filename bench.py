@@ -76,7 +76,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run the 256 MiB copy kernel (profiler counter calibration)")
-    ap.add_argument("--mode", choices=["update", "newton", "quad", "frame", "ensemble", "adjoint"], default="update")
+    ap.add_argument("--mode", choices=["update", "newton", "quad", "frame", "force", "ensemble", "adjoint"],
+                    default="update")
     ap.add_argument("--adj-frame", default="5,5,10", help="bays x, bays y, stories of the adjoint mode")
     ap.add_argument("--adj-steps", type=int, default=500)
     ap.add_argument("--ckpt", type=int, default=25)
@@ -101,6 +102,8 @@ def main():
         return newton_main(args)
     if args.mode == "frame":
         return frame_main(args)
+    if args.mode == "force":
+        return force_main(args)
     if args.mode == "quad":
         return quad_main(args)
     if args.mode == "ensemble":
@@ -363,6 +366,55 @@ def frame_main(args):
         out["newton"] = {"steps": args.frame_steps, "newton_iters": c["newton_iters"] - c0["newton_iters"],
                          "cg_iters": c["cg_iters"] - c0["cg_iters"], "wall_s": wall,
                          "newton_iters_per_s": (c["newton_iters"] - c0["newton_iters"]) / wall}
+    print(json.dumps(out))
+
+
+def force_main(args):
+    """Pushover of the 3D RC moment frame built from forceBeamColumn3d
+    elements (5 Lobatto sections of the same 120-fiber RC section): Newton
+    iterations/s of a DisplacementControl pushover on the GPU, and the same
+    pushover on a reduced frame by the CPU oracle."""
+    from xara_amd import meshes
+    from xara_amd.model import Model
+
+    def pushover(m, fr, steps):
+        meshes.build_rc_frame_3d(m, fr, element="forceBeamColumn")
+        m.system(args.system, 1e-12, 200000)
+        m.test("NormUnbalance", 1e-3, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 0.03 * fr["height"] / 300.0)
+        t0 = time.time()
+        m.setup()
+        setup_s = time.time() - t0
+        c0 = m.counters()
+        t0 = time.perf_counter()
+        for st in range(steps):
+            rc = m.analyze(1)
+            if rc != 0:
+                raise RuntimeError(f"force-based pushover step {st + 1} failed: {rc}")
+        wall = time.perf_counter() - t0
+        c = m.counters()
+        return wall, c["newton_iters"] - c0["newton_iters"], c["cg_iters"] - c0["cg_iters"], setup_s
+
+    nbx, nby, nst = (int(v) for v in args.frame.split(","))
+    fr = meshes.rc_frame_3d(nbx, nby, nst)
+    steps = args.frame_steps if args.frame_steps > 0 else 10
+    wall, it, cg, setup_s = pushover(Model(3, 6, device=0), fr, steps)
+    nele = len(fr["columns"]) + len(fr["beams_x"]) + len(fr["beams_y"])
+    out = {
+        "metric": "Newton iterations/s", "value": it / wall, "unit": "iterations/s", "n_gpus": 1,
+        "steps": steps, "warmup": 0, "ms_per_step": 1e3 * wall / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3D RC frame {nbx}x{nby} bays x {nst} stories of forceBeamColumn3d (5 Lobatto "
+                               f"sections), DisplacementControl pushover to {steps * 0.01:.2f}% roof drift",
+                   "elements": nele, "sections": 5 * nele, "fibers": 120 * 5 * nele, "newton_iters": it,
+                   "cg_iters": cg, "setup_s": round(setup_s, 2), "system": args.system},
+    }
+    if not args.no_cpu:
+        from tests import _oracle
+        small = meshes.rc_frame_3d(2, 2, 4)
+        cwall, cit, _, _ = pushover(_oracle.oracle_model(3, 6), small, steps)
+        out["cpu_baseline"] = {"value": cit / cwall, "unit": "iterations/s", "cores": 1, "kind": "port",
+                               "sample": f"the same pushover of a 2x2x4 frame ({steps} steps), oracle direct solve"}
     print(json.dumps(out))
 
 

@@ -106,15 +106,16 @@ def test_quad_strip_from_json():
     same_model(loaded, direct, 4 * 16, U)
 
 
+@pytest.mark.parametrize("element", ["dispBeamColumn", "forceBeamColumn"])
 @pytest.mark.parametrize("three_d", [False, True])
-def test_rc_frame_from_json(three_d):
+def test_rc_frame_from_json(three_d, element):
     fr = meshes.rc_frame_3d(1, 1, 2) if three_d else meshes.rc_frame_2d(2, 2)
     ndm, ndf = (3, 6) if three_d else (2, 3)
     conn, transf = meshes.frame_elements(fr)
     theta = meshes.default_theta(len(conn))
     theta[:, 0] *= np.linspace(0.9, 1.1, len(conn))
     direct = _oracle.oracle_model(ndm, ndf)
-    meshes.build_rc_frame_per_element(direct, fr, theta)
+    meshes.build_rc_frame_per_element(direct, fr, theta, element=element)
     floor = fr["load_tags"] if three_d else fr["floor_nodes"]
     m = np.zeros((len(floor), ndf))
     m[:, 0] = 60.0
@@ -136,9 +137,10 @@ def test_rc_frame_from_json(three_d):
                    {"name": "2", "type": "LinearCrdTransf3d", "vecInLocXZPlane": [0.0, 0.0, 1.0]}]
     else:
         transfs = [{"name": "1", "type": "LinearCrdTransf2d"}]
-    kind = "DispBeamColumn3d" if three_d else "DispBeamColumn2d"
+    kind = ("DispBeamColumn" if element == "dispBeamColumn" else "ForceBeamColumn") + ("3d" if three_d else "2d")
+    rule = "Legendre" if element == "dispBeamColumn" else "Lobatto"
     eles = [{"name": e + 1, "type": kind, "nodes": [int(conn[e, 0]), int(conn[e, 1])],
-             "sections": [str(e + 1)] * 5, "integration": {"type": "Legendre"}, "massperlength": 0,
+             "sections": [str(e + 1)] * 5, "integration": {"type": rule}, "massperlength": 0,
              "crdTransformation": str(int(transf[e]))} for e in range(len(conn))]
     d = doc({"sections": secs, "nDMaterials": [], "uniaxialMaterials": uni, "crdTransformations": transfs},
             {"nodes": node_json(fr["node_tags"], fr["crds"], ndf, mass), "elements": eles,

@@ -20,7 +20,9 @@ SRC/runtime/commands/modeling/printing.cpp:135-295):
   LinearCrdTransf3d.cpp:1536-1548);
 * geometry.elements: Brick (Brick.cpp:224-237), FourNodeQuad
   (FourNodeQuad.cpp:776-797, plane strain), DispBeamColumn2d/3d
-  (DispBeamColumn2d.cpp:1064-1077, DispBeamColumn3d.cpp:1149-1162).
+  (DispBeamColumn2d.cpp:1064-1077, DispBeamColumn3d.cpp:1149-1162),
+  ForceBeamColumn2d/3d with Lobatto integration (ForceBeamColumn2d.cpp
+  Print, the same fields).
 
 Anything else raises ``XaraError`` naming the object, rather than being
 skipped.  The reference's JSON carries no load values (NodalLoad.cpp:210-215
@@ -178,6 +180,17 @@ def load_json(model, doc):
                 raise _err(f"element {tag}: {integ} integration is not on the GPU path")
             key = ("dispBeamColumn", _tag(e["crdTransformation"]), secs[0], len(secs),
                    _num(e.get("massperlength", 0.0)))
+        elif t in ("ForceBeamColumn2d", "ForceBeamColumn3d"):
+            secs = [_tag(x) for x in e["sections"]]
+            if len(set(secs)) != 1:
+                raise _err(f"element {tag}: one section at every integration point is required")
+            integ = e.get("integration", {}).get("type", "Lobatto")
+            if integ != "Lobatto":
+                raise _err(f"element {tag}: {integ} integration is not on the GPU path")
+            if _num(e.get("massperlength", 0.0)) != 0.0:
+                raise _err(f"element {tag}: forceBeamColumn mass is not supported")
+            # maxIters and tol are not printed: the command's defaults (10, 1e-12)
+            key = ("forceBeamColumn", _tag(e["crdTransformation"]), secs[0], len(secs))
         else:
             raise _err(f"element {tag} of type {t} is not on the GPU path")
         groups.setdefault(key, ([], []))
