@@ -91,21 +91,30 @@ __device__ __forceinline__ double read_lane_d(double x, int l) {
 // and sends the other half, so the K totals take K - 1 + log2(64/K)
 // shuffles instead of 6K.  The total of value v ends in lanes (64/K) v ..
 // (64/K) v + 64/K - 1 and is returned to every lane in x[v].
+// The values are held in a vector register value rather than an array, so
+// that the compiler keeps each keep/send choice a register select (with an
+// array it may turn the two loads into one load at a lane-dependent index,
+// which it then expands into a compare-and-select chain over all K values).
 template <int K, int N>
 __device__ __forceinline__ void wave_sums(double (&x)[K], int lane) {
   constexpr int G = 64 / K;
+  typedef double vec_t __attribute__((ext_vector_type(K)));
+  vec_t v;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = x[k];
 #pragma unroll
   for (int c = K; c > 1; c >>= 1) {
     const int o = G * c / 2;  // 32, 16, ... for c = K, K/2, ...
     const bool up = (lane & o) != 0;
 #pragma unroll
     for (int i = 0; i < c / 2; i++) {
-      const double keep = up ? x[i + c / 2] : x[i];
-      const double send = up ? x[i] : x[i + c / 2];
-      x[i] = keep + __shfl_xor(send, o, 64);
+      const double lo = v[i], hi = v[i + c / 2];
+      const double keep = up ? hi : lo;
+      const double send = up ? lo : hi;
+      v[i] = keep + __shfl_xor(send, o, 64);
     }
   }
-  double t = x[0];
+  double t = v[0];
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
 #pragma unroll
