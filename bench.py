@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--system", default="GpuBlockPCG")
     ap.add_argument("--spmv", choices=["auto", "csr"], default="auto", help="SpMV layout of the PCG")
     ap.add_argument("--lanes", type=int, default=0, help="SpMV threads per row (0 = from the row length)")
+    ap.add_argument("--cg-graph", type=int, choices=[0, 1], default=1,
+                    help="replay PCG chunks from captured graphs (0 = launch each kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -177,6 +179,13 @@ def main():
     print(json.dumps(out))
 
 
+def gpu_system(m, args, rtol, maxit):
+    """The PCG system of a GPU model with the bench's issue pattern."""
+    m.system(args.system, rtol, maxit)
+    if m._b.prefix == "xg_":
+        m.cg_config(16, bool(args.cg_graph))
+
+
 def newton_main(args):
     """Newton iterations/s of the full pushover: cube under uniform top
     pressure ramped to 1.5x the first-yield pressure in `newton_steps`
@@ -188,7 +197,7 @@ def newton_main(args):
     mesh = meshes.brick_cube(n)
     m = Model(3, 3, device=0)
     meshes.build_brick_cube(m, mesh, pressure=1.0)
-    m.system(args.system, 1e-10, 50000)
+    gpu_system(m, args, 1e-10, 50000)
     m.spmv_config(args.spmv, args.lanes)
     t0 = time.time()
     m.setup()
@@ -265,7 +274,7 @@ def quad_main(args):
     for name, m in (("gpu", Model(2, 2)), ("cpu", _oracle.oracle_model(2, 2))):
         define(m)
         if name == "gpu":
-            m.system(args.system, 1e-12, 100000)
+            gpu_system(m, args, 1e-12, 100000)
         m.setup()
         c0 = m.counters()
         t0 = time.perf_counter()
@@ -321,7 +330,7 @@ def frame_main(args):
     fr = meshes.rc_frame_3d(nbx, nby, nst)
     m = Model(3, 6, device=0)
     meshes.build_rc_frame_3d(m, fr)
-    m.system(args.system, 1e-12, 200000)
+    gpu_system(m, args, 1e-12, 200000)
     t0 = time.time()
     m.setup()
     setup_s = time.time() - t0
@@ -379,7 +388,7 @@ def force_main(args):
 
     def pushover(m, fr, steps):
         meshes.build_rc_frame_3d(m, fr, element="forceBeamColumn")
-        m.system(args.system, 1e-12, 200000)
+        gpu_system(m, args, 1e-12, 200000)
         m.test("NormUnbalance", 1e-3, 40)
         m.integrator("DisplacementControl", fr["roof"], 1, 0.03 * fr["height"] / 300.0)
         t0 = time.time()
@@ -486,6 +495,7 @@ def ensemble_main(args):
         m = Model(2, 3)
         define(m, k)
         m.system("GpuBlockPCG", 1e-14, 100000)
+        m.cg_config(16, bool(args.cg_graph))
         m.setup()
         for _ in range(args.warmup):
             m.analyze(1, dt)
@@ -572,7 +582,7 @@ def adjoint_main(args):
 
     def build(m, theta):
         meshes.define_transient_frame(m, fr, theta, ag, dt, tol=1e-12, maxit=40)
-        m.system(args.system, 1e-12, 200000)
+        gpu_system(m, args, 1e-12, 200000)
 
     roof = meshes.roof_nodes(fr)
     prob = FrameTransient(3, 6, build, roof, N, dt, ckpt=args.ckpt)

@@ -159,6 +159,12 @@ int  xg_system(xg_domain* d, int kind, double rtol, int maxit);
 #define XG_SPMV_AUTO 0
 #define XG_SPMV_CSR  1
 int  xg_spmv_config(xg_domain* d, int format, int lanes);
+/* PCG issue pattern.  No counterpart in the reference: a tuning knob.  The
+   host issues `chunk` iterations (even, 2..1024) between reads of the
+   convergence flag (iterations after convergence return at once); with
+   graph = 1 (default) a full chunk is replayed from one captured hipGraph
+   of its kernel launches.  Results do not depend on either setting.          */
+int  xg_cg_config(xg_domain* d, int chunk, int graph);
 /* `numberer RCM|Plain`                                                       */
 int  xg_numberer(xg_domain* d, int kind);
 /* `test NormUnbalance tol maxIter` etc.                                      */

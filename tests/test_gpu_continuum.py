@@ -6,7 +6,7 @@ import pytest
 
 from tests import _oracle
 from xara_amd import meshes
-from xara_amd.model import Model
+from xara_amd.model import Model, XaraError
 
 pytestmark = pytest.mark.gpu
 
@@ -189,6 +189,44 @@ def test_pcg_solve_csr_layout():
         m.form_tangent(values=False)
     b = np.random.default_rng(4).normal(size=g.neq)
     assert relerr(g.solve(b), o.solve(b)) <= 1e-9
+
+
+def test_pcg_graph_replay_matches_eager_launches():
+    """PCG chunks replayed from a captured graph give the same iterates, bit
+    for bit, as the same launches issued one by one and as another chunk
+    length: a frame DisplacementControl pushover (two solution vectors per
+    iteration, hence two graphs, with element updates between the replays)
+    interleaved with direct solves into a third buffer."""
+    print("case: pcg graph replay vs eager", flush=True)
+    fr = meshes.rc_frame_2d(3, 2)
+    conn, _ = meshes.frame_elements(fr)
+    theta = meshes.default_theta(len(conn))
+    runs = []
+    for chunk, graph in ((16, True), (16, False), (10, True)):
+        m = Model(2, 3)
+        meshes.build_rc_frame_per_element(m, fr, theta)
+        m.system("GpuBlockPCG", 1e-12, 100000)
+        m.cg_config(chunk, graph)
+        P = np.zeros((len(fr["floor_nodes"]), 3))
+        P[:, 0] = 1.0
+        m.loads(fr["floor_nodes"], P)
+        m.test("NormUnbalance", 1e-5, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 6.0)
+        m.setup()
+        b = np.random.default_rng(11).normal(size=m.neq)
+        t = []
+        for step in range(6):
+            assert m.analyze(1) == 0
+            t.append(m.trial_disp().copy())
+            t.append(m.solve(b))
+        t.append(np.array([m.counters()["cg_iters"]], dtype=float))
+        runs.append(t)
+        m.close()
+    for other in runs[1:]:
+        for a, c in zip(runs[0], other):
+            assert np.array_equal(a, c)
+    with pytest.raises(XaraError):
+        Model(2, 3).cg_config(7, True)
 
 
 def test_revert_to_start_and_to_last_commit():

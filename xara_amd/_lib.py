@@ -58,6 +58,7 @@ SIGNATURES = {
     "system": (C.c_int, [VP, C.c_int, C.c_double, C.c_int]),
     "numberer": (C.c_int, [VP, C.c_int]),
     "spmv_config": (C.c_int, [VP, C.c_int, C.c_int]),
+    "cg_config": (C.c_int, [VP, C.c_int, C.c_int]),
     "test": (C.c_int, [VP, C.c_int, C.c_double, C.c_int]),
     "integrator_load_control": (C.c_int, [VP, C.c_double]),
     "integrator_disp_control": (C.c_int, [VP, C.c_int, C.c_int, C.c_double]),
@@ -110,7 +111,7 @@ SIGNATURES = {
     "material_vjp": (C.c_int, [C.c_int, P_D, P_D, C.c_int, C.c_double, P_D, P_D, P_D, P_D]),
 }
 GPU_ONLY = {k for k in SIGNATURES if k.startswith(("ensemble_", "adjoint_"))} | {
-    "device_name", "material_vjp", "spmv_config"}
+    "device_name", "material_vjp", "spmv_config", "cg_config"}
 
 
 class Binding:

@@ -205,6 +205,7 @@ struct J2Group : Group {
 // Domain
 // ---------------------------------------------------------------------------
 Domain::~Domain() {
+  cg_graph_clear();
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   groups.clear();
@@ -581,6 +582,58 @@ void Domain::precond_setup() {
   launch_block_inverse(stream, nblk, blk_first.p, blk_size.p, rowptr.p, colind.p, vals.p, bw, binv.p, pcbad.p);
 }
 
+void Domain::cg_graph_clear() {
+  for (auto& g : cg_graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  cg_graphs.clear();
+}
+
+// One chunk of PCG iterations.  A full chunk (cg_chunk iterations, even, so
+// that it starts on an even iteration like the captured one) is replayed
+// from a graph of its launches: one host call instead of five kernel
+// launches per iteration, which bounds the iteration rate of small systems.
+// Graphs hold kernels only, one per set of launch arguments (at most
+// kCgGraphs; further sets are launched one by one).
+void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
+  const int n = num.neq, nbm = spmv_nbm();
+  auto eager = [&](hipStream_t s, int k, int i0) {
+    cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, q.p, bw, binv.p, bfirst.p,
+                  bsize.p, part.p, scal.p, nbm, nbcol.p);
+  };
+  if (!cg_graph_on || iters != cg_chunk || (it0 & 1) || (cg_chunk & 1)) {
+    eager(stream, iters, it0);
+    return;
+  }
+  const std::vector<uintptr_t> key = {
+      (uintptr_t)iters,   (uintptr_t)L,       (uintptr_t)n,       (uintptr_t)bw,      (uintptr_t)nbm,
+      (uintptr_t)x,       (uintptr_t)rowptr.p, (uintptr_t)colind.p, (uintptr_t)vals.p, (uintptr_t)r0.p,
+      (uintptr_t)r1.p,    (uintptr_t)z.p,     (uintptr_t)p.p,     (uintptr_t)q.p,     (uintptr_t)binv.p,
+      (uintptr_t)bfirst.p, (uintptr_t)bsize.p, (uintptr_t)part.p, (uintptr_t)scal.p,  (uintptr_t)nbcol.p};
+  for (auto& g : cg_graphs)
+    if (g.key == key) {
+      XG_HIP(hipGraphLaunch(g.exec, stream));
+      return;
+    }
+  if ((int)cg_graphs.size() >= kCgGraphs) {
+    eager(stream, iters, it0);
+    return;
+  }
+  CgGraph g;
+  g.key = key;
+  XG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  eager(stream, iters, 0);
+  XG_HIP(hipStreamEndCapture(stream, &g.graph));
+  const hipError_t e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g.graph);
+    XG_HIP(e);
+  }
+  cg_graphs.push_back(g);
+  XG_HIP(hipGraphLaunch(g.exec, stream));
+}
+
 int Domain::solve(const double* b, double* x) {
   const int n = num.neq;
   if (n == 0) return 0;
@@ -599,8 +652,7 @@ int Domain::solve(const double* b, double* x) {
     if (conv) break;
     if (issued >= maxit) break;
     const int chunk = std::min(cg_chunk, maxit - issued);
-    cg_iterations(stream, chunk, issued, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, q.p, bw,
-                  binv.p, bfirst.p, bsize.p, part.p, scal.p, spmv_nbm(), nbcol.p);
+    cg_chunk_run(chunk, issued, L, bw, x);
     issued += chunk;
   }
   counters.solves++;

@@ -1,5 +1,6 @@
 // One finite-element model resident on one GPU (the xg_domain handle).
 #pragma once
+#include <cstdint>
 #include <map>
 #include <memory>
 #include <string>
@@ -119,7 +120,18 @@ struct Domain {
   int maxit = 20000;
   int spmv_lanes = 0;  // 0 = choose from the mean row length
   int spmv_format = XG_SPMV_AUTO;
-  int cg_chunk = 16;
+  int cg_chunk = 16;  // PCG iterations issued between convergence reads (even)
+  // full chunks replayed from captured graphs (kernels only), one per set of
+  // launch arguments (the solution vector differs between the solves of
+  // DisplacementControl, for example); kept until teardown or cg_config
+  struct CgGraph {
+    std::vector<uintptr_t> key;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  static constexpr int kCgGraphs = 8;
+  bool cg_graph_on = true;
+  std::vector<CgGraph> cg_graphs;
   int test_kind = XG_TEST_NORM_UNBALANCE;
   double test_tol = 1e-6;
   int test_maxit = 25;
@@ -169,6 +181,8 @@ struct Domain {
   int update_checked();  // update(); < 0 if an element update failed (AnalysisModel::updateDomain)
   int run_test(int& iter, std::vector<double>& norms, int& numIncr);
   void precond_setup();
+  void cg_chunk_run(int iters, int it0, int L, int bw, double* x);
+  void cg_graph_clear();
   int lanes();
   std::vector<int> node_block_columns();
   int spmv_nbm() const { return spmv_format == XG_SPMV_CSR ? 0 : nb_m; }

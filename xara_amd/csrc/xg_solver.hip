@@ -272,16 +272,20 @@ __global__ __launch_bounds__(256) void k_cg_alpha(int nb, const double* __restri
 }
 
 // x += alpha p; r_out = r_in - alpha q; z = M^{-1} r_out; partials of r.z
-// and r.r.  M^{-1}: block rows binv[i*bw + j] over the nodal block starting
-// at bfirst[i] with bsize[i] entries (bw = 1 gives Jacobi).  r is double
+// and r.r.  M^{-1}: block rows binv[i*BW + j] over the nodal block starting
+// at bfirst[i] with bsize[i] <= BW entries (BW = 1 gives Jacobi).  r is double
 // buffered, so a row's block neighbours are recomputed from r_in with the
 // same arithmetic instead of being read while another thread block writes.
+// The BW loads of a row are issued together (clamped into the block) and the
+// products summed in column order, so a small system waits for memory once
+// per row rather than once per block column.
+template <int BW>
 __global__ __launch_bounds__(256) void k_cg_update(int n, const CGScal* __restrict__ s,
                                                    const double* __restrict__ p,
                                                    const double* __restrict__ q, double* __restrict__ x,
                                                    const double* __restrict__ r_in,
                                                    double* __restrict__ r_out, double* __restrict__ z,
-                                                   int bw, const double* __restrict__ binv,
+                                                   const double* __restrict__ binv,
                                                    const int* __restrict__ bfirst,
                                                    const unsigned char* __restrict__ bsize,
                                                    double* __restrict__ part) {
@@ -294,15 +298,21 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, const CGScal* __restri
     const double ri = r_in[i] - alpha * q[i];
     r_out[i] = ri;
     double zi;
-    if (bw == 1) {
+    if constexpr (BW == 1) {
       zi = binv[i] * ri;
     } else {
       const int f = bfirst[i], m = bsize[i];
-      zi = 0.0;
-      for (int j = 0; j < m; j++) {
-        const int c = f + j;
-        zi += binv[(long long)i * bw + j] * (r_in[c] - alpha * q[c]);
+      double w[BW], rc[BW];
+#pragma unroll
+      for (int j = 0; j < BW; j++) {
+        const int c = f + min(j, m - 1);
+        w[j] = binv[(long long)i * BW + j];
+        rc[j] = r_in[c] - alpha * q[c];
       }
+      zi = 0.0;
+#pragma unroll
+      for (int j = 0; j < BW; j++)
+        if (j < m) zi += w[j] * rc[j];
     }
     z[i] = zi;
     rz += ri * zi;
@@ -672,8 +682,19 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
     launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop, nbm, nbcol);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, nbs, part, S);
     const bool even = ((it0 + it) & 1) == 0;
-    hipLaunchKernelGGL(k_cg_update, dim3(nbv), dim3(256), 0, s, n, S, p, q, x, even ? r0 : r1,
-                       even ? r1 : r0, z, bw, binv, bfirst, bsize, part);
+    double* ri = even ? r0 : r1;
+    double* ro = even ? r1 : r0;
+    switch (bw) {
+#define XG_CGU(W)                                                                                              \
+  case W:                                                                                                     \
+    hipLaunchKernelGGL(k_cg_update<W>, dim3(nbv), dim3(256), 0, s, n, S, p, q, x, ri, ro, z, binv, bfirst, bsize, \
+                       part);                                                                                 \
+    break;
+      XG_CGU(1) XG_CGU(2) XG_CGU(3) XG_CGU(4) XG_CGU(5) XG_CGU(6)
+#undef XG_CGU
+      default:
+        throw Error(XG_ERR_ARG, "PCG block width must be 1..6");
+    }
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, nbv, part, S);
     hipLaunchKernelGGL(k_cg_newp, dim3(nbv), dim3(256), 0, s, n, S, z, p);
   }
