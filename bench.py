@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="SpMV threads per row (0 = from the row length)")
     ap.add_argument("--cg-graph", type=int, choices=[0, 1], default=1,
                     help="replay PCG chunks from captured graphs (0 = launch each kernel)")
+    ap.add_argument("--cg-fuse", type=int, choices=[0, 1], default=1,
+                    help="alpha / beta reductions inside the PCG vector kernels on small systems")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -183,7 +185,7 @@ def gpu_system(m, args, rtol, maxit):
     """The PCG system of a GPU model with the bench's issue pattern."""
     m.system(args.system, rtol, maxit)
     if m._b.prefix == "xg_":
-        m.cg_config(16, bool(args.cg_graph))
+        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse))
 
 
 def newton_main(args):
@@ -495,7 +497,7 @@ def ensemble_main(args):
         m = Model(2, 3)
         define(m, k)
         m.system("GpuBlockPCG", 1e-14, 100000)
-        m.cg_config(16, bool(args.cg_graph))
+        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse))
         m.setup()
         for _ in range(args.warmup):
             m.analyze(1, dt)

@@ -160,11 +160,17 @@ int  xg_system(xg_domain* d, int kind, double rtol, int maxit);
 #define XG_SPMV_CSR  1
 int  xg_spmv_config(xg_domain* d, int format, int lanes);
 /* PCG issue pattern.  No counterpart in the reference: a tuning knob.  The
-   host issues `chunk` iterations (even, 2..1024) between reads of the
-   convergence flag (iterations after convergence return at once); with
-   graph = 1 (default) a full chunk is replayed from one captured hipGraph
-   of its kernel launches.  Results do not depend on either setting.          */
-int  xg_cg_config(xg_domain* d, int chunk, int graph);
+   host issues `chunk` iterations (even, 2..1024; default 16) between reads
+   of the convergence flag (iterations after convergence return at once).
+   XG_CG_GRAPH: a full chunk is replayed from a captured hipGraph of its
+   kernel launches.  XG_CG_FUSE: on systems small enough (at most 1024
+   partial sums per reduction) the alpha and beta reductions run inside the
+   update and new-direction kernels, three launches per iteration instead of
+   five.  Default XG_CG_GRAPH | XG_CG_FUSE; the iterates are the same bits
+   under every setting.                                                       */
+#define XG_CG_GRAPH 1
+#define XG_CG_FUSE  2
+int  xg_cg_config(xg_domain* d, int chunk, int flags);
 /* `numberer RCM|Plain`                                                       */
 int  xg_numberer(xg_domain* d, int kind);
 /* `test NormUnbalance tol maxIter` etc.                                      */

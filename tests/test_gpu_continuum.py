@@ -192,8 +192,9 @@ def test_pcg_solve_csr_layout():
 
 
 def test_pcg_graph_replay_matches_eager_launches():
-    """PCG chunks replayed from a captured graph give the same iterates, bit
-    for bit, as the same launches issued one by one and as another chunk
+    """PCG chunks replayed from a captured graph, and the alpha / beta
+    reductions fused into the vector kernels, give the same iterates, bit
+    for bit, as the separate launches issued one by one and as another chunk
     length: a frame DisplacementControl pushover (two solution vectors per
     iteration, hence two graphs, with element updates between the replays)
     interleaved with direct solves into a third buffer."""
@@ -202,11 +203,12 @@ def test_pcg_graph_replay_matches_eager_launches():
     conn, _ = meshes.frame_elements(fr)
     theta = meshes.default_theta(len(conn))
     runs = []
-    for chunk, graph in ((16, True), (16, False), (10, True)):
+    for chunk, graph, fuse in ((16, True, True), (16, False, False), (10, True, True), (16, False, True),
+                               (16, True, False)):
         m = Model(2, 3)
         meshes.build_rc_frame_per_element(m, fr, theta)
         m.system("GpuBlockPCG", 1e-12, 100000)
-        m.cg_config(chunk, graph)
+        m.cg_config(chunk, graph, fuse)
         P = np.zeros((len(fr["floor_nodes"]), 3))
         P[:, 0] = 1.0
         m.loads(fr["floor_nodes"], P)

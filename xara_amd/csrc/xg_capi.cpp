@@ -389,14 +389,15 @@ int xg_spmv_config(xg_domain* h, int format, int lanes) {
   });
 }
 
-int xg_cg_config(xg_domain* h, int chunk, int graph) {
+int xg_cg_config(xg_domain* h, int chunk, int flags) {
   return guard([&] {
     auto& d = D(h);
     if (chunk < 2 || chunk > 1024 || (chunk & 1)) throw xg::Error(XG_ERR_ARG, "cg chunk must be even, 2..1024");
-    if (graph != 0 && graph != 1) throw xg::Error(XG_ERR_ARG, "cg graph flag must be 0 or 1");
+    if (flags & ~(XG_CG_GRAPH | XG_CG_FUSE)) throw xg::Error(XG_ERR_ARG, "cg flags: XG_CG_GRAPH | XG_CG_FUSE");
     d.cg_graph_clear();
     d.cg_chunk = chunk;
-    d.cg_graph_on = graph != 0;
+    d.cg_graph_on = (flags & XG_CG_GRAPH) != 0;
+    d.cg_fuse_on = (flags & XG_CG_FUSE) != 0;
     return 0;
   });
 }

@@ -480,7 +480,7 @@ void Domain::setup() {
   p.resize(neq);
   q.resize(neq);
   tmpv.resize(neq);
-  part.resize(2 * (size_t)std::max(ceil_div(std::max(neq, 1), 256), spmv_blocks(std::max(neq, 1), 64)) + 16);
+  part.resize(cg_part_size(neq));
   pcbad.resize(1);
   binv.resize((size_t)neq * ndf);
   bfirst.upload(bf, s);
@@ -600,14 +600,14 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
   const int n = num.neq, nbm = spmv_nbm();
   auto eager = [&](hipStream_t s, int k, int i0) {
     cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, q.p, bw, binv.p, bfirst.p,
-                  bsize.p, part.p, scal.p, nbm, nbcol.p);
+                  bsize.p, part.p, scal.p, nbm, nbcol.p, cg_fuse_on);
   };
   if (!cg_graph_on || iters != cg_chunk || (it0 & 1) || (cg_chunk & 1)) {
     eager(stream, iters, it0);
     return;
   }
   const std::vector<uintptr_t> key = {
-      (uintptr_t)iters,   (uintptr_t)L,       (uintptr_t)n,       (uintptr_t)bw,      (uintptr_t)nbm,
+      (uintptr_t)iters,   (uintptr_t)L,       (uintptr_t)n,       (uintptr_t)bw,      (uintptr_t)nbm, (uintptr_t)cg_fuse_on,
       (uintptr_t)x,       (uintptr_t)rowptr.p, (uintptr_t)colind.p, (uintptr_t)vals.p, (uintptr_t)r0.p,
       (uintptr_t)r1.p,    (uintptr_t)z.p,     (uintptr_t)p.p,     (uintptr_t)q.p,     (uintptr_t)binv.p,
       (uintptr_t)bfirst.p, (uintptr_t)bsize.p, (uintptr_t)part.p, (uintptr_t)scal.p,  (uintptr_t)nbcol.p};

@@ -57,13 +57,15 @@ void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, dou
 void launch_dot(hipStream_t s, int n, const double* x, const double* y, double* part, double* out);
 
 size_t cg_scal_bytes();
+size_t cg_part_size(int n);  // doubles of the partial-sum buffer of an n-equation PCG
 void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, double* z, double* p, int bw,
               const double* binv, const int* bfirst, const unsigned char* bsize, double* part, double tol2,
               int maxit, void* scal);
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
                    double* p, double* q, int bw, const double* binv, const int* bfirst,
-                   const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol);
+                   const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol,
+                   bool fuse);  // fuse: alpha / beta reductions in the update / new-direction kernels when small
 void cg_read_scal(hipStream_t s, const void* scal, int* converged, int* iter, double* rr, double* bb,
                   int* breakdown);
 

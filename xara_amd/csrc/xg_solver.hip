@@ -238,17 +238,20 @@ __global__ __launch_bounds__(256) void k_spmv_nb_dot(int n, const int* __restric
 // ---------------------------------------------------------------------------
 // PCG scalar state kept on the device
 // ---------------------------------------------------------------------------
+// r.z and the iteration count are kept per iteration parity: iteration k
+// reads entry k & 1 and writes entry (k + 1) & 1, so that the workgroups of
+// a fused kernel that all recompute the same reduction never read a field
+// that one of them rewrites.
 struct CGScal {
-  double rz, pq, rr, bb, alpha, beta, tol2;
-  int converged, iter, maxit, breakdown;
+  double rzp[2], pq, rr, bb, alpha, beta, tol2;
+  int converged, iterp[2], maxit, breakdown;
 };
 
-// Single-block reduction of `nb` partials (column c of a [nb][ncol] array).
-__global__ __launch_bounds__(256) void k_cg_alpha(int nb, const double* __restrict__ part,
-                                                  CGScal* __restrict__ s) {
-  __shared__ double sh[256];
-  if (s->converged) return;
-  // a thread's partials in ascending order, eight loads in flight at a time
+// Sum of `nb` partials by one workgroup of 256: a thread's partials in
+// ascending order, eight loads in flight at a time, then the tree sum.
+// Every caller sums in this order, so every workgroup that reduces the same
+// partials gets the same bits.
+__device__ __forceinline__ double sum_partials(int nb, const double* __restrict__ part, double* sh) {
   double v = 0.0;
   int i = threadIdx.x;
   for (; i + 7 * 256 < nb; i += 8 * 256) {
@@ -259,14 +262,44 @@ __global__ __launch_bounds__(256) void k_cg_alpha(int nb, const double* __restri
     for (int u = 0; u < 8; u++) v += a[u];
   }
   for (; i < nb; i += 256) v += part[i];
-  const double pq = block_sum<256>(v, sh);
+  return block_sum<256>(v, sh);
+}
+
+// Sums of the `nb` interleaved partial pairs (r.z, r.r) of the update.
+__device__ __forceinline__ void sum_partial_pairs(int nb, const double* __restrict__ part, double* sh, double& a,
+                                                  double& b) {
+  double v1 = 0.0, v2 = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    v1 += part[2 * i];
+    v2 += part[2 * i + 1];
+  }
+  a = block_sum<256>(v1, sh);
+  b = block_sum<256>(v2, sh);
+}
+
+// The converged flag as one value for the whole workgroup.  In a fused
+// kernel workgroup 0 may set the flag while another workgroup's waves are
+// still reading it; one read by thread 0 keeps its waves on one path (and
+// its block sums whole).
+__device__ __forceinline__ bool block_converged(const CGScal* s, int* flag) {
+  if (threadIdx.x == 0) *flag = s->converged;
+  __syncthreads();
+  return *flag != 0;
+}
+
+// alpha = r.z / p.q from the SpMV's partials of p.q (one workgroup).
+__global__ __launch_bounds__(256) void k_cg_alpha(int nb, const double* __restrict__ part,
+                                                  CGScal* __restrict__ s, int par) {
+  __shared__ double sh[256];
+  if (s->converged) return;
+  const double pq = sum_partials(nb, part, sh);
   if (threadIdx.x == 0) {
     s->pq = pq;
     if (pq == 0.0 || !isfinite(pq)) {
       s->breakdown = 1;
       s->converged = 1;
     } else {
-      s->alpha = s->rz / pq;
+      s->alpha = s->rzp[par] / pq;
     }
   }
 }
@@ -280,18 +313,13 @@ __global__ __launch_bounds__(256) void k_cg_alpha(int nb, const double* __restri
 // products summed in column order, so a small system waits for memory once
 // per row rather than once per block column.
 template <int BW>
-__global__ __launch_bounds__(256) void k_cg_update(int n, const CGScal* __restrict__ s,
-                                                   const double* __restrict__ p,
-                                                   const double* __restrict__ q, double* __restrict__ x,
-                                                   const double* __restrict__ r_in,
-                                                   double* __restrict__ r_out, double* __restrict__ z,
-                                                   const double* __restrict__ binv,
-                                                   const int* __restrict__ bfirst,
-                                                   const unsigned char* __restrict__ bsize,
-                                                   double* __restrict__ part) {
-  __shared__ double sh[256];
-  if (s->converged) return;
-  const double alpha = s->alpha;
+__device__ __forceinline__ void cg_update_rows(int n, double alpha, const double* __restrict__ p,
+                                               const double* __restrict__ q, double* __restrict__ x,
+                                               const double* __restrict__ r_in, double* __restrict__ r_out,
+                                               double* __restrict__ z, const double* __restrict__ binv,
+                                               const int* __restrict__ bfirst,
+                                               const unsigned char* __restrict__ bsize, double* __restrict__ part,
+                                               double* sh) {
   double rz = 0.0, rr = 0.0;
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     x[i] += alpha * p[i];
@@ -326,23 +354,69 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, const CGScal* __restri
   }
 }
 
-__global__ __launch_bounds__(256) void k_cg_beta(int nb, const double* __restrict__ part,
-                                                 CGScal* __restrict__ s) {
+template <int BW>
+__global__ __launch_bounds__(256) void k_cg_update(int n, const CGScal* __restrict__ s,
+                                                   const double* __restrict__ p,
+                                                   const double* __restrict__ q, double* __restrict__ x,
+                                                   const double* __restrict__ r_in,
+                                                   double* __restrict__ r_out, double* __restrict__ z,
+                                                   const double* __restrict__ binv,
+                                                   const int* __restrict__ bfirst,
+                                                   const unsigned char* __restrict__ bsize,
+                                                   double* __restrict__ part) {
   __shared__ double sh[256];
   if (s->converged) return;
-  double v1 = 0.0, v2 = 0.0;
-  for (int i = threadIdx.x; i < nb; i += 256) {
-    v1 += part[2 * i];
-    v2 += part[2 * i + 1];
+  cg_update_rows<BW>(n, s->alpha, p, q, x, r_in, r_out, z, binv, bfirst, bsize, part, sh);
+}
+
+// k_cg_alpha fused into the update, for systems whose SpMV leaves few
+// partials: every workgroup sums the nbs partials of p.q itself (same order,
+// same bits as k_cg_alpha) and workgroup 0 records pq and alpha.  The
+// update's partials go to part_out, apart from the SpMV's.
+template <int BW>
+__global__ __launch_bounds__(256) void k_cg_update_fused(int n, CGScal* __restrict__ s, int par, int nbs,
+                                                         const double* __restrict__ part_in,
+                                                         const double* __restrict__ p,
+                                                         const double* __restrict__ q, double* __restrict__ x,
+                                                         const double* __restrict__ r_in,
+                                                         double* __restrict__ r_out, double* __restrict__ z,
+                                                         const double* __restrict__ binv,
+                                                         const int* __restrict__ bfirst,
+                                                         const unsigned char* __restrict__ bsize,
+                                                         double* __restrict__ part_out) {
+  __shared__ double sh[256];
+  __shared__ int cflag;
+  if (block_converged(s, &cflag)) return;
+  const double pq = sum_partials(nbs, part_in, sh);
+  const bool bad = pq == 0.0 || !isfinite(pq);
+  const double alpha = bad ? 0.0 : s->rzp[par] / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s->pq = pq;
+    if (bad) {
+      s->breakdown = 1;
+      s->converged = 1;
+    } else {
+      s->alpha = alpha;
+    }
   }
-  const double rz = block_sum<256>(v1, sh);
-  const double rr = block_sum<256>(v2, sh);
+  if (bad) return;
+  cg_update_rows<BW>(n, alpha, p, q, x, r_in, r_out, z, binv, bfirst, bsize, part_out, sh);
+}
+
+// beta = r.z (new) / r.z (old) and the convergence test (one workgroup).
+__global__ __launch_bounds__(256) void k_cg_beta(int nb, const double* __restrict__ part,
+                                                 CGScal* __restrict__ s, int par) {
+  __shared__ double sh[256];
+  if (s->converged) return;
+  double rz, rr;
+  sum_partial_pairs(nb, part, sh, rz, rr);
   if (threadIdx.x == 0) {
-    s->iter += 1;
+    const int iter = s->iterp[par] + 1;
+    s->iterp[par ^ 1] = iter;
     s->rr = rr;
-    s->beta = rz / s->rz;
-    s->rz = rz;
-    if (rr <= s->tol2 * s->bb || s->iter >= s->maxit) s->converged = 1;
+    s->beta = rz / s->rzp[par];
+    s->rzp[par ^ 1] = rz;
+    if (rr <= s->tol2 * s->bb || iter >= s->maxit) s->converged = 1;
   }
 }
 
@@ -351,6 +425,32 @@ __global__ void k_cg_newp(int n, const CGScal* __restrict__ s, const double* __r
   if (s->converged) return;
   const double beta = s->beta;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
+}
+
+// k_cg_beta fused into the new direction: every workgroup sums the update's
+// partials itself (same order, same bits as k_cg_beta), applies the same
+// convergence test and, unless converged, forms its rows of p; workgroup 0
+// records the scalars.
+__global__ __launch_bounds__(256) void k_cg_newp_fused(int n, CGScal* __restrict__ s, int par, int nbv,
+                                                       const double* __restrict__ part,
+                                                       const double* __restrict__ z, double* __restrict__ p) {
+  __shared__ double sh[256];
+  __shared__ int cflag;
+  if (block_converged(s, &cflag)) return;
+  double rz, rr;
+  sum_partial_pairs(nbv, part, sh, rz, rr);
+  const int iter = s->iterp[par] + 1;
+  const double beta = rz / s->rzp[par];
+  const bool conv = rr <= s->tol2 * s->bb || iter >= s->maxit;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s->iterp[par ^ 1] = iter;
+    s->rr = rr;
+    s->beta = beta;
+    s->rzp[par ^ 1] = rz;
+    if (conv) s->converged = 1;
+  }
+  if (conv) return;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = z[i] + beta * p[i];
 }
 
 // Start: x = 0, r = b, z = M^{-1} b, p = z; partials of r.z, b.b
@@ -398,12 +498,13 @@ __global__ __launch_bounds__(256) void k_cg_init_scal(int nb, const double* __re
   const double rz = block_sum<256>(v1, sh);
   const double bb = block_sum<256>(v2, sh);
   if (threadIdx.x == 0) {
-    s->rz = rz;
+    s->rzp[0] = rz;
+    s->rzp[1] = 0.0;
     s->bb = bb;
     s->rr = bb;
     s->tol2 = tol2;
     s->maxit = maxit;
-    s->iter = 0;
+    s->iterp[0] = s->iterp[1] = 0;
     s->breakdown = 0;
     s->alpha = s->beta = s->pq = 0.0;
     s->converged = (bb == 0.0) ? 1 : 0;
@@ -589,6 +690,13 @@ constexpr int kCgMaxBlocks = 2048;
 constexpr int kSpmvMaxBlocks = 8192;
 int spmv_blocks(int n, int lanes) { return std::min(ceil_div(n, 256 / lanes), kSpmvMaxBlocks); }
 static int vec_blocks(int n) { return std::min(ceil_div(n, 256), kCgMaxBlocks); }
+// the fused alpha / beta kernels: at most this many partials per reduction;
+// the update's partials then start at part + kCgFuseMax
+constexpr int kCgFuseMax = 1024;
+size_t cg_part_size(int n) {
+  n = std::max(n, 1);
+  return 2 * (size_t)std::max({ceil_div(n, 256), spmv_blocks(n, 64), kCgFuseMax}) + kCgFuseMax + 16;
+}
 template <int M>
 static void launch_spmv_nb(hipStream_t s, int lanes, int n, const int* rowptr, const int* nbcol,
                            const double* vals, const double* x, double* y, double* partial, const int* stop) {
@@ -673,30 +781,44 @@ void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, doubl
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
                    double* p, double* q, int bw, const double* binv, const int* bfirst,
-                   const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol) {
+                   const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol, bool fuse) {
   CGScal* S = (CGScal*)scal;
   const int nbs = spmv_blocks(n, lanes);
   const int nbv = vec_blocks(n);
   const int* stop = &S->converged;
+  fuse = fuse && nbs <= kCgFuseMax && nbv <= kCgFuseMax;
+  double* part2 = part + kCgFuseMax;
+  if (bw < 1 || bw > 6) throw Error(XG_ERR_ARG, "PCG block width must be 1..6");
   for (int it = 0; it < iters; it++) {
+    const int par = (it0 + it) & 1;
+    double* ri = par ? r1 : r0;
+    double* ro = par ? r0 : r1;
     launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop, nbm, nbcol);
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, nbs, part, S);
-    const bool even = ((it0 + it) & 1) == 0;
-    double* ri = even ? r0 : r1;
-    double* ro = even ? r1 : r0;
-    switch (bw) {
+    if (fuse) {
+      switch (bw) {
+#define XG_CGU(W)                                                                                                \
+  case W:                                                                                                       \
+    hipLaunchKernelGGL(k_cg_update_fused<W>, dim3(nbv), dim3(256), 0, s, n, S, par, nbs, part, p, q, x, ri, ro, z, \
+                       binv, bfirst, bsize, part2);                                                             \
+    break;
+        XG_CGU(1) XG_CGU(2) XG_CGU(3) XG_CGU(4) XG_CGU(5) XG_CGU(6)
+#undef XG_CGU
+      }
+      hipLaunchKernelGGL(k_cg_newp_fused, dim3(nbv), dim3(256), 0, s, n, S, par, nbv, part2, z, p);
+    } else {
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, nbs, part, S, par);
+      switch (bw) {
 #define XG_CGU(W)                                                                                              \
   case W:                                                                                                     \
     hipLaunchKernelGGL(k_cg_update<W>, dim3(nbv), dim3(256), 0, s, n, S, p, q, x, ri, ro, z, binv, bfirst, bsize, \
                        part);                                                                                 \
     break;
-      XG_CGU(1) XG_CGU(2) XG_CGU(3) XG_CGU(4) XG_CGU(5) XG_CGU(6)
+        XG_CGU(1) XG_CGU(2) XG_CGU(3) XG_CGU(4) XG_CGU(5) XG_CGU(6)
 #undef XG_CGU
-      default:
-        throw Error(XG_ERR_ARG, "PCG block width must be 1..6");
+      }
+      hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, nbv, part, S, par);
+      hipLaunchKernelGGL(k_cg_newp, dim3(nbv), dim3(256), 0, s, n, S, z, p);
     }
-    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, nbv, part, S);
-    hipLaunchKernelGGL(k_cg_newp, dim3(nbv), dim3(256), 0, s, n, S, z, p);
   }
 }
 
@@ -706,7 +828,7 @@ void cg_read_scal(hipStream_t s, const void* scal, int* converged, int* iter, do
   XG_HIP(hipMemcpyAsync(&h, scal, sizeof(CGScal), hipMemcpyDeviceToHost, s));
   XG_HIP(hipStreamSynchronize(s));
   *converged = h.converged;
-  *iter = h.iter;
+  *iter = std::max(h.iterp[0], h.iterp[1]);
   *rr = h.rr;
   *bb = h.bb;
   *breakdown = h.breakdown;

@@ -269,11 +269,12 @@ class Model:
         from the mean row length)."""
         self._check(self._b.spmv_config(self._h, {"auto": 0, "csr": 1}[fmt], int(lanes)))
 
-    def cg_config(self, chunk=16, graph=True):
+    def cg_config(self, chunk=16, graph=True, fuse=True):
         """PCG issue pattern: iterations issued between convergence reads
-        (even) and whether a full chunk is replayed from a captured graph;
-        the iterates do not depend on either."""
-        self._check(self._b.cg_config(self._h, int(chunk), int(bool(graph))))
+        (even), whether a full chunk is replayed from a captured graph, and
+        whether small systems run the alpha / beta reductions inside the
+        vector kernels; the iterates do not depend on any of them."""
+        self._check(self._b.cg_config(self._h, int(chunk), int(bool(graph)) | 2 * int(bool(fuse))))
 
     def numberer(self, kind):
         self._check(self._b.numberer(self._h, NUMBERERS[kind.lower()]))
