@@ -91,7 +91,7 @@ def main():
     ap.add_argument("--frame-steps", type=int, default=0, help="DisplacementControl steps in frame mode")
     ap.add_argument("--newton-steps", type=int, default=40)
     ap.add_argument("--system", default="GpuBlockPCG")
-    ap.add_argument("--spmv", choices=["auto", "csr"], default="auto", help="SpMV layout of the PCG")
+    ap.add_argument("--spmv", choices=["auto", "csr", "sym"], default="auto", help="SpMV layout of the PCG")
     ap.add_argument("--lanes", type=int, default=0, help="SpMV threads per row (0 = from the row length)")
     ap.add_argument("--cg-graph", type=int, choices=[0, 1], default=1,
                     help="replay PCG chunks from captured graphs (0 = launch each kernel)")

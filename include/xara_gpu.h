@@ -155,9 +155,17 @@ int  xg_system(xg_domain* d, int kind, double rtol, int maxit);
    XG_SPMV_AUTO uses node-blocked column indices (one index per neighbour
    node instead of one per nonzero, same values and pattern) when every node
    has all its DOFs free or none, and plain CSR otherwise; lanes = threads per
-   matrix row (4, 8, 16, 32 or 64; 0 = chosen from the mean row length).     */
+   matrix row (4, 8, 16, 32 or 64; 0 = chosen from the mean row length).
+   XG_SPMV_SYM (2 or 3 DOFs per node, node-blocked pattern): the products
+   read a block-symmetric copy of the matrix (upper node blocks only,
+   refreshed after each assembly), for symmetric tangents; lanes = threads
+   per node block row.  Under XG_SPMV_AUTO the PCG's own products use it too
+   (the PCG assumes a symmetric matrix) while xg_spmv keeps the node-blocked
+   CSR product.  Falls back to the node-blocked or plain CSR product where
+   the layout does not apply.  xg_time_op(XG_OP_SPMV) times the PCG's.      */
 #define XG_SPMV_AUTO 0
 #define XG_SPMV_CSR  1
+#define XG_SPMV_SYM  2
 int  xg_spmv_config(xg_domain* d, int format, int lanes);
 /* PCG issue pattern.  No counterpart in the reference: a tuning knob.  The
    host issues `chunk` iterations (even, 2..1024; default 16) between reads

@@ -85,10 +85,11 @@ def test_brick_assembly_parity():
     assert relerr(g.element_resisting_force(tag), o.element_resisting_force(tag)) <= RTOL
 
 
-@pytest.mark.parametrize("system", ["GpuPCG", "GpuBlockPCG"])
-def test_pcg_solve_matches_direct(system):
-    print(f"case: pcg {system}", flush=True)
+@pytest.mark.parametrize("system,fmt", [("GpuPCG", "auto"), ("GpuBlockPCG", "auto"), ("GpuBlockPCG", "sym")])
+def test_pcg_solve_matches_direct(system, fmt):
+    print(f"case: pcg {system} {fmt}", flush=True)
     mesh, (g, o) = cube_pair(5, system=system)
+    g.spmv_config(fmt)
     for m in (g, o):
         m.form_tangent(values=False)
     rng = np.random.default_rng(3)
@@ -98,12 +99,14 @@ def test_pcg_solve_matches_direct(system):
     assert relerr(g.spmv(xg), b) <= 1e-11
 
 
-def test_brick_pushover_trajectory():
+@pytest.mark.parametrize("fmt", ["auto", "sym"])
+def test_brick_pushover_trajectory(fmt):
     """Reduced cube under top pressure, LoadControl + Newton: converged
     displacement and plastic state after every step."""
-    print("case: brick pushover n=4", flush=True)
+    print(f"case: brick pushover n=4 spmv {fmt}", flush=True)
     n = 4
     mesh, (g, o) = cube_pair(n)
+    g.spmv_config(fmt)
     npts = 8 * n ** 3
     for m in (g, o):
         m.test("NormUnbalance", 1e-9, 30)
@@ -178,6 +181,30 @@ def test_node_blocked_spmv_matches_csr(lanes):
         y_ref = np.zeros(m.neq)
         np.add.at(y_ref, rows, K * x[ci])
         assert relerr(y_nb, y_ref) <= 1e-13
+        m.close()
+
+
+@pytest.mark.parametrize("lanes", [0, 8, 64])
+def test_symmetric_spmv_matches_csr(lanes):
+    """The block-symmetric copy (upper node blocks, lower ones read
+    transposed) gives K x to rounding, and follows a new assembly; where it
+    does not apply (6 DOFs per node) the node-blocked product is used."""
+    for name, m in _spmv_models():
+        print(f"case: sym spmv {name} lanes {lanes}", flush=True)
+        m.setup()
+        rng = np.random.default_rng(9)
+        m.spmv_config("sym", lanes)
+        x = rng.normal(size=m.neq)
+        rp, ci = m.csr_pattern()
+        rows = np.repeat(np.arange(m.neq), np.diff(rp))
+        for step in range(2):
+            m.set_trial_disp((step + 1) * 1e-4 * rng.normal(size=(m.num_nodes, m.ndf)))
+            m.update()
+            K = m.form_tangent()
+            y_ref = np.zeros(m.neq)
+            np.add.at(y_ref, rows, K * x[ci])
+            y = m.spmv(x)
+            assert relerr(y, y_ref) <= 1e-13, (name, step)
         m.close()
 
 

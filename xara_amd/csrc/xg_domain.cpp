@@ -11,6 +11,7 @@
 #include "xg_frame.h"
 #include "xg_j2.h"
 #include "xg_kernels.h"
+#include "xg_sym_layout.h"
 
 namespace xg {
 
@@ -266,6 +267,61 @@ std::vector<int> Domain::node_block_columns() {
   return out;
 }
 
+// The block-symmetric copy of the matrix for the SpMV (XG_SPMV_SYM, and the
+// PCG's products under XG_SPMV_AUTO; M = 2 or 3): the PCG assumes a
+// symmetric matrix, so its products may read the upper blocks only.  Layout
+// built on first use (sym_layout), stored blocks converted
+// from the CSR values whenever an assembly has changed them.  sym_ok stays
+// false where the block pattern is not symmetric, and the SpMV keeps the
+// CSR values.
+const SymSpmv* Domain::sym_prepare(bool solver) {
+  const bool want = spmv_format == XG_SPMV_SYM || (solver && spmv_format == XG_SPMV_AUTO);
+  if (!want || (nb_m != 2 && nb_m != 3) || num.neq == 0) return nullptr;
+  if (!sym_built) {
+    sym_built = true;
+    sym_ok = false;
+    const int M = nb_m, nbr = num.neq / M;
+    const std::vector<int> nbc = node_block_columns();
+    SymLayout L;
+    if (nb_m != M || nbc.empty() || !sym_layout(M, nbr, num.rowptr, nbc, L)) return nullptr;
+    hipStream_t s = stream;
+    sym_ptr.upload(L.ptr, s);
+    sym_col.upload(L.col, s);
+    sym_blk.upload(L.blk, s);
+    sym_src.upload(L.src, s);
+    sym_len.upload(L.len, s);
+    sym_val.resize((size_t)L.nup * M * M);
+    XG_HIP(hipStreamSynchronize(s));
+    sym_nup = L.nup;
+    sym_view.m = M;
+    sym_view.nbr = nbr;
+    sym_view.ptr = sym_ptr.p;
+    sym_view.col = sym_col.p;
+    sym_view.blk = sym_blk.p;
+    sym_view.val = sym_val.p;
+    sym_ok = true;
+    sym_stale = true;
+  }
+  if (!sym_ok) return nullptr;
+  if (spmv_lanes > 0) {
+    sym_view.lanes = spmv_lanes;
+  } else {
+    // two passes over a mean block row (measured best on hexahedral meshes)
+    const double half = 0.5 * (double)sym_col.n / std::max(sym_view.nbr, 1);
+    sym_view.lanes = half > 32 ? 64 : half > 16 ? 32 : half > 8 ? 16 : half > 4 ? 8 : 4;
+  }
+  if (sym_stale) {
+    launch_sym_convert(stream, sym_view, sym_nup, sym_src.p, sym_len.p, vals.p, sym_val.p);
+    sym_stale = false;
+  }
+  return &sym_view;
+}
+
+void Domain::spmv(const double* x, double* y, bool solver) {
+  launch_spmv(stream, lanes(), num.neq, rowptr.p, colind.p, vals.p, x, y, nullptr, nullptr, spmv_nbm(), nbcol.p,
+              sym_prepare(solver));
+}
+
 void Domain::setup() {
   XG_HIP(hipSetDevice(device));
   if (nodedefs.empty()) throw Error(XG_ERR_STATE, "no nodes defined");
@@ -459,6 +515,11 @@ void Domain::setup() {
   rowptr.upload(num.rowptr, s);
   colind.upload(num.colind, s);
   nbcol.upload(nbc, s);
+  sym_built = sym_ok = false;
+  sym_stale = true;
+  sym_view = SymSpmv{};
+  for (auto* v : {&sym_ptr, &sym_col, &sym_blk, &sym_src, &sym_len}) v->release();
+  sym_val.release();
   vals.resize(nnz);
   vals.zero(s);
   Ke.resize(ke_total);
@@ -572,6 +633,7 @@ void Domain::form_unbalance() {
 void Domain::form_tangent() {
   for (auto& g : groups) g->tangent(*this, Ke.p + g->ke_off);
   launch_assemble_csr(stream, (long long)num.colind.size(), asm_ptr.p, asm_idx.p, Ke.p, vals.p);
+  sym_stale = true;
   // Newmark::formNodTangent: c2 C + c3 M of the nodal masses (C = alphaM M)
   if (integ == 3) launch_add_diag(stream, nmass_eq, mass_pos.p, mass_val.p, nm_c3 + nm_c2 * alphaM, vals.p);
 }
@@ -598,9 +660,10 @@ void Domain::cg_graph_clear() {
 // kCgGraphs; further sets are launched one by one).
 void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
   const int n = num.neq, nbm = spmv_nbm();
+  const SymSpmv* sym = sym_prepare(true);
   auto eager = [&](hipStream_t s, int k, int i0) {
     cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, q.p, bw, binv.p, bfirst.p,
-                  bsize.p, part.p, scal.p, nbm, nbcol.p, cg_fuse_on);
+                  bsize.p, part.p, scal.p, nbm, nbcol.p, cg_fuse_on, sym);
   };
   if (!cg_graph_on || iters != cg_chunk || (it0 & 1) || (cg_chunk & 1)) {
     eager(stream, iters, it0);
@@ -610,7 +673,8 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
       (uintptr_t)iters,   (uintptr_t)L,       (uintptr_t)n,       (uintptr_t)bw,      (uintptr_t)nbm, (uintptr_t)cg_fuse_on,
       (uintptr_t)x,       (uintptr_t)rowptr.p, (uintptr_t)colind.p, (uintptr_t)vals.p, (uintptr_t)r0.p,
       (uintptr_t)r1.p,    (uintptr_t)z.p,     (uintptr_t)p.p,     (uintptr_t)q.p,     (uintptr_t)binv.p,
-      (uintptr_t)bfirst.p, (uintptr_t)bsize.p, (uintptr_t)part.p, (uintptr_t)scal.p,  (uintptr_t)nbcol.p};
+      (uintptr_t)bfirst.p, (uintptr_t)bsize.p, (uintptr_t)part.p, (uintptr_t)scal.p,  (uintptr_t)nbcol.p,
+      (uintptr_t)(sym ? sym->val : nullptr), (uintptr_t)(sym ? sym->lanes : 0)};
   for (auto& g : cg_graphs)
     if (g.key == key) {
       XG_HIP(hipGraphLaunch(g.exec, stream));

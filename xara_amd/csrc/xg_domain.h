@@ -8,6 +8,7 @@
 
 #include "../../include/xara_gpu.h"
 #include "xg_common.h"
+#include "xg_kernels.h"
 #include "xg_memory.h"
 #include "xg_numbering.h"
 
@@ -104,6 +105,13 @@ struct Domain {
   // every node has all ndf DOFs free or none: nb_m = ndf, else 0
   DVec<int> nbcol;
   int nb_m = 0;
+  // block-symmetric copy for the SpMV (XG_SPMV_SYM), built on first use and
+  // refreshed from the CSR values after each assembly
+  DVec<int> sym_ptr, sym_col, sym_blk, sym_src, sym_len;
+  DVec<double> sym_val;
+  int sym_nup = 0;
+  bool sym_built = false, sym_ok = false, sym_stale = true;
+  SymSpmv sym_view;
   DVec<double> vals, Ke, Re, B, X, Peq;
   DVec<int> asm_ptr, asm_idx, rhs_ptr, rhs_idx;
   DVec<int> nfail, pcbad;
@@ -187,6 +195,13 @@ struct Domain {
   int lanes();
   std::vector<int> node_block_columns();
   int spmv_nbm() const { return spmv_format == XG_SPMV_CSR ? 0 : nb_m; }
+  // the symmetric view to pass to the SpMV launches (nullptr when another
+  // layout is in use); converts stale values first.  solver: for the PCG,
+  // which assumes a symmetric matrix and takes it under XG_SPMV_AUTO too
+  const SymSpmv* sym_prepare(bool solver);
+  // y = K x (device pointers) in the configured layout (solver: as the PCG
+  // forms it)
+  void spmv(const double* x, double* y, bool solver = false);
 };
 
 }  // namespace xg

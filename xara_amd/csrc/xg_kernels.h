@@ -42,11 +42,28 @@ void launch_newmark_predict(hipStream_t s, long long ndofs, const int* eqmap, co
 void launch_newmark_correct(hipStream_t s, long long ndofs, const int* eqmap, const double* dU, double c2, double c3,
                             double* U, double* V, double* A);
 int spmv_blocks(int n, int lanes);
+// Block-symmetric copy of a node-blocked matrix (k_spmv_sym_dot): the upper
+// m x m blocks only; block row I lists its neighbour blocks J ascending with
+// the stored block holding each, (I, J) for J >= I and (J, I) for J < I.
+struct SymSpmv {
+  int m = 0;      // block size, 2 or 3; 0 = not in use
+  int nbr = 0;    // block rows
+  int lanes = 32; // lanes per block row
+  const int* ptr = nullptr;     // nbr + 1
+  const int* col = nullptr;     // block column of each entry
+  const int* blk = nullptr;     // stored block of each entry
+  const double* val = nullptr;  // stored blocks, m*m row-major each
+};
+// workgroups (= partial sums) of one SpMV launch
+int spmv_grid(int n, int lanes, const SymSpmv* sym);
 // nbm = 2, 3 or 6 with nbcol: node-blocked column indices (k_spmv_nb_dot);
-// nbm = 0: plain CSR
+// nbm = 0: plain CSR; sym (when sym->m != 0) takes precedence over both
 void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
                  const double* x, double* y, double* partial, const int* stop, int nbm = 0,
-                 const int* nbcol = nullptr);
+                 const int* nbcol = nullptr, const SymSpmv* sym = nullptr);
+// stored block k <- CSR values vals[src[k] + r*len[k] + c]
+void launch_sym_convert(hipStream_t s, const SymSpmv& S, int nup, const int* src, const int* len,
+                        const double* vals, double* sval);
 void launch_block_inverse(hipStream_t s, int nblk, const int* first, const unsigned char* size,
                           const int* rowptr, const int* colind, const double* vals, int bw, double* binv,
                           int* bad);
@@ -65,7 +82,8 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
                    double* p, double* q, int bw, const double* binv, const int* bfirst,
                    const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol,
-                   bool fuse);  // fuse: alpha / beta reductions in the update / new-direction kernels when small
+                   bool fuse,  // fuse: alpha / beta reductions in the update / new-direction kernels when small
+                   const SymSpmv* sym = nullptr);
 void cg_read_scal(hipStream_t s, const void* scal, int* converged, int* iter, double* rr, double* bb,
                   int* breakdown);
 

@@ -236,6 +236,90 @@ __global__ __launch_bounds__(256) void k_spmv_nb_dot(int n, const int* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Block-symmetric SpMV.  The consistent tangents of this path are symmetric,
+// so a node-blocked matrix is kept a second time as its upper M x M blocks
+// only (block-major, row-major inside a block, copied from the CSR values
+// after each assembly): about half the bytes per product.  Block row I lists
+// its neighbours J ascending, each with the stored block that holds it:
+// (I, J) itself for J >= I, (J, I) for J < I, used transposed.  A stored
+// block is read twice per product, by its own row and by its partner's, a
+// few thousand block rows apart in an ascending sweep, so the second read
+// finds it in the memory-side cache.  G lanes per block row, one neighbour
+// block per lane and pass (its M*M values, column and M entries of x
+// loaded together), the M row sums reduced across the G lanes.
+// ---------------------------------------------------------------------------
+template <int M, int G>
+__global__ __launch_bounds__(256) void k_spmv_sym_dot(int nbr, const int* __restrict__ ptr,
+                                                      const int* __restrict__ col, const int* __restrict__ blk,
+                                                      const double* __restrict__ sv, const double* __restrict__ x,
+                                                      double* __restrict__ y, double* __restrict__ partial,
+                                                      const int* __restrict__ stop) {
+  __shared__ double sh[256];
+  if (stop && stop[0]) return;
+  const int t = threadIdx.x;
+  const int lane = t % G;
+  constexpr int R = 256 / G;  // block rows per workgroup and sweep
+  double d = 0.0;
+  for (int I0 = blockIdx.x * R; I0 < nbr; I0 += gridDim.x * R) {
+    const int I = I0 + t / G;
+    double acc[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) acc[r] = 0.0;
+    if (I < nbr) {
+      const int b = ptr[I], e = ptr[I + 1];
+      for (int k = b + lane; k < e; k += G) {
+        const int J = col[k];
+        const double* a = sv + (long long)blk[k] * (M * M);
+        double v[M * M], xj[M];
+#pragma unroll
+        for (int q = 0; q < M * M; q++) v[q] = a[q];
+#pragma unroll
+        for (int c = 0; c < M; c++) xj[c] = x[M * J + c];
+        const bool lower = J < I;
+#pragma unroll
+        for (int r = 0; r < M; r++)
+#pragma unroll
+          for (int c = 0; c < M; c++) acc[r] += (lower ? v[c * M + r] : v[r * M + c]) * xj[c];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < M; r++)
+#pragma unroll
+      for (int off = G / 2; off > 0; off >>= 1) acc[r] += __shfl_down(acc[r], off, G);
+    if (I < nbr && lane == 0) {
+#pragma unroll
+      for (int r = 0; r < M; r++) {
+        y[M * I + r] = acc[r];
+        d += acc[r] * x[M * I + r];
+      }
+    }
+  }
+  if (partial) {
+    const double bs = block_sum<256>(d, sh);
+    if (t == 0) partial[blockIdx.x] = bs;
+  }
+}
+
+// Stored block k = rows M*I.. of the CSR values from entry src[k] of row M*I
+// (the block's first column), len[k] entries per row of block row I.
+template <int M>
+__global__ __launch_bounds__(256) void k_sym_convert(int nup, const int* __restrict__ src,
+                                                     const int* __restrict__ len, const double* __restrict__ vals,
+                                                     double* __restrict__ sv) {
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < nup; k += gridDim.x * 256) {
+    const long long s0 = src[k];
+    const int L = len[k];
+    double v[M * M];
+#pragma unroll
+    for (int r = 0; r < M; r++)
+#pragma unroll
+      for (int c = 0; c < M; c++) v[r * M + c] = vals[s0 + (long long)r * L + c];
+#pragma unroll
+    for (int q = 0; q < M * M; q++) sv[(long long)k * (M * M) + q] = v[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // PCG scalar state kept on the device
 // ---------------------------------------------------------------------------
 // r.z and the iteration count are kept per iteration parity: iteration k
@@ -716,9 +800,45 @@ static void launch_spmv_nb(hipStream_t s, int lanes, int n, const int* rowptr, c
   }
 }
 
+template <int M>
+static void launch_spmv_sym(hipStream_t s, const SymSpmv& S, const double* x, double* y, double* partial,
+                            const int* stop) {
+  const dim3 g(spmv_grid(S.nbr * M, S.lanes, &S)), b(256);
+  switch (S.lanes) {
+#define XG_SYM(G)                                                                                         \
+  case G:                                                                                                \
+    hipLaunchKernelGGL((k_spmv_sym_dot<M, G>), g, b, 0, s, S.nbr, S.ptr, S.col, S.blk, S.val, x, y, partial, \
+                       stop);                                                                            \
+    break;
+    XG_SYM(4) XG_SYM(8) XG_SYM(16) XG_SYM(32) XG_SYM(64)
+#undef XG_SYM
+    default: throw Error(XG_ERR_ARG, "symmetric SpMV lanes must be 4, 8, 16, 32 or 64");
+  }
+}
+
+int spmv_grid(int n, int lanes, const SymSpmv* sym) {
+  if (sym && sym->m) return std::min(ceil_div(std::max(sym->nbr, 1), 256 / lanes), kSpmvMaxBlocks);
+  return spmv_blocks(n, lanes);
+}
+
+void launch_sym_convert(hipStream_t s, const SymSpmv& S, int nup, const int* src, const int* len,
+                        const double* vals, double* sval) {
+  if (nup <= 0) return;
+  const dim3 g(std::min(ceil_div(nup, 256), 65536)), b(256);
+  if (S.m == 2) hipLaunchKernelGGL(k_sym_convert<2>, g, b, 0, s, nup, src, len, vals, sval);
+  else if (S.m == 3) hipLaunchKernelGGL(k_sym_convert<3>, g, b, 0, s, nup, src, len, vals, sval);
+  else throw Error(XG_ERR_ARG, "symmetric SpMV block size must be 2 or 3");
+}
+
 void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
-                 const double* x, double* y, double* partial, const int* stop, int nbm, const int* nbcol) {
+                 const double* x, double* y, double* partial, const int* stop, int nbm, const int* nbcol,
+                 const SymSpmv* sym) {
   if (n <= 0) return;
+  if (sym && sym->m) {
+    if (sym->m == 2) return launch_spmv_sym<2>(s, *sym, x, y, partial, stop);
+    if (sym->m == 3) return launch_spmv_sym<3>(s, *sym, x, y, partial, stop);
+    throw Error(XG_ERR_ARG, "symmetric SpMV block size must be 2 or 3");
+  }
   switch (nbm) {
     case 2: return launch_spmv_nb<2>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop);
     case 3: return launch_spmv_nb<3>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop);
@@ -781,9 +901,11 @@ void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, doubl
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
                    double* p, double* q, int bw, const double* binv, const int* bfirst,
-                   const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol, bool fuse) {
+                   const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol, bool fuse,
+                   const SymSpmv* sym) {
   CGScal* S = (CGScal*)scal;
-  const int nbs = spmv_blocks(n, lanes);
+  const int lanes_mv = sym && sym->m ? sym->lanes : lanes;
+  const int nbs = spmv_grid(n, lanes_mv, sym);
   const int nbv = vec_blocks(n);
   const int* stop = &S->converged;
   fuse = fuse && nbs <= kCgFuseMax && nbv <= kCgFuseMax;
@@ -793,7 +915,7 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
     const int par = (it0 + it) & 1;
     double* ri = par ? r1 : r0;
     double* ro = par ? r0 : r1;
-    launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop, nbm, nbcol);
+    launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop, nbm, nbcol, sym);
     if (fuse) {
       switch (bw) {
 #define XG_CGU(W)                                                                                                \

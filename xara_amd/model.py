@@ -265,9 +265,10 @@ class Model:
 
     def spmv_config(self, fmt="auto", lanes=0):
         """SpMV layout of the PCG: "auto" (node-blocked column indices when
-        every node has all DOFs free or none) or "csr"; lanes per row (0 =
-        from the mean row length)."""
-        self._check(self._b.spmv_config(self._h, {"auto": 0, "csr": 1}[fmt], int(lanes)))
+        every node has all DOFs free or none), "csr", or "sym" (upper node
+        blocks of a symmetric matrix, 2 or 3 DOFs per node); lanes per row
+        or block row (0 = from the mean length)."""
+        self._check(self._b.spmv_config(self._h, {"auto": 0, "csr": 1, "sym": 2}[fmt], int(lanes)))
 
     def cg_config(self, chunk=16, graph=True, fuse=True):
         """PCG issue pattern: iterations issued between convergence reads
