@@ -332,12 +332,19 @@ struct CGScal {
 };
 
 // Sum of `nb` partials by one workgroup of 256: a thread's partials in
-// ascending order, eight loads in flight at a time, then the tree sum.
-// Every caller sums in this order, so every workgroup that reduces the same
-// partials gets the same bits.
+// ascending order (up to 32 loads in flight, so 8,192 partials cost one
+// memory round trip), then the tree sum.  Every caller sums in this order,
+// so every workgroup that reduces the same partials gets the same bits.
 __device__ __forceinline__ double sum_partials(int nb, const double* __restrict__ part, double* sh) {
   double v = 0.0;
   int i = threadIdx.x;
+  for (; i + 31 * 256 < nb; i += 32 * 256) {
+    double a[32];
+#pragma unroll
+    for (int u = 0; u < 32; u++) a[u] = part[i + u * 256];
+#pragma unroll
+    for (int u = 0; u < 32; u++) v += a[u];
+  }
   for (; i + 7 * 256 < nb; i += 8 * 256) {
     double a[8];
 #pragma unroll
@@ -349,11 +356,26 @@ __device__ __forceinline__ double sum_partials(int nb, const double* __restrict_
   return block_sum<256>(v, sh);
 }
 
-// Sums of the `nb` interleaved partial pairs (r.z, r.r) of the update.
+// Sums of the `nb` interleaved partial pairs (r.z, r.r) of the update, each
+// thread's pairs in ascending order with eight pairs' loads in flight.
 __device__ __forceinline__ void sum_partial_pairs(int nb, const double* __restrict__ part, double* sh, double& a,
                                                   double& b) {
   double v1 = 0.0, v2 = 0.0;
-  for (int i = threadIdx.x; i < nb; i += 256) {
+  int i = threadIdx.x;
+  for (; i + 7 * 256 < nb; i += 8 * 256) {
+    double c[8], d[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      c[u] = part[2 * (i + u * 256)];
+      d[u] = part[2 * (i + u * 256) + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      v1 += c[u];
+      v2 += d[u];
+    }
+  }
+  for (; i < nb; i += 256) {
     v1 += part[2 * i];
     v2 += part[2 * i + 1];
   }
