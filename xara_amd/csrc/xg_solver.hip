@@ -244,24 +244,44 @@ __global__ __launch_bounds__(256) void k_spmv_nb_dot(int n, const int* __restric
 // (I, J) itself for J >= I, (J, I) for J < I, used transposed.  A stored
 // block is read twice per product, by its own row and by its partner's, a
 // few thousand block rows apart in an ascending sweep, so the second read
-// finds it in the memory-side cache.  G lanes per block row, one neighbour
-// block per lane and pass (its M*M values, column and M entries of x
-// loaded together), the M row sums reduced across the G lanes.
+// finds it in a cache.  Units of 256/G block rows are dealt out to the XCDs
+// in contiguous eighths (the hardware dispatches workgroup b to XCD b % 8)
+// and each workgroup takes a contiguous run of units, so that a block row's
+// lower partners, read shortly before by the same XCD, are more often found
+// in its L2 (4% faster at 64^3 than a plain grid-stride sweep).  G lanes per
+// block row, one neighbour block per lane and pass (its M*M values, column
+// and M entries of x loaded together), the M row sums reduced across the G
+// lanes.  (Staging a row's blocks in LDS with consecutive lanes reading
+// consecutive doubles measured 1.7x slower: the barrier between the copy and
+// the products and the LDS footprint cost more than the scattered loads.)
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void xcd_units(int units, int& ua, int& ub) {
+  const int nx = min(8, (int)gridDim.x);
+  const int xcd = blockIdx.x % nx, l = blockIdx.x / nx;
+  const int gx = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups of this XCD
+  const int u0 = (int)((long long)units * xcd / nx), u1 = (int)((long long)units * (xcd + 1) / nx);
+  const int cnt = (u1 - u0 + gx - 1) / gx;
+  ua = min(u1, u0 + l * cnt);
+  ub = min(u1, ua + cnt);
+}
+
 template <int M, int G>
 __global__ __launch_bounds__(256) void k_spmv_sym_dot(int nbr, const int* __restrict__ ptr,
                                                       const int* __restrict__ col, const int* __restrict__ blk,
                                                       const double* __restrict__ sv, const double* __restrict__ x,
                                                       double* __restrict__ y, double* __restrict__ partial,
                                                       const int* __restrict__ stop) {
+  constexpr int R = 256 / G;  // block rows per workgroup and unit
+  constexpr int MM = M * M;
   __shared__ double sh[256];
   if (stop && stop[0]) return;
   const int t = threadIdx.x;
   const int lane = t % G;
-  constexpr int R = 256 / G;  // block rows per workgroup and sweep
   double d = 0.0;
-  for (int I0 = blockIdx.x * R; I0 < nbr; I0 += gridDim.x * R) {
-    const int I = I0 + t / G;
+  int ua, ub;
+  xcd_units((nbr + R - 1) / R, ua, ub);
+  for (int u = ua; u < ub; u++) {
+    const int I = u * R + t / G;
     double acc[M];
 #pragma unroll
     for (int r = 0; r < M; r++) acc[r] = 0.0;
@@ -269,10 +289,10 @@ __global__ __launch_bounds__(256) void k_spmv_sym_dot(int nbr, const int* __rest
       const int b = ptr[I], e = ptr[I + 1];
       for (int k = b + lane; k < e; k += G) {
         const int J = col[k];
-        const double* a = sv + (long long)blk[k] * (M * M);
-        double v[M * M], xj[M];
+        const double* a = sv + (long long)blk[k] * MM;
+        double v[MM], xj[M];
 #pragma unroll
-        for (int q = 0; q < M * M; q++) v[q] = a[q];
+        for (int q = 0; q < MM; q++) v[q] = a[q];
 #pragma unroll
         for (int c = 0; c < M; c++) xj[c] = x[M * J + c];
         const bool lower = J < I;
