@@ -552,6 +552,9 @@ void Domain::setup() {
   dscalar.resize(4);
   for (auto& g : groups) g->setup(*this);
   setup_transient();
+  // the PCG's block-symmetric layout depends only on the pattern: built here
+  // with the other index maps rather than in the first solve
+  if (spmv_format != XG_SPMV_CSR) sym_prepare(true);
   XG_HIP(hipStreamSynchronize(s));
   lambda = 0;
   time = 0;
