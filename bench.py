@@ -97,6 +97,8 @@ def main():
                     help="replay PCG chunks from captured graphs (0 = launch each kernel)")
     ap.add_argument("--cg-fuse", type=int, choices=[0, 1], default=1,
                     help="alpha / beta reductions inside the PCG vector kernels on small systems")
+    ap.add_argument("--cg-small", type=int, choices=[0, 1], default=1,
+                    help="whole PCG solve in one workgroup for systems that fit it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -185,7 +187,7 @@ def gpu_system(m, args, rtol, maxit):
     """The PCG system of a GPU model with the bench's issue pattern."""
     m.system(args.system, rtol, maxit)
     if m._b.prefix == "xg_":
-        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse))
+        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small))
 
 
 def newton_main(args):
@@ -497,7 +499,7 @@ def ensemble_main(args):
         m = Model(2, 3)
         define(m, k)
         m.system("GpuBlockPCG", 1e-14, 100000)
-        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse))
+        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small))
         m.setup()
         for _ in range(args.warmup):
             m.analyze(1, dt)

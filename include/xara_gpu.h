@@ -174,10 +174,13 @@ int  xg_spmv_config(xg_domain* d, int format, int lanes);
    kernel launches.  XG_CG_FUSE: on systems small enough (at most 1024
    partial sums per reduction) the alpha and beta reductions run inside the
    update and new-direction kernels, three launches per iteration instead of
-   five.  Default XG_CG_GRAPH | XG_CG_FUSE; the iterates are the same bits
-   under every setting.                                                       */
+   five; the iterates are the same bits with and without either.
+   XG_CG_SMALL: systems of at most 1,024 equations and 32,768 nonzeros are
+   solved whole by one workgroup in one launch (same iteration, its sums in
+   another order: the same result to rounding).  Default all three.         */
 #define XG_CG_GRAPH 1
 #define XG_CG_FUSE  2
+#define XG_CG_SMALL 4
 int  xg_cg_config(xg_domain* d, int chunk, int flags);
 /* `numberer RCM|Plain`                                                       */
 int  xg_numberer(xg_domain* d, int kind);

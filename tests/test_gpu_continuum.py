@@ -235,7 +235,7 @@ def test_pcg_graph_replay_matches_eager_launches():
         m = Model(2, 3)
         meshes.build_rc_frame_per_element(m, fr, theta)
         m.system("GpuBlockPCG", 1e-12, 100000)
-        m.cg_config(chunk, graph, fuse)
+        m.cg_config(chunk, graph, fuse, small=False)
         P = np.zeros((len(fr["floor_nodes"]), 3))
         P[:, 0] = 1.0
         m.loads(fr["floor_nodes"], P)
@@ -256,6 +256,50 @@ def test_pcg_graph_replay_matches_eager_launches():
             assert np.array_equal(a, c)
     with pytest.raises(XaraError):
         Model(2, 3).cg_config(7, True)
+
+
+@pytest.mark.parametrize("system", ["GpuPCG", "GpuBlockPCG"])
+def test_single_workgroup_pcg_matches_launch_per_step(system):
+    """The whole-solve kernel of small systems (one workgroup, one launch)
+    runs the same iteration as the kernel-per-step PCG: the same converged
+    trajectories and iteration counts within rounding's reach, over a frame
+    pushover (block width 3) and a quad strip (2)."""
+    fr = meshes.rc_frame_2d(3, 2)
+    conn, _ = meshes.frame_elements(fr)
+    theta = meshes.default_theta(len(conn))
+    mesh = meshes.quad_strip(16, 4)
+    for case in ("frame", "quad"):
+        print(f"case: small pcg {system} {case}", flush=True)
+        res = []
+        for small in (True, False):
+            if case == "frame":
+                m = Model(2, 3)
+                meshes.build_rc_frame_per_element(m, fr, theta)
+                P = np.zeros((len(fr["floor_nodes"]), 3))
+                P[:, 0] = 1.0
+                m.loads(fr["floor_nodes"], P)
+                node, dof, du, tol = fr["roof"], 1, 6.0, 1e-5
+            else:
+                m = Model(2, 2)
+                meshes.build_quad_strip(m, mesh)
+                m.load(mesh["tip"], 0.0, 1.0)
+                node, dof, du, tol = mesh["tip"], 2, 0.001, 1e-8
+            m.system(system, 1e-12, 100000)
+            m.cg_config(16, True, True, small=small)
+            m.test("NormUnbalance", tol, 40)
+            m.integrator("DisplacementControl", node, dof, du)
+            m.setup()
+            t = []
+            for step in range(6):
+                assert m.analyze(1) == 0
+                t.append(m.trial_disp().copy())
+            res.append((np.array(t), m.counters()["cg_iters"]))
+            m.close()
+        (a, ia), (b, ib) = res
+        assert relerr(a, b) <= 1e-9
+        # the sums' order differs, and on an ill-conditioned frame rounding
+        # moves the iteration at which a solve crosses its tolerance
+        assert abs(ia - ib) <= 0.1 * ib
 
 
 def test_revert_to_start_and_to_last_commit():

@@ -74,7 +74,11 @@ def test_force_frame_pushover_trajectory(three_d):
         P = np.zeros((len(nodes), ndf))
         P[:, 0] = 1.0
         m.loads(nodes, P)
-        m.test("NormUnbalance", 1e-5, 40)
+        # the element iterations leave an unbalance of 1e-6..1e-5 on this
+        # frame: a test at 1e-5 sits on that floor (19-29 Newton iterations
+        # in some steps, convergence decided by rounding); at 1e-4 every step
+        # converges in 3 iterations on both sides
+        m.test("NormUnbalance", 1e-4, 40)
         m.integrator("DisplacementControl", fr["roof"], 1, 6.0)
         m.setup()
         t = []

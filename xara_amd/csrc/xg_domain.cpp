@@ -709,18 +709,25 @@ int Domain::solve(const double* b, double* x) {
   int bad = 0;
   pcbad.download(&bad, 1, stream);
   if (bad) return XG_BAD_LINEAR_SOLVE;
-  cg_start(stream, n, b, x, r0.p, z.p, p.p, bw, binv.p, bfirst.p, bsize.p, part.p, rtol * rtol, maxit, scal.p);
   int conv = 0, it = 0, brk = 0;
   double rr = 0, bb = 0;
-  int issued = 0;
-  const int L = lanes();
-  while (true) {
+  if (cg_small_on && cg_small_fits(n, (long long)num.colind.size())) {
+    // the whole solve in one workgroup, one launch
+    cg_solve_small(stream, n, rowptr.p, colind.p, vals.p, b, x, r0.p, z.p, p.p, q.p, bw, binv.p, bfirst.p, bsize.p,
+                   rtol * rtol, maxit, scal.p);
     cg_read_scal(stream, scal.p, &conv, &it, &rr, &bb, &brk);
-    if (conv) break;
-    if (issued >= maxit) break;
-    const int chunk = std::min(cg_chunk, maxit - issued);
-    cg_chunk_run(chunk, issued, L, bw, x);
-    issued += chunk;
+  } else {
+    cg_start(stream, n, b, x, r0.p, z.p, p.p, bw, binv.p, bfirst.p, bsize.p, part.p, rtol * rtol, maxit, scal.p);
+    int issued = 0;
+    const int L = lanes();
+    while (true) {
+      cg_read_scal(stream, scal.p, &conv, &it, &rr, &bb, &brk);
+      if (conv) break;
+      if (issued >= maxit) break;
+      const int chunk = std::min(cg_chunk, maxit - issued);
+      cg_chunk_run(chunk, issued, L, bw, x);
+      issued += chunk;
+    }
   }
   counters.solves++;
   counters.cg_iters += it;

@@ -140,6 +140,7 @@ struct Domain {
   static constexpr int kCgGraphs = 8;
   bool cg_graph_on = true;
   bool cg_fuse_on = true;  // alpha / beta reductions fused into their neighbours (small systems)
+  bool cg_small_on = true; // whole solve in one workgroup where the system fits (cg_small_fits)
   std::vector<CgGraph> cg_graphs;
   int test_kind = XG_TEST_NORM_UNBALANCE;
   double test_tol = 1e-6;

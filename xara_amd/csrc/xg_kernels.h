@@ -84,6 +84,11 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
                    const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol,
                    bool fuse,  // fuse: alpha / beta reductions in the update / new-direction kernels when small
                    const SymSpmv* sym = nullptr);
+// whole solve in one workgroup (k_pcg_small) for systems that fit it
+bool cg_small_fits(int n, long long nnz);
+void cg_solve_small(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
+                    double* x, double* r, double* z, double* p, double* q, int bw, const double* binv,
+                    const int* bfirst, const unsigned char* bsize, double tol2, int maxit, void* scal);
 void cg_read_scal(hipStream_t s, const void* scal, int* converged, int* iter, double* rr, double* bb,
                   int* breakdown);
 

@@ -637,6 +637,143 @@ __global__ __launch_bounds__(256) void k_cg_init_scal(int nb, const double* __re
   }
 }
 
+// ---------------------------------------------------------------------------
+// Whole PCG solve of a small system in one workgroup of 1024 threads: the
+// same iteration as cg_start + cg_iterations (x = 0, r = b, z = M^{-1} r,
+// alpha = r.z / p.q, beta = r.z (new) / r.z (old), stop when r.r <= tol2 b.b
+// or after maxit iterations, breakdown on a zero or non-finite p.q), with
+// workgroup barriers in place of kernel boundaries and no host round trip
+// per chunk: a few microseconds per iteration where five (or three) launches
+// per iteration take tens.  Thread t owns rows t, t + 1024, ...; sums go
+// through a fixed wave butterfly and a fixed pass over the 16 wave sums, so
+// every thread holds the same bits and takes the same branches.  The final
+// scalars go to the CGScal the host reads.
+// ---------------------------------------------------------------------------
+constexpr int kSmallPcgThreads = 1024;
+
+__device__ __forceinline__ double block_sum_1024(double v, double* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < kSmallPcgThreads / 64; w++) s += sh[w];
+  __syncthreads();
+  return s;
+}
+
+template <int BW>
+__device__ __forceinline__ double small_precond(int i, const double* __restrict__ r, const double* __restrict__ binv,
+                                                const int* __restrict__ bfirst,
+                                                const unsigned char* __restrict__ bsize) {
+  if constexpr (BW == 1) {
+    return binv[i] * r[i];
+  } else {
+    const int f = bfirst[i], m = bsize[i];
+    double zi = 0.0;
+#pragma unroll
+    for (int j = 0; j < BW; j++)
+      if (j < m) zi += binv[(long long)i * BW + j] * r[f + j];
+    return zi;
+  }
+}
+
+template <int BW>
+__global__ __launch_bounds__(kSmallPcgThreads) void k_pcg_small(
+    int n, const int* __restrict__ rowptr, const int* __restrict__ colind, const double* __restrict__ vals,
+    const double* __restrict__ b, double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+    double* __restrict__ p, double* __restrict__ q, const double* __restrict__ binv, const int* __restrict__ bfirst,
+    const unsigned char* __restrict__ bsize, double tol2, int maxit, CGScal* __restrict__ S) {
+  __shared__ double sh[kSmallPcgThreads / 64];
+  const int t = threadIdx.x;
+  constexpr int NT = kSmallPcgThreads;
+  for (int i = t; i < n; i += NT) {
+    x[i] = 0.0;
+    r[i] = b[i];
+  }
+  __syncthreads();
+  double l1 = 0.0, l2 = 0.0;
+  for (int i = t; i < n; i += NT) {
+    const double zi = small_precond<BW>(i, r, binv, bfirst, bsize);
+    z[i] = zi;
+    p[i] = zi;
+    l1 += r[i] * zi;
+    l2 += r[i] * r[i];
+  }
+  double rz = block_sum_1024(l1, sh);
+  const double bb = block_sum_1024(l2, sh);
+  double rr = bb, pq = 0.0, alpha = 0.0, beta = 0.0;
+  int it = 0, brk = 0;
+  bool done = bb == 0.0;
+  while (!done) {
+    double lpq = 0.0;
+    for (int i = t; i < n; i += NT) {
+      // sixteen entries' loads in flight at a time, summed in order
+      const int rb = rowptr[i], re = rowptr[i + 1];
+      double s = 0.0;
+      for (int k0 = rb; k0 < re; k0 += 16) {
+        int c[16];
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const int k = min(k0 + u, re - 1);
+          c[u] = colind[k];
+          v[u] = vals[k];
+        }
+        double pc[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) pc[u] = p[c[u]];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+          if (k0 + u < re) s += v[u] * pc[u];
+      }
+      q[i] = s;
+      lpq += p[i] * s;
+    }
+    pq = block_sum_1024(lpq, sh);
+    if (pq == 0.0 || !isfinite(pq)) {
+      brk = 1;
+      break;
+    }
+    alpha = rz / pq;
+    for (int i = t; i < n; i += NT) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+    }
+    __syncthreads();
+    l1 = 0.0;
+    l2 = 0.0;
+    for (int i = t; i < n; i += NT) {
+      const double zi = small_precond<BW>(i, r, binv, bfirst, bsize);
+      z[i] = zi;
+      l1 += r[i] * zi;
+      l2 += r[i] * r[i];
+    }
+    const double rzn = block_sum_1024(l1, sh);
+    rr = block_sum_1024(l2, sh);
+    it++;
+    beta = rzn / rz;
+    rz = rzn;
+    if (rr <= tol2 * bb || it >= maxit) break;
+    for (int i = t; i < n; i += NT) p[i] = z[i] + beta * p[i];
+    __syncthreads();
+  }
+  if (t == 0) {
+    S->rzp[0] = S->rzp[1] = rz;
+    S->pq = pq;
+    S->rr = rr;
+    S->bb = bb;
+    S->alpha = alpha;
+    S->beta = beta;
+    S->tol2 = tol2;
+    S->maxit = maxit;
+    S->iterp[0] = S->iterp[1] = it;
+    S->breakdown = brk;
+    S->converged = 1;
+  }
+}
+
 // Jacobi / nodal block-Jacobi inverse from the assembled CSR matrix.
 // One thread per node block: gathers the m x m diagonal block (m <= M, the
 // nodal DOF count), inverts it by Gauss-Jordan with partial pivoting and
@@ -983,6 +1120,29 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
       hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, nbv, part, S, par);
       hipLaunchKernelGGL(k_cg_newp, dim3(nbv), dim3(256), 0, s, n, S, z, p);
     }
+  }
+}
+
+// One CU's memory parallelism bounds this kernel: it beats the launch-per-
+// step PCG on models of a few hundred equations (2D frames of an ensemble)
+// and loses from about two thousand on (a 64 x 16 quad strip, a 5 x 5 x 10
+// frame), so only systems of up to 1,024 equations and 32,768 nonzeros.
+bool cg_small_fits(int n, long long nnz) { return n > 0 && n <= 1024 && nnz <= 32768; }
+
+void cg_solve_small(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
+                    double* x, double* r, double* z, double* p, double* q, int bw, const double* binv,
+                    const int* bfirst, const unsigned char* bsize, double tol2, int maxit, void* scal) {
+  CGScal* S = (CGScal*)scal;
+  const dim3 g(1), blk(kSmallPcgThreads);
+  switch (bw) {
+#define XG_PS(W)                                                                                              \
+  case W:                                                                                                    \
+    hipLaunchKernelGGL(k_pcg_small<W>, g, blk, 0, s, n, rowptr, colind, vals, b, x, r, z, p, q, binv, bfirst, \
+                       bsize, tol2, maxit, S);                                                               \
+    break;
+    XG_PS(1) XG_PS(2) XG_PS(3) XG_PS(4) XG_PS(5) XG_PS(6)
+#undef XG_PS
+    default: throw Error(XG_ERR_ARG, "PCG block width must be 1..6");
   }
 }
 
