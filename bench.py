@@ -190,10 +190,60 @@ def gpu_system(m, args, rtol, maxit):
         m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small))
 
 
+def newton_pushover(m, n, steps, max_seconds=None, max_steps=None, log=True):
+    """The brick-cube pushover on a set-up model: first-yield pressure p_y
+    from one elastic solve at unit pressure, then `steps` LoadControl steps
+    of 1.5 p_y / steps (NewtonRaphson, NormUnbalance).  Returns (Newton
+    iterations, wall seconds, steps run, p_y, PCG iterations, solves) of the
+    load steps; stops early after max_steps steps or max_seconds."""
+    from xara_amd import meshes
+    mesh = meshes.brick_cube(n)
+    npts = 8 * len(mesh["ele_tags"])
+    m.set_load_factor(1.0)
+    B = m.form_unbalance()
+    m.form_tangent(values=False)
+    u = m.solve(B)
+    m.incr_disp(u)
+    m.update()
+    sig = m.point_data("stress", npts)
+    s0, s1, s2, s3, s4, s5 = sig.T
+    vm = np.sqrt(0.5 * ((s0 - s1) ** 2 + (s1 - s2) ** 2 + (s2 - s0) ** 2) + 3 * (s3 ** 2 + s4 ** 2 + s5 ** 2))
+    p_y = 250.0 / vm.max()
+    m.revert_to_start()
+    m.set_load_factor(0.0)
+    m.test("NormUnbalance", 1e-6 * p_y * (1.0 / n) ** 2 * n, 25)
+    m.integrator("LoadControl", 1.5 * p_y / steps)
+    if log:
+        print(f"bench: first-yield pressure {p_y:.4f}; running {steps} steps", file=sys.stderr, flush=True)
+    c0 = m.counters()
+    t0 = time.perf_counter()
+    run = 0
+    for st in range(steps if max_steps is None else min(steps, max_steps)):
+        rc = m.analyze(1)
+        run += 1
+        c = m.counters()
+        if log:
+            print(f"bench: step {st + 1} rc={rc} newton={c['newton_iters'] - c0['newton_iters']} "
+                  f"cg={c['cg_iters'] - c0['cg_iters']} t={time.perf_counter() - t0:.1f}s", file=sys.stderr,
+                  flush=True)
+        if rc != 0:
+            break
+        if max_seconds is not None and time.perf_counter() - t0 > max_seconds:
+            break
+    wall = time.perf_counter() - t0
+    c = m.counters()
+    return (c["newton_iters"] - c0["newton_iters"], wall, run, p_y, c["cg_iters"] - c0["cg_iters"],
+            c["solves"] - c0["solves"])
+
+
 def newton_main(args):
     """Newton iterations/s of the full pushover: cube under uniform top
     pressure ramped to 1.5x the first-yield pressure in `newton_steps`
-    LoadControl steps, NewtonRaphson with NormUnbalance, GPU PCG."""
+    LoadControl steps, NewtonRaphson with NormUnbalance, GPU PCG.  The CPU
+    baseline: the oracle (one core, its profile LDL^T direct solve, the
+    role of the reference's default solver) on the same pushover of a 12^3
+    cube for about --cpu-seconds, with the GPU on that cube beside it (a
+    64^3 factorisation would need tens of GB and hours)."""
     from xara_amd import meshes
     from xara_amd.model import Model
 
@@ -207,37 +257,8 @@ def newton_main(args):
     m.setup()
     setup_s = time.time() - t0
     npts = 8 * len(mesh["ele_tags"])
-    # first-yield pressure from one elastic solve at unit pressure
-    m.set_load_factor(1.0)
-    B = m.form_unbalance()
-    m.form_tangent(values=False)
-    u = m.solve(B)
-    spmv_ms = m.time_op("spmv", 20)  # on the assembled elastic tangent
-    m.incr_disp(u)
-    m.update()
-    sig = m.point_data("stress", npts)
-    s0, s1, s2, s3, s4, s5 = sig.T
-    vm = np.sqrt(0.5 * ((s0 - s1) ** 2 + (s1 - s2) ** 2 + (s2 - s0) ** 2) + 3 * (s3 ** 2 + s4 ** 2 + s5 ** 2))
-    p_y = 250.0 / vm.max()
-    m.revert_to_start()
-    m.set_load_factor(0.0)
-    m.test("NormUnbalance", 1e-6 * p_y * (1.0 / n) ** 2 * n, 25)
-    m.integrator("LoadControl", 1.5 * p_y / args.newton_steps)
-    print(f"bench: first-yield pressure {p_y:.4f}; running {args.newton_steps} steps", file=sys.stderr,
-          flush=True)
-    c0 = m.counters()
-    t0 = time.perf_counter()
-    for st in range(args.newton_steps):
-        rc = m.analyze(1)
-        c = m.counters()
-        print(f"bench: step {st + 1} rc={rc} newton={c['newton_iters'] - c0['newton_iters']} "
-              f"cg={c['cg_iters'] - c0['cg_iters']} t={time.perf_counter() - t0:.1f}s", file=sys.stderr,
-              flush=True)
-        if rc != 0:
-            break
-    wall = time.perf_counter() - t0
-    c = m.counters()
-    iters = c["newton_iters"] - c0["newton_iters"]
+    iters, wall, _, p_y, cg, solves = newton_pushover(m, n, args.newton_steps)
+    spmv_ms = m.time_op("spmv", 20)
     plastic = int(np.count_nonzero(m.point_data("plastic", npts)[:, 6] > 0))
     out = {
         "metric": "Newton iterations/s", "value": iters / wall, "unit": "iterations/s", "n_gpus": 1,
@@ -246,11 +267,40 @@ def newton_main(args):
         "data": "synthetic",
         "config": {"workload": f"stdBrick J2 cube {n}^3 pushover to 1.5 p_y, LoadControl, "
                                f"{args.system} rtol 1e-10", "newton_iters": iters,
-                   "cg_iters": c["cg_iters"] - c0["cg_iters"], "solves": c["solves"] - c0["solves"],
+                   "cg_iters": cg, "solves": solves,
                    "spmv": args.spmv, "spmv_lanes": args.lanes, "spmv_ms": round(spmv_ms, 4),
                    "plastic_points": plastic, "points": npts, "setup_s": round(setup_s, 2),
                    "p_y": p_y, "final_load_factor": m.load_factor},
     }
+    m.close()
+    if not args.no_cpu:
+        from tests import _oracle
+        nc = 12
+        res = {}
+        for name in ("cpu", "gpu"):  # the GPU runs the steps the CPU sample ran
+            mm = _oracle.oracle_model(3, 3) if name == "cpu" else Model(3, 3, device=0)
+            meshes.build_brick_cube(mm, meshes.brick_cube(nc), pressure=1.0)
+            if name == "cpu":
+                mm.system(args.system, 1e-10, 50000)
+            else:
+                gpu_system(mm, args, 1e-10, 50000)
+            mm.setup()
+            it_c, wall_c, run_c = newton_pushover(
+                mm, nc, args.newton_steps, max_seconds=args.cpu_seconds if name == "cpu" else None,
+                max_steps=res["cpu"][2] if name == "gpu" else None, log=False)[:3]
+            res[name] = (it_c, wall_c, run_c)
+            mm.close()
+            print(f"bench: newton {nc}^3 {name} {it_c} iterations in {wall_c:.2f}s ({run_c} steps)",
+                  file=sys.stderr, flush=True)
+        cpu_rate = res["cpu"][0] / res["cpu"][1]
+        gpu_rate = res["gpu"][0] / res["gpu"][1]
+        out["cpu_baseline"] = {
+            "value": cpu_rate, "unit": "iterations/s", "cores": 1, "kind": "port",
+            "sample": f"the same pushover on a {nc}^3 cube ({res['cpu'][2]} of {args.newton_steps} load steps, "
+                      f"{res['cpu'][0]} Newton iterations), oracle/liboracle.so with its profile LDL^T direct "
+                      f"solve, one thread"}
+        out["same_sample_gpu"] = {"cube": nc, "value": gpu_rate, "unit": "iterations/s",
+                                  "gpu_over_cpu": gpu_rate / cpu_rate}
     print(json.dumps(out))
 
 
