@@ -324,6 +324,7 @@ void Domain::spmv(const double* x, double* y, bool solver) {
 
 void Domain::setup() {
   XG_HIP(hipSetDevice(device));
+  cg_graph_clear();  // captured with the buffers of an earlier set-up
   if (nodedefs.empty()) throw Error(XG_ERR_STATE, "no nodes defined");
   // nodes in ascending tag order
   nnode = (int)nodedefs.size();
