@@ -429,6 +429,30 @@ def frame_main(args):
         out["newton"] = {"steps": args.frame_steps, "newton_iters": c["newton_iters"] - c0["newton_iters"],
                          "cg_iters": c["cg_iters"] - c0["cg_iters"], "wall_s": wall,
                          "newton_iters_per_s": (c["newton_iters"] - c0["newton_iters"]) / wall}
+    if not args.no_cpu:
+        # the oracle's Domain::update over a 3 x 3 x 5 frame of the same
+        # sections with the same kind of synthetic field, about --cpu-seconds
+        from tests import _oracle
+        frs = meshes.rc_frame_3d(3, 3, 5)
+        o = _oracle.oracle_model(3, 6)
+        meshes.build_rc_frame_3d(o, frs)
+        o.setup()
+        o.set_trial_disp(frame_field(frs))
+        nf = 120 * 5 * (len(frs["columns"]) + len(frs["beams_x"]) + len(frs["beams_y"]))
+        o.update()
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            o.update()
+            reps += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        cw = time.perf_counter() - t0
+        o.close()
+        out["cpu_baseline"] = {"value": nf * reps / cw, "unit": "fibers/s", "cores": 1, "kind": "port",
+                               "sample": f"{reps} serial updates of a 3x3-bay, 5-story frame of the same "
+                                         f"sections ({nf} fibers) with the same kind of synthetic field, "
+                                         f"oracle/liboracle.so, one thread"}
+        out["gpu_over_cpu"] = value / (nf * reps / cw)
     print(json.dumps(out))
 
 
