@@ -168,8 +168,9 @@ int  xg_system(xg_domain* d, int kind, double rtol, int maxit);
 #define XG_SPMV_SYM  2
 int  xg_spmv_config(xg_domain* d, int format, int lanes);
 /* PCG issue pattern.  No counterpart in the reference: a tuning knob.  The
-   host issues `chunk` iterations (even, 2..1024; default 16) between reads
-   of the convergence flag (iterations after convergence return at once).
+   host issues `chunk` iterations (even, 2..1024; default 16) before its
+   first read of the convergence flag, then twice as many before each next
+   read, up to 8 x chunk (iterations after convergence return at once).
    XG_CG_GRAPH: a full chunk is replayed from a captured hipGraph of its
    kernel launches.  XG_CG_FUSE: on systems small enough (at most 1024
    partial sums per reduction) the alpha and beta reductions run inside the

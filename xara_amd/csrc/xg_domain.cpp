@@ -656,12 +656,13 @@ void Domain::cg_graph_clear() {
   cg_graphs.clear();
 }
 
-// One chunk of PCG iterations.  A full chunk (cg_chunk iterations, even, so
-// that it starts on an even iteration like the captured one) is replayed
-// from a graph of its launches: one host call instead of five kernel
-// launches per iteration, which bounds the iteration rate of small systems.
-// Graphs hold kernels only, one per set of launch arguments (at most
-// kCgGraphs; further sets are launched one by one).
+// One chunk of PCG iterations.  A chunk of an even length starting on an
+// even iteration (all of them but a solve's last, shortened one) is
+// replayed from a graph of its launches: one host call instead of three to
+// five kernel launches per iteration, which bounds the iteration rate of
+// small systems.  Graphs hold kernels only, one per chunk length and set of
+// launch arguments (at most kCgGraphs; further ones are launched one by
+// one).
 void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
   const int n = num.neq, nbm = spmv_nbm();
   const SymSpmv* sym = sym_prepare(true);
@@ -669,7 +670,7 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
     cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, q.p, bw, binv.p, bfirst.p,
                   bsize.p, part.p, scal.p, nbm, nbcol.p, cg_fuse_on, sym);
   };
-  if (!cg_graph_on || iters != cg_chunk || (it0 & 1) || (cg_chunk & 1)) {
+  if (!cg_graph_on || (iters & 1) || (it0 & 1)) {
     eager(stream, iters, it0);
     return;
   }
@@ -719,15 +720,19 @@ int Domain::solve(const double* b, double* x) {
     cg_read_scal(stream, scal.p, &conv, &it, &rr, &bb, &brk);
   } else {
     cg_start(stream, n, b, x, r0.p, z.p, p.p, bw, binv.p, bfirst.p, bsize.p, part.p, rtol * rtol, maxit, scal.p);
-    int issued = 0;
+    int issued = 0, next = cg_chunk;
     const int L = lanes();
     while (true) {
       cg_read_scal(stream, scal.p, &conv, &it, &rr, &bb, &brk);
       if (conv) break;
       if (issued >= maxit) break;
-      const int chunk = std::min(cg_chunk, maxit - issued);
+      // chunks double (up to 8x cg_chunk) while the solve goes on: fewer
+      // host round trips on long solves; iterations issued past
+      // convergence return at once
+      const int chunk = std::min(next, maxit - issued);
       cg_chunk_run(chunk, issued, L, bw, x);
       issued += chunk;
+      next = std::min(2 * next, 8 * cg_chunk);
     }
   }
   counters.solves++;

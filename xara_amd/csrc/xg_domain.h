@@ -137,7 +137,7 @@ struct Domain {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   };
-  static constexpr int kCgGraphs = 8;
+  static constexpr int kCgGraphs = 16;
   bool cg_graph_on = true;
   bool cg_fuse_on = true;  // alpha / beta reductions fused into their neighbours (small systems)
   bool cg_small_on = true; // whole solve in one workgroup where the system fits (cg_small_fits)
