@@ -173,9 +173,10 @@ int  xg_spmv_config(xg_domain* d, int format, int lanes);
    read, up to 8 x chunk (iterations after convergence return at once).
    XG_CG_GRAPH: a full chunk is replayed from a captured hipGraph of its
    kernel launches.  XG_CG_FUSE: on systems small enough (at most 1024
-   partial sums per reduction) the alpha and beta reductions run inside the
-   update and new-direction kernels, three launches per iteration instead of
-   five; the iterates are the same bits with and without either.
+   partial sums per reduction) an iteration is two launches instead of
+   five: the product forms the new direction on the fly (beta reduced by
+   every workgroup) and the update reduces alpha itself; the iterates are
+   the same bits with and without either.
    XG_CG_SMALL: systems of at most 1,024 equations and 32,768 nonzeros are
    solved whole by one workgroup in one launch (same iteration, its sums in
    another order: the same result to rounding).  Default all three.         */

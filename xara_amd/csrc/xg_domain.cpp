@@ -540,6 +540,7 @@ void Domain::setup() {
   r1.resize(neq);
   z.resize(neq);
   p.resize(neq);
+  p2.resize(neq);
   q.resize(neq);
   tmpv.resize(neq);
   part.resize(cg_part_size(neq));
@@ -667,8 +668,8 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
   const int n = num.neq, nbm = spmv_nbm();
   const SymSpmv* sym = sym_prepare(true);
   auto eager = [&](hipStream_t s, int k, int i0) {
-    cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, q.p, bw, binv.p, bfirst.p,
-                  bsize.p, part.p, scal.p, nbm, nbcol.p, cg_fuse_on, sym);
+    cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, p2.p, q.p, bw, binv.p,
+                  bfirst.p, bsize.p, part.p, scal.p, nbm, nbcol.p, cg_fuse_on, sym);
   };
   if (!cg_graph_on || (iters & 1) || (it0 & 1)) {
     eager(stream, iters, it0);
@@ -677,7 +678,7 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
   const std::vector<uintptr_t> key = {
       (uintptr_t)iters,   (uintptr_t)L,       (uintptr_t)n,       (uintptr_t)bw,      (uintptr_t)nbm, (uintptr_t)cg_fuse_on,
       (uintptr_t)x,       (uintptr_t)rowptr.p, (uintptr_t)colind.p, (uintptr_t)vals.p, (uintptr_t)r0.p,
-      (uintptr_t)r1.p,    (uintptr_t)z.p,     (uintptr_t)p.p,     (uintptr_t)q.p,     (uintptr_t)binv.p,
+      (uintptr_t)r1.p,    (uintptr_t)z.p,     (uintptr_t)p.p,     (uintptr_t)p2.p,    (uintptr_t)q.p,     (uintptr_t)binv.p,
       (uintptr_t)bfirst.p, (uintptr_t)bsize.p, (uintptr_t)part.p, (uintptr_t)scal.p,  (uintptr_t)nbcol.p,
       (uintptr_t)(sym ? sym->val : nullptr), (uintptr_t)(sym ? sym->lanes : 0)};
   for (auto& g : cg_graphs)
@@ -725,11 +726,15 @@ int Domain::solve(const double* b, double* x) {
     while (true) {
       cg_read_scal(stream, scal.p, &conv, &it, &rr, &bb, &brk);
       if (conv) break;
-      if (issued >= maxit) break;
+      // maxit + 1 iterations at most: the fused iteration tests the previous
+      // update's convergence at its start, so the last test comes one
+      // iteration later (the extra iteration then returns at once, as it
+      // does on the other path, where the test ends the iteration)
+      if (issued >= maxit + 1) break;
       // chunks double (up to 8x cg_chunk) while the solve goes on: fewer
       // host round trips on long solves; iterations issued past
       // convergence return at once
-      const int chunk = std::min(next, maxit - issued);
+      const int chunk = std::min(next, maxit + 1 - issued);
       cg_chunk_run(chunk, issued, L, bw, x);
       issued += chunk;
       next = std::min(2 * next, 8 * cg_chunk);

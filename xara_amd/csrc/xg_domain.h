@@ -116,7 +116,7 @@ struct Domain {
   DVec<int> asm_ptr, asm_idx, rhs_ptr, rhs_idx;
   DVec<int> nfail, pcbad;
   // solver workspace
-  DVec<double> r0, r1, z, p, q, part, binv, tmpv;
+  DVec<double> r0, r1, z, p, p2, q, part, binv, tmpv;
   DVec<int> bfirst, blk_first;
   DVec<unsigned char> bsize, blk_size;
   DVec<char> scal;

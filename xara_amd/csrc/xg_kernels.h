@@ -80,7 +80,7 @@ void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, doubl
               int maxit, void* scal);
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
-                   double* p, double* q, int bw, const double* binv, const int* bfirst,
+                   double* p, double* p2, double* q, int bw, const double* binv, const int* bfirst,
                    const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol,
                    bool fuse,  // fuse: alpha / beta reductions in the update / new-direction kernels when small
                    const SymSpmv* sym = nullptr);

@@ -120,226 +120,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 // ---------------------------------------------------------------------------
-// CSR SpMV y = A x with L lanes per row; per-block partial of x.y
-// ---------------------------------------------------------------------------
-// One lane's share of a row: sum over k = b + lane, b + lane + L, ... < e of
-// vals[k] * x[col(raw(k), k)], in that order.  raw(k) is the stored column
-// index of entry k and col() forms the column from it.  The value and index
-// loads of U entries are issued together, then their x loads, then the
-// products, so a wave waits for memory twice per U entries rather than twice
-// per entry.  Entries past e load entry e - 1 again (in bounds, cached) and
-// add nothing.
-template <int L, int U = 4, class Raw, class Col>
-__device__ __forceinline__ double row_dot(int b, int e, int lane, const double* __restrict__ vals,
-                                          const double* __restrict__ x, Raw raw, Col col) {
-  double sum = 0.0;
-  for (int k0 = b + lane; k0 < e; k0 += U * L) {
-    double v[U], xv[U];
-    int r[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int k = min(k0 + u * L, e - 1);
-      v[u] = vals[k];
-      r[u] = raw(k);
-    }
-#pragma unroll
-    for (int u = 0; u < U; u++) xv[u] = x[col(r[u], min(k0 + u * L, e - 1))];
-#pragma unroll
-    for (int u = 0; u < U; u++)
-      if (k0 + u * L < e) sum += v[u] * xv[u];
-  }
-  return sum;
-}
-
-template <int L>
-__global__ __launch_bounds__(256) void k_spmv_dot(int n, const int* __restrict__ rowptr,
-                                                  const int* __restrict__ colind,
-                                                  const double* __restrict__ vals,
-                                                  const double* __restrict__ x, double* __restrict__ y,
-                                                  double* __restrict__ partial,
-                                                  const int* __restrict__ stop) {
-  __shared__ double sh[256];
-  if (stop && stop[0]) return;
-  const int t = threadIdx.x;
-  const int lane = t % L;
-  constexpr int R = 256 / L;  // rows per block and sweep
-  double d = 0.0;
-  // grid-stride over row groups: a fixed assignment, so the partial of each
-  // block (and the dot product) does not depend on timing
-  for (int row0 = blockIdx.x * R; row0 < n; row0 += gridDim.x * R) {
-    const int row = row0 + t / L;
-    double sum = 0.0;
-    if (row < n) {
-      const int b = rowptr[row], e = rowptr[row + 1];
-      sum = row_dot<L>(
-          b, e, lane, vals, x, [&](int k) { return colind[k]; }, [](int c, int) { return c; });
-    }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
-    if (row < n && lane == 0) {
-      y[row] = sum;
-      d += sum * x[row];
-    }
-  }
-  if (partial) {
-    const double bs = block_sum<256>(d, sh);
-    if (t == 0) partial[blockIdx.x] = bs;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Node-blocked SpMV.  When every node owns M consecutive equations (all its
-// DOFs free, M = ndf) or none, equation M*J + r belongs to node block J and
-// its CSR row lists the neighbouring blocks' equations as ascending runs of
-// M, the same runs in each of the M rows of a block.  The columns then follow
-// from one block column per neighbour (nbcol, 4 bytes per M*M nonzeros
-// instead of 4 per nonzero) and rowptr[M*J] / (M*M) is block J's first entry
-// in nbcol; the values are the CSR values themselves.  Same lane layout,
-// reductions and dot-product partials as k_spmv_dot.
-// ---------------------------------------------------------------------------
-template <int L, int M>
-__global__ __launch_bounds__(256) void k_spmv_nb_dot(int n, const int* __restrict__ rowptr,
-                                                     const int* __restrict__ nbcol,
-                                                     const double* __restrict__ vals,
-                                                     const double* __restrict__ x, double* __restrict__ y,
-                                                     double* __restrict__ partial,
-                                                     const int* __restrict__ stop) {
-  __shared__ double sh[256];
-  if (stop && stop[0]) return;
-  const int t = threadIdx.x;
-  const int lane = t % L;
-  constexpr int R = 256 / L;
-  double d = 0.0;
-  for (int row0 = blockIdx.x * R; row0 < n; row0 += gridDim.x * R) {
-    const int row = row0 + t / L;
-    double sum = 0.0;
-    if (row < n) {
-      const int b = rowptr[row], e = rowptr[row + 1];
-      // the M rows of a block have the same length, so the block's first
-      // entry is b - (row % M) * (e - b)
-      const int* nc = nbcol + (b - (row % M) * (e - b)) / (M * M);
-      sum = row_dot<L>(
-          b, e, lane, vals, x, [&](int k) { return nc[(k - b) / M]; },
-          [&](int c, int k) { return c * M + (k - b) % M; });
-    }
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
-    if (row < n && lane == 0) {
-      y[row] = sum;
-      d += sum * x[row];
-    }
-  }
-  if (partial) {
-    const double bs = block_sum<256>(d, sh);
-    if (t == 0) partial[blockIdx.x] = bs;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Block-symmetric SpMV.  The consistent tangents of this path are symmetric,
-// so a node-blocked matrix is kept a second time as its upper M x M blocks
-// only (block-major, row-major inside a block, copied from the CSR values
-// after each assembly): about half the bytes per product.  Block row I lists
-// its neighbours J ascending, each with the stored block that holds it:
-// (I, J) itself for J >= I, (J, I) for J < I, used transposed.  A stored
-// block is read twice per product, by its own row and by its partner's, a
-// few thousand block rows apart in an ascending sweep, so the second read
-// finds it in a cache.  Units of 256/G block rows are dealt out to the XCDs
-// in contiguous eighths (the hardware dispatches workgroup b to XCD b % 8)
-// and each workgroup takes a contiguous run of units, so that a block row's
-// lower partners, read shortly before by the same XCD, are more often found
-// in its L2 (4% faster at 64^3 than a plain grid-stride sweep).  G lanes per
-// block row, one neighbour block per lane and pass (its M*M values, column
-// and M entries of x loaded together), the M row sums reduced across the G
-// lanes.  (Staging a row's blocks in LDS with consecutive lanes reading
-// consecutive doubles measured 1.7x slower: the barrier between the copy and
-// the products and the LDS footprint cost more than the scattered loads.)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void xcd_units(int units, int& ua, int& ub) {
-  const int nx = min(8, (int)gridDim.x);
-  const int xcd = blockIdx.x % nx, l = blockIdx.x / nx;
-  const int gx = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups of this XCD
-  const int u0 = (int)((long long)units * xcd / nx), u1 = (int)((long long)units * (xcd + 1) / nx);
-  const int cnt = (u1 - u0 + gx - 1) / gx;
-  ua = min(u1, u0 + l * cnt);
-  ub = min(u1, ua + cnt);
-}
-
-template <int M, int G>
-__global__ __launch_bounds__(256) void k_spmv_sym_dot(int nbr, const int* __restrict__ ptr,
-                                                      const int* __restrict__ col, const int* __restrict__ blk,
-                                                      const double* __restrict__ sv, const double* __restrict__ x,
-                                                      double* __restrict__ y, double* __restrict__ partial,
-                                                      const int* __restrict__ stop) {
-  constexpr int R = 256 / G;  // block rows per workgroup and unit
-  constexpr int MM = M * M;
-  __shared__ double sh[256];
-  if (stop && stop[0]) return;
-  const int t = threadIdx.x;
-  const int lane = t % G;
-  double d = 0.0;
-  int ua, ub;
-  xcd_units((nbr + R - 1) / R, ua, ub);
-  for (int u = ua; u < ub; u++) {
-    const int I = u * R + t / G;
-    double acc[M];
-#pragma unroll
-    for (int r = 0; r < M; r++) acc[r] = 0.0;
-    if (I < nbr) {
-      const int b = ptr[I], e = ptr[I + 1];
-      for (int k = b + lane; k < e; k += G) {
-        const int J = col[k];
-        const double* a = sv + (long long)blk[k] * MM;
-        double v[MM], xj[M];
-#pragma unroll
-        for (int q = 0; q < MM; q++) v[q] = a[q];
-#pragma unroll
-        for (int c = 0; c < M; c++) xj[c] = x[M * J + c];
-        const bool lower = J < I;
-#pragma unroll
-        for (int r = 0; r < M; r++)
-#pragma unroll
-          for (int c = 0; c < M; c++) acc[r] += (lower ? v[c * M + r] : v[r * M + c]) * xj[c];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < M; r++)
-#pragma unroll
-      for (int off = G / 2; off > 0; off >>= 1) acc[r] += __shfl_down(acc[r], off, G);
-    if (I < nbr && lane == 0) {
-#pragma unroll
-      for (int r = 0; r < M; r++) {
-        y[M * I + r] = acc[r];
-        d += acc[r] * x[M * I + r];
-      }
-    }
-  }
-  if (partial) {
-    const double bs = block_sum<256>(d, sh);
-    if (t == 0) partial[blockIdx.x] = bs;
-  }
-}
-
-// Stored block k = rows M*I.. of the CSR values from entry src[k] of row M*I
-// (the block's first column), len[k] entries per row of block row I.
-template <int M>
-__global__ __launch_bounds__(256) void k_sym_convert(int nup, const int* __restrict__ src,
-                                                     const int* __restrict__ len, const double* __restrict__ vals,
-                                                     double* __restrict__ sv) {
-  for (int k = blockIdx.x * 256 + threadIdx.x; k < nup; k += gridDim.x * 256) {
-    const long long s0 = src[k];
-    const int L = len[k];
-    double v[M * M];
-#pragma unroll
-    for (int r = 0; r < M; r++)
-#pragma unroll
-      for (int c = 0; c < M; c++) v[r * M + c] = vals[s0 + (long long)r * L + c];
-#pragma unroll
-    for (int q = 0; q < M * M; q++) sv[(long long)k * (M * M) + q] = v[q];
-  }
-}
-
-// ---------------------------------------------------------------------------
 // PCG scalar state kept on the device
 // ---------------------------------------------------------------------------
 // r.z and the iteration count are kept per iteration parity: iteration k
@@ -349,6 +129,7 @@ __global__ __launch_bounds__(256) void k_sym_convert(int nup, const int* __restr
 struct CGScal {
   double rzp[2], pq, rr, bb, alpha, beta, tol2;
   int converged, iterp[2], maxit, breakdown;
+  int fresh;  // no update yet in this solve (the first new direction is z itself)
 };
 
 // Sum of `nb` partials by one workgroup of 256: a thread's partials in
@@ -411,6 +192,302 @@ __device__ __forceinline__ bool block_converged(const CGScal* s, int* flag) {
   if (threadIdx.x == 0) *flag = s->converged;
   __syncthreads();
   return *flag != 0;
+}
+
+// On-the-fly new direction for the fused small-system PCG: the product of
+// iteration k reads p = z + beta p_old for its columns (beta from the
+// partials of iteration k-1's update, reduced by every workgroup in the
+// order of k_cg_beta, with the convergence test of k_cg_beta) and writes p
+// of its own rows to the other buffer; the separate new-direction kernel
+// goes away.  The first product of a solve (fresh) takes beta = 0, p = z.
+struct OtfArgs {
+  const double* z;
+  const double* pold;
+  double* pnew;
+  CGScal* S;
+  int par;      // parity of the iteration of the product
+  int nbv;      // partial pairs of the previous update
+  const double* part2;
+};
+
+// beta and the convergence test of the previous iteration; false when the
+// solve has stopped (the product is skipped).  Workgroup-uniform.
+__device__ __forceinline__ bool otf_beta(const OtfArgs& o, double* sh, int* flag, double& beta) {
+  if (block_converged(o.S, flag)) return false;
+  if (o.S->fresh) {
+    beta = 0.0;
+    return true;
+  }
+  double rz, rr;
+  sum_partial_pairs(o.nbv, o.part2, sh, rz, rr);
+  const int bpar = o.par ^ 1;  // the previous iteration's parity
+  const int iter = o.S->iterp[bpar] + 1;
+  beta = rz / o.S->rzp[bpar];
+  const bool conv = rr <= o.S->tol2 * o.S->bb || iter >= o.S->maxit;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    o.S->iterp[o.par] = iter;
+    o.S->rr = rr;
+    o.S->beta = beta;
+    o.S->rzp[o.par] = rz;
+    if (conv) o.S->converged = 1;
+  }
+  return !conv;
+}
+
+// ---------------------------------------------------------------------------
+// CSR SpMV y = A x with L lanes per row; per-block partial of x.y
+// ---------------------------------------------------------------------------
+// One lane's share of a row: sum over k = b + lane, b + lane + L, ... < e of
+// vals[k] * x[col(raw(k), k)], in that order.  raw(k) is the stored column
+// index of entry k and col() forms the column from it.  The value and index
+// loads of U entries are issued together, then their x loads, then the
+// products, so a wave waits for memory twice per U entries rather than twice
+// per entry.  Entries past e load entry e - 1 again (in bounds, cached) and
+// add nothing.
+template <int L, int U = 4, class XR, class Raw, class Col>
+__device__ __forceinline__ double row_dot(int b, int e, int lane, const double* __restrict__ vals, XR xr, Raw raw,
+                                          Col col) {
+  double sum = 0.0;
+  for (int k0 = b + lane; k0 < e; k0 += U * L) {
+    double v[U], xv[U];
+    int r[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = min(k0 + u * L, e - 1);
+      v[u] = vals[k];
+      r[u] = raw(k);
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) xv[u] = xr(col(r[u], min(k0 + u * L, e - 1)));
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if (k0 + u * L < e) sum += v[u] * xv[u];
+  }
+  return sum;
+}
+
+template <int L, bool OTF = false>
+__global__ __launch_bounds__(256) void k_spmv_dot(int n, const int* __restrict__ rowptr,
+                                                  const int* __restrict__ colind,
+                                                  const double* __restrict__ vals,
+                                                  const double* __restrict__ x, double* __restrict__ y,
+                                                  double* __restrict__ partial,
+                                                  const int* __restrict__ stop, OtfArgs o) {
+  __shared__ double sh[256];
+  __shared__ int cflag;
+  double beta = 0.0;
+  if constexpr (OTF) {
+    if (!otf_beta(o, sh, &cflag, beta)) return;
+  } else {
+    if (stop && stop[0]) return;
+  }
+  auto xr = [&](int c) -> double {
+    if constexpr (OTF) return o.z[c] + beta * o.pold[c];
+    else return x[c];
+  };
+  const int t = threadIdx.x;
+  const int lane = t % L;
+  constexpr int R = 256 / L;  // rows per block and sweep
+  double d = 0.0;
+  // grid-stride over row groups: a fixed assignment, so the partial of each
+  // block (and the dot product) does not depend on timing
+  for (int row0 = blockIdx.x * R; row0 < n; row0 += gridDim.x * R) {
+    const int row = row0 + t / L;
+    double sum = 0.0;
+    if (row < n) {
+      const int b = rowptr[row], e = rowptr[row + 1];
+      sum = row_dot<L>(
+          b, e, lane, vals, xr, [&](int k) { return colind[k]; }, [](int c, int) { return c; });
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
+    if (row < n && lane == 0) {
+      y[row] = sum;
+      const double xrow = xr(row);
+      if constexpr (OTF) o.pnew[row] = xrow;
+      d += sum * xrow;
+    }
+  }
+  if (partial) {
+    const double bs = block_sum<256>(d, sh);
+    if (t == 0) partial[blockIdx.x] = bs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Node-blocked SpMV.  When every node owns M consecutive equations (all its
+// DOFs free, M = ndf) or none, equation M*J + r belongs to node block J and
+// its CSR row lists the neighbouring blocks' equations as ascending runs of
+// M, the same runs in each of the M rows of a block.  The columns then follow
+// from one block column per neighbour (nbcol, 4 bytes per M*M nonzeros
+// instead of 4 per nonzero) and rowptr[M*J] / (M*M) is block J's first entry
+// in nbcol; the values are the CSR values themselves.  Same lane layout,
+// reductions and dot-product partials as k_spmv_dot.
+// ---------------------------------------------------------------------------
+template <int L, int M, bool OTF = false>
+__global__ __launch_bounds__(256) void k_spmv_nb_dot(int n, const int* __restrict__ rowptr,
+                                                     const int* __restrict__ nbcol,
+                                                     const double* __restrict__ vals,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     double* __restrict__ partial,
+                                                     const int* __restrict__ stop, OtfArgs o) {
+  __shared__ double sh[256];
+  __shared__ int cflag;
+  double beta = 0.0;
+  if constexpr (OTF) {
+    if (!otf_beta(o, sh, &cflag, beta)) return;
+  } else {
+    if (stop && stop[0]) return;
+  }
+  auto xr = [&](int c) -> double {
+    if constexpr (OTF) return o.z[c] + beta * o.pold[c];
+    else return x[c];
+  };
+  const int t = threadIdx.x;
+  const int lane = t % L;
+  constexpr int R = 256 / L;
+  double d = 0.0;
+  for (int row0 = blockIdx.x * R; row0 < n; row0 += gridDim.x * R) {
+    const int row = row0 + t / L;
+    double sum = 0.0;
+    if (row < n) {
+      const int b = rowptr[row], e = rowptr[row + 1];
+      // the M rows of a block have the same length, so the block's first
+      // entry is b - (row % M) * (e - b)
+      const int* nc = nbcol + (b - (row % M) * (e - b)) / (M * M);
+      sum = row_dot<L>(
+          b, e, lane, vals, xr, [&](int k) { return nc[(k - b) / M]; },
+          [&](int c, int k) { return c * M + (k - b) % M; });
+    }
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) sum += __shfl_down(sum, off, L);
+    if (row < n && lane == 0) {
+      y[row] = sum;
+      const double xrow = xr(row);
+      if constexpr (OTF) o.pnew[row] = xrow;
+      d += sum * xrow;
+    }
+  }
+  if (partial) {
+    const double bs = block_sum<256>(d, sh);
+    if (t == 0) partial[blockIdx.x] = bs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block-symmetric SpMV.  The consistent tangents of this path are symmetric,
+// so a node-blocked matrix is kept a second time as its upper M x M blocks
+// only (block-major, row-major inside a block, copied from the CSR values
+// after each assembly): about half the bytes per product.  Block row I lists
+// its neighbours J ascending, each with the stored block that holds it:
+// (I, J) itself for J >= I, (J, I) for J < I, used transposed.  A stored
+// block is read twice per product, by its own row and by its partner's, a
+// few thousand block rows apart in an ascending sweep, so the second read
+// finds it in a cache.  Units of 256/G block rows are dealt out to the XCDs
+// in contiguous eighths (the hardware dispatches workgroup b to XCD b % 8)
+// and each workgroup takes a contiguous run of units, so that a block row's
+// lower partners, read shortly before by the same XCD, are more often found
+// in its L2 (4% faster at 64^3 than a plain grid-stride sweep).  G lanes per
+// block row, one neighbour block per lane and pass (its M*M values, column
+// and M entries of x loaded together), the M row sums reduced across the G
+// lanes.  (Staging a row's blocks in LDS with consecutive lanes reading
+// consecutive doubles measured 1.7x slower: the barrier between the copy and
+// the products and the LDS footprint cost more than the scattered loads.)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void xcd_units(int units, int& ua, int& ub) {
+  const int nx = min(8, (int)gridDim.x);
+  const int xcd = blockIdx.x % nx, l = blockIdx.x / nx;
+  const int gx = ((int)gridDim.x - xcd + nx - 1) / nx;  // workgroups of this XCD
+  const int u0 = (int)((long long)units * xcd / nx), u1 = (int)((long long)units * (xcd + 1) / nx);
+  const int cnt = (u1 - u0 + gx - 1) / gx;
+  ua = min(u1, u0 + l * cnt);
+  ub = min(u1, ua + cnt);
+}
+
+template <int M, int G, bool OTF = false>
+__global__ __launch_bounds__(256) void k_spmv_sym_dot(int nbr, const int* __restrict__ ptr,
+                                                      const int* __restrict__ col, const int* __restrict__ blk,
+                                                      const double* __restrict__ sv, const double* __restrict__ x,
+                                                      double* __restrict__ y, double* __restrict__ partial,
+                                                      const int* __restrict__ stop, OtfArgs o) {
+  constexpr int R = 256 / G;  // block rows per workgroup and unit
+  constexpr int MM = M * M;
+  __shared__ double sh[256];
+  __shared__ int cflag;
+  double beta = 0.0;
+  if constexpr (OTF) {
+    if (!otf_beta(o, sh, &cflag, beta)) return;
+  } else {
+    if (stop && stop[0]) return;
+  }
+  auto xr = [&](int c) -> double {
+    if constexpr (OTF) return o.z[c] + beta * o.pold[c];
+    else return x[c];
+  };
+  const int t = threadIdx.x;
+  const int lane = t % G;
+  double d = 0.0;
+  int ua, ub;
+  xcd_units((nbr + R - 1) / R, ua, ub);
+  for (int u = ua; u < ub; u++) {
+    const int I = u * R + t / G;
+    double acc[M];
+#pragma unroll
+    for (int r = 0; r < M; r++) acc[r] = 0.0;
+    if (I < nbr) {
+      const int b = ptr[I], e = ptr[I + 1];
+      for (int k = b + lane; k < e; k += G) {
+        const int J = col[k];
+        const double* a = sv + (long long)blk[k] * MM;
+        double v[MM], xj[M];
+#pragma unroll
+        for (int q = 0; q < MM; q++) v[q] = a[q];
+#pragma unroll
+        for (int c = 0; c < M; c++) xj[c] = xr(M * J + c);
+        const bool lower = J < I;
+#pragma unroll
+        for (int r = 0; r < M; r++)
+#pragma unroll
+          for (int c = 0; c < M; c++) acc[r] += (lower ? v[c * M + r] : v[r * M + c]) * xj[c];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < M; r++)
+#pragma unroll
+      for (int off = G / 2; off > 0; off >>= 1) acc[r] += __shfl_down(acc[r], off, G);
+    if (I < nbr && lane == 0) {
+#pragma unroll
+      for (int r = 0; r < M; r++) {
+        y[M * I + r] = acc[r];
+        const double xrow = xr(M * I + r);
+        if constexpr (OTF) o.pnew[M * I + r] = xrow;
+        d += acc[r] * xrow;
+      }
+    }
+  }
+  if (partial) {
+    const double bs = block_sum<256>(d, sh);
+    if (t == 0) partial[blockIdx.x] = bs;
+  }
+}
+
+// Stored block k = rows M*I.. of the CSR values from entry src[k] of row M*I
+// (the block's first column), len[k] entries per row of block row I.
+template <int M>
+__global__ __launch_bounds__(256) void k_sym_convert(int nup, const int* __restrict__ src,
+                                                     const int* __restrict__ len, const double* __restrict__ vals,
+                                                     double* __restrict__ sv) {
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < nup; k += gridDim.x * 256) {
+    const long long s0 = src[k];
+    const int L = len[k];
+    double v[M * M];
+#pragma unroll
+    for (int r = 0; r < M; r++)
+#pragma unroll
+      for (int c = 0; c < M; c++) v[r * M + c] = vals[s0 + (long long)r * L + c];
+#pragma unroll
+    for (int q = 0; q < M * M; q++) sv[(long long)k * (M * M) + q] = v[q];
+  }
 }
 
 // alpha = r.z / p.q from the SpMV's partials of p.q (one workgroup).
@@ -518,6 +595,7 @@ __global__ __launch_bounds__(256) void k_cg_update_fused(int n, CGScal* __restri
   const double alpha = bad ? 0.0 : s->rzp[par] / pq;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     s->pq = pq;
+    s->fresh = 0;
     if (bad) {
       s->breakdown = 1;
       s->converged = 1;
@@ -551,32 +629,6 @@ __global__ void k_cg_newp(int n, const CGScal* __restrict__ s, const double* __r
   if (s->converged) return;
   const double beta = s->beta;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = z[i] + beta * p[i];
-}
-
-// k_cg_beta fused into the new direction: every workgroup sums the update's
-// partials itself (same order, same bits as k_cg_beta), applies the same
-// convergence test and, unless converged, forms its rows of p; workgroup 0
-// records the scalars.
-__global__ __launch_bounds__(256) void k_cg_newp_fused(int n, CGScal* __restrict__ s, int par, int nbv,
-                                                       const double* __restrict__ part,
-                                                       const double* __restrict__ z, double* __restrict__ p) {
-  __shared__ double sh[256];
-  __shared__ int cflag;
-  if (block_converged(s, &cflag)) return;
-  double rz, rr;
-  sum_partial_pairs(nbv, part, sh, rz, rr);
-  const int iter = s->iterp[par] + 1;
-  const double beta = rz / s->rzp[par];
-  const bool conv = rr <= s->tol2 * s->bb || iter >= s->maxit;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    s->iterp[par ^ 1] = iter;
-    s->rr = rr;
-    s->beta = beta;
-    s->rzp[par ^ 1] = rz;
-    if (conv) s->converged = 1;
-  }
-  if (conv) return;
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = z[i] + beta * p[i];
 }
 
 // Start: x = 0, r = b, z = M^{-1} b, p = z; partials of r.z, b.b
@@ -631,6 +683,7 @@ __global__ __launch_bounds__(256) void k_cg_init_scal(int nb, const double* __re
     s->tol2 = tol2;
     s->maxit = maxit;
     s->iterp[0] = s->iterp[1] = 0;
+    s->fresh = 1;
     s->breakdown = 0;
     s->alpha = s->beta = s->pq = 0.0;
     s->converged = (bb == 0.0) ? 1 : 0;
@@ -769,6 +822,7 @@ __global__ __launch_bounds__(kSmallPcgThreads) void k_pcg_small(
     S->tol2 = tol2;
     S->maxit = maxit;
     S->iterp[0] = S->iterp[1] = it;
+    S->fresh = 0;
     S->breakdown = brk;
     S->converged = 1;
   }
@@ -960,38 +1014,67 @@ size_t cg_part_size(int n) {
   n = std::max(n, 1);
   return 2 * (size_t)std::max({ceil_div(n, 256), spmv_blocks(n, 64), kCgFuseMax}) + kCgFuseMax + 16;
 }
-template <int M>
-static void launch_spmv_nb(hipStream_t s, int lanes, int n, const int* rowptr, const int* nbcol,
-                           const double* vals, const double* x, double* y, double* partial, const int* stop) {
+// SpMV launches; otf != nullptr: the fused PCG's product with the new
+// direction formed on the fly (OtfArgs)
+template <bool OTF, int M>
+static void launch_spmv_nb_t(hipStream_t s, int lanes, int n, const int* rowptr, const int* nbcol,
+                             const double* vals, const double* x, double* y, double* partial, const int* stop,
+                             const OtfArgs& o) {
   const dim3 g(spmv_blocks(n, lanes)), b(256);
   switch (lanes) {
-    case 4: hipLaunchKernelGGL((k_spmv_nb_dot<4, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop); break;
-    case 8: hipLaunchKernelGGL((k_spmv_nb_dot<8, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop); break;
-    case 16:
-      hipLaunchKernelGGL((k_spmv_nb_dot<16, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop);
-      break;
-    case 32:
-      hipLaunchKernelGGL((k_spmv_nb_dot<32, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop);
-      break;
-    default:
-      hipLaunchKernelGGL((k_spmv_nb_dot<64, M>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop);
-      break;
+#define XG_NB(L)                                                                                           \
+  case L:                                                                                                 \
+    hipLaunchKernelGGL((k_spmv_nb_dot<L, M, OTF>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop, o); \
+    break;
+    XG_NB(4) XG_NB(8) XG_NB(16) XG_NB(32)
+#undef XG_NB
+    default: hipLaunchKernelGGL((k_spmv_nb_dot<64, M, OTF>), g, b, 0, s, n, rowptr, nbcol, vals, x, y, partial, stop,
+                                o); break;
   }
 }
 
-template <int M>
-static void launch_spmv_sym(hipStream_t s, const SymSpmv& S, const double* x, double* y, double* partial,
-                            const int* stop) {
+template <bool OTF, int M>
+static void launch_spmv_sym_t(hipStream_t s, const SymSpmv& S, const double* x, double* y, double* partial,
+                              const int* stop, const OtfArgs& o) {
   const dim3 g(spmv_grid(S.nbr * M, S.lanes, &S)), b(256);
   switch (S.lanes) {
-#define XG_SYM(G)                                                                                         \
-  case G:                                                                                                \
-    hipLaunchKernelGGL((k_spmv_sym_dot<M, G>), g, b, 0, s, S.nbr, S.ptr, S.col, S.blk, S.val, x, y, partial, \
-                       stop);                                                                            \
+#define XG_SYM(G)                                                                                              \
+  case G:                                                                                                     \
+    hipLaunchKernelGGL((k_spmv_sym_dot<M, G, OTF>), g, b, 0, s, S.nbr, S.ptr, S.col, S.blk, S.val, x, y, partial, \
+                       stop, o);                                                                              \
     break;
     XG_SYM(4) XG_SYM(8) XG_SYM(16) XG_SYM(32) XG_SYM(64)
 #undef XG_SYM
     default: throw Error(XG_ERR_ARG, "symmetric SpMV lanes must be 4, 8, 16, 32 or 64");
+  }
+}
+
+template <bool OTF>
+static void launch_spmv_t(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
+                          const double* x, double* y, double* partial, const int* stop, int nbm, const int* nbcol,
+                          const SymSpmv* sym, const OtfArgs& o) {
+  if (n <= 0) return;
+  if (sym && sym->m) {
+    if (sym->m == 2) return launch_spmv_sym_t<OTF, 2>(s, *sym, x, y, partial, stop, o);
+    if (sym->m == 3) return launch_spmv_sym_t<OTF, 3>(s, *sym, x, y, partial, stop, o);
+    throw Error(XG_ERR_ARG, "symmetric SpMV block size must be 2 or 3");
+  }
+  switch (nbm) {
+    case 2: return launch_spmv_nb_t<OTF, 2>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop, o);
+    case 3: return launch_spmv_nb_t<OTF, 3>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop, o);
+    case 6: return launch_spmv_nb_t<OTF, 6>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop, o);
+    default: break;
+  }
+  const dim3 g(spmv_blocks(n, lanes)), b(256);
+  switch (lanes) {
+#define XG_CSR(L)                                                                                       \
+  case L:                                                                                              \
+    hipLaunchKernelGGL((k_spmv_dot<L, OTF>), g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop, o); \
+    break;
+    XG_CSR(4) XG_CSR(8) XG_CSR(16) XG_CSR(32)
+#undef XG_CSR
+    default: hipLaunchKernelGGL((k_spmv_dot<64, OTF>), g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop, o);
+             break;
   }
 }
 
@@ -1012,26 +1095,7 @@ void launch_sym_convert(hipStream_t s, const SymSpmv& S, int nup, const int* src
 void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
                  const double* x, double* y, double* partial, const int* stop, int nbm, const int* nbcol,
                  const SymSpmv* sym) {
-  if (n <= 0) return;
-  if (sym && sym->m) {
-    if (sym->m == 2) return launch_spmv_sym<2>(s, *sym, x, y, partial, stop);
-    if (sym->m == 3) return launch_spmv_sym<3>(s, *sym, x, y, partial, stop);
-    throw Error(XG_ERR_ARG, "symmetric SpMV block size must be 2 or 3");
-  }
-  switch (nbm) {
-    case 2: return launch_spmv_nb<2>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop);
-    case 3: return launch_spmv_nb<3>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop);
-    case 6: return launch_spmv_nb<6>(s, lanes, n, rowptr, nbcol, vals, x, y, partial, stop);
-    default: break;
-  }
-  const dim3 g(spmv_blocks(n, lanes)), b(256);
-  switch (lanes) {
-    case 4: hipLaunchKernelGGL(k_spmv_dot<4>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
-    case 8: hipLaunchKernelGGL(k_spmv_dot<8>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
-    case 16: hipLaunchKernelGGL(k_spmv_dot<16>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
-    case 32: hipLaunchKernelGGL(k_spmv_dot<32>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
-    default: hipLaunchKernelGGL(k_spmv_dot<64>, g, b, 0, s, n, rowptr, colind, vals, x, y, partial, stop); break;
-  }
+  launch_spmv_t<false>(s, lanes, n, rowptr, colind, vals, x, y, partial, stop, nbm, nbcol, sym, OtfArgs{});
 }
 void launch_block_inverse(hipStream_t s, int nblk, const int* first, const unsigned char* size,
                           const int* rowptr, const int* colind, const double* vals, int bw, double* binv,
@@ -1079,7 +1143,7 @@ void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, doubl
 
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
-                   double* p, double* q, int bw, const double* binv, const int* bfirst,
+                   double* p, double* p2, double* q, int bw, const double* binv, const int* bfirst,
                    const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol, bool fuse,
                    const SymSpmv* sym) {
   CGScal* S = (CGScal*)scal;
@@ -1094,19 +1158,24 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
     const int par = (it0 + it) & 1;
     double* ri = par ? r1 : r0;
     double* ro = par ? r0 : r1;
-    launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop, nbm, nbcol, sym);
     if (fuse) {
+      // two launches: the product forming the new direction on the fly
+      // (p alternates between p and p2), then the update with alpha
+      double* pold = par ? p2 : p;
+      double* pnew = par ? p : p2;
+      const OtfArgs o{z, pold, pnew, S, par, nbv, part2};
+      launch_spmv_t<true>(s, lanes, n, rowptr, colind, vals, pold, q, part, stop, nbm, nbcol, sym, o);
       switch (bw) {
-#define XG_CGU(W)                                                                                                \
-  case W:                                                                                                       \
-    hipLaunchKernelGGL(k_cg_update_fused<W>, dim3(nbv), dim3(256), 0, s, n, S, par, nbs, part, p, q, x, ri, ro, z, \
-                       binv, bfirst, bsize, part2);                                                             \
+#define XG_CGU(W)                                                                                                  \
+  case W:                                                                                                         \
+    hipLaunchKernelGGL(k_cg_update_fused<W>, dim3(nbv), dim3(256), 0, s, n, S, par, nbs, part, pnew, q, x, ri, ro, z, \
+                       binv, bfirst, bsize, part2);                                                               \
     break;
         XG_CGU(1) XG_CGU(2) XG_CGU(3) XG_CGU(4) XG_CGU(5) XG_CGU(6)
 #undef XG_CGU
       }
-      hipLaunchKernelGGL(k_cg_newp_fused, dim3(nbv), dim3(256), 0, s, n, S, par, nbv, part2, z, p);
     } else {
+      launch_spmv(s, lanes, n, rowptr, colind, vals, p, q, part, stop, nbm, nbcol, sym);
       hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, nbs, part, S, par);
       switch (bw) {
 #define XG_CGU(W)                                                                                              \
@@ -1123,10 +1192,6 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
   }
 }
 
-// One CU's memory parallelism bounds this kernel: it beats the launch-per-
-// step PCG on models of a few hundred equations (2D frames of an ensemble)
-// and loses from about two thousand on (a 64 x 16 quad strip, a 5 x 5 x 10
-// frame), so only systems of up to 1,024 equations and 32,768 nonzeros.
 bool cg_small_fits(int n, long long nnz) { return n > 0 && n <= 1024 && nnz <= 32768; }
 
 void cg_solve_small(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
