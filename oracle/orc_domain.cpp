@@ -803,6 +803,17 @@ static int analyze(Dom& d, int nsteps, double dt) {
     form_unbalance(d);
     d.phat = d.B;
     d.lambda -= 1.0;
+    // check there is a reference load (DisplacementControl.cpp:386-398)
+    bool have_load = false;
+    for (double v : d.phat)
+      if (v != 0.0) {
+        have_load = true;
+        break;
+      }
+    if (!have_load) {
+      d.phat.clear();
+      return fail(XG_ERR_ARG, "DisplacementControl: zero reference load");
+    }
     auto it = d.node_index.find(d.dc_node);
     if (it == d.node_index.end()) return fail(XG_ERR_ARG, "control node not found");
     d.dc_dofid = d.nodes[it->second].eq[d.dc_dof];

@@ -18,7 +18,18 @@
 #include <Matrix.h>
 #include <Vector.h>
 #include <ID.h>
+#include <Domain.h>
+#include <Node.h>
+#include <Brick.h>
+#include <FourNodeQuad.h>
+#include <DispBeamColumn2d.h>
+#include <DispBeamColumn3d.h>
+#include <LinearCrdTransf2d.h>
+#include <LinearCrdTransf3d.h>
+#include <FrameQuadrature.hpp>
+#include <GaussLegendre1D.hpp>
 
+#include <array>
 #include <vector>
 
 extern "C" {
@@ -147,6 +158,119 @@ int ref_rcm(int nv, const int* ptr, const int* adj, int* order) {
   const ID& res = rcm.number(g, -1);
   for (int i = 0; i < nv; i++) order[i] = res(i);
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Whole elements over the reference's own Node / Domain / CrdTransf classes.
+// Each call builds a fresh Domain (left allocated: the harness is short
+// lived), adds the element's nodes and the element (Domain::addElement sets
+// the domain and runs the first update), then walks n states of the
+// element's nodal trial displacements U (n x element DOFs, node by node):
+// Node::setTrialDisp, Element::update, getTangentStiff, getResistingForce;
+// action[i] = 1 commits nodes and element after step i.  K is written
+// row-major (nd x nd per state), R (nd per state).  Returns the number of
+// failed updates.
+// ---------------------------------------------------------------------------
+static int walk_element(Domain* dom, Element* e, int nen, int ndf, int n, const double* U, const int* action,
+                        double* K, double* R) {
+  const int nd = nen * ndf;
+  int bad = 0;
+  Vector u(ndf);
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < nen; a++) {
+      for (int j = 0; j < ndf; j++) u(j) = U[(size_t)i * nd + a * ndf + j];
+      dom->getNode(a + 1)->setTrialDisp(u);
+    }
+    if (e->update() != 0) bad++;
+    const Matrix& k = e->getTangentStiff();
+    for (int r = 0; r < nd; r++)
+      for (int c = 0; c < nd; c++) K[(size_t)i * nd * nd + r * nd + c] = k(r, c);
+    const Vector& f = e->getResistingForce();
+    for (int r = 0; r < nd; r++) R[(size_t)i * nd + r] = f(r);
+    if (action[i] == 1) {
+      for (int a = 0; a < nen; a++) dom->getNode(a + 1)->commitState();
+      e->commitState();
+    }
+  }
+  return bad;
+}
+
+// stdBrick (Brick.cpp) over J2 (prm = K G sig0 sigInf delta H eta), nodes
+// crd (8 x 3), body force b (3).
+int ref_brick_path(const double* crd, const double* prm, const double* b, int n, const double* U,
+                   const int* action, double* K, double* R) {
+  Domain* dom = new Domain();
+  for (int a = 0; a < 8; a++) dom->addNode(new Node(a + 1, 3, crd[3 * a], crd[3 * a + 1], crd[3 * a + 2]));
+  J2ThreeDimensional mat(1, prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], 0.0);
+  Brick* e = new Brick(1, 1, 2, 3, 4, 5, 6, 7, 8, mat, b[0], b[1], b[2]);
+  dom->addElement(e);
+  return walk_element(dom, e, 8, 3, n, U, action, K, R);
+}
+
+// quad (FourNodeQuad.cpp) over J2PlaneStrain, nodes crd (4 x 2), thickness,
+// body force b (2).
+int ref_quad_path(const double* crd, const double* prm, double thk, const double* b, int n, const double* U,
+                  const int* action, double* K, double* R) {
+  Domain* dom = new Domain();
+  for (int a = 0; a < 4; a++) dom->addNode(new Node(a + 1, 2, crd[2 * a], crd[2 * a + 1]));
+  J2PlaneStrain mat(1, prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], 0.0);
+  std::array<int, 4> nodes{1, 2, 3, 4};
+  FourNodeQuad* e = new FourNodeQuad(1, nodes, mat, thk, 0.0, 0.0, b[0], b[1]);
+  dom->addElement(e);
+  return walk_element(dom, e, 4, 2, n, U, action, K, R);
+}
+
+// dispBeamColumn (DispBeamColumn2d.cpp / DispBeamColumn3d.cpp) with
+// `nip` Gauss-Legendre sections (FrameQuadrature<GaussLegendre<1, nip>>,
+// nip = 2..5) of one fiber section (fibers as ref_fiber_section_path),
+// geomTransf Linear (vecxz used in 3D), nodes crd (2 x ndm).
+int ref_dispbeam_path(int is3d, const double* crd, const double* vecxz, int nip, int nfib, const int* kinds,
+                      const double* prm, const double* y, const double* z, const double* A, double GJ,
+                      int compute_centroid, int n, const double* U, const int* action, double* K, double* R) {
+  Domain* dom = new Domain();
+  const int ndf = is3d ? 6 : 3;
+  for (int a = 0; a < 2; a++) {
+    if (is3d) dom->addNode(new Node(a + 1, 6, crd[3 * a], crd[3 * a + 1], crd[3 * a + 2]));
+    else dom->addNode(new Node(a + 1, 3, crd[2 * a], crd[2 * a + 1]));
+  }
+  SectionForceDeformation* sec;
+  if (is3d) {
+    ElasticMaterial torsion(99, GJ);
+    FiberSection3d* s3 = new FiberSection3d(1, nfib, torsion, compute_centroid != 0);
+    for (int f = 0; f < nfib; f++) {
+      UniaxialMaterial* m = make_uniaxial(kinds[f], prm + 12 * f);
+      s3->addFiber(*m, A[f], y[f], z[f]);
+    }
+    sec = s3;
+  } else {
+    FiberSection2d* s2 = new FiberSection2d(1, nfib, compute_centroid != 0);
+    for (int f = 0; f < nfib; f++) {
+      UniaxialMaterial* m = make_uniaxial(kinds[f], prm + 12 * f);
+      s2->addFiber(*m, A[f], y[f]);
+    }
+    sec = s2;
+  }
+  BeamIntegration* bi = nullptr;
+  switch (nip) {
+    case 2: bi = new FrameQuadrature<GaussLegendre<1, 2>>(); break;
+    case 3: bi = new FrameQuadrature<GaussLegendre<1, 3>>(); break;
+    case 4: bi = new FrameQuadrature<GaussLegendre<1, 4>>(); break;
+    case 5: bi = new FrameQuadrature<GaussLegendre<1, 5>>(); break;
+    default: return -1;
+  }
+  std::vector<SectionForceDeformation*> secs(nip, sec);
+  Element* e;
+  if (is3d) {
+    Vector v(3);
+    for (int k = 0; k < 3; k++) v(k) = vecxz[k];
+    LinearCrdTransf3d tr(1, v);
+    e = new DispBeamColumn3d(1, 1, 2, nip, secs.data(), *bi, tr);
+  } else {
+    LinearCrdTransf2d tr(1);
+    e = new DispBeamColumn2d(1, 1, 2, nip, secs.data(), *bi, tr);
+  }
+  dom->addElement(e);
+  return walk_element(dom, e, 2, ndf, n, U, action, K, R);
 }
 
 }  // extern "C"

@@ -96,6 +96,10 @@ class Ref:
             "ref_fiber_section_path": [C.c_int, C.c_int, P_I, P_D, P_D, P_D, P_D, C.c_double, C.c_int,
                                        C.c_int, P_D, P_I, P_D, P_D],
             "ref_rcm": [C.c_int, P_I, P_I, P_I],
+            "ref_brick_path": [P_D, P_D, P_D, C.c_int, P_D, P_I, P_D, P_D],
+            "ref_quad_path": [P_D, P_D, C.c_double, P_D, C.c_int, P_D, P_I, P_D, P_D],
+            "ref_dispbeam_path": [C.c_int, P_D, P_D, C.c_int, C.c_int, P_I, P_D, P_D, P_D, P_D, C.c_double,
+                                  C.c_int, C.c_int, P_D, P_I, P_D, P_D],
         }.items():
             f = getattr(lib, name)
             f.argtypes = args

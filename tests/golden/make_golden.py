@@ -3,8 +3,9 @@
 Runs in the build container only: it loads oracle/_ref/libxararef.so (built
 by `make -C oracle ref` from the unmodified reference material, section and
 graph translation units) and records, for seeded strain paths and graphs,
-the stresses, tangents, section resultants and RCM orders the reference
-computes.  The fixtures are data only (inputs and expected outputs).
+the stresses, tangents, section resultants, RCM orders and whole-element
+tangents and resisting forces (Brick, FourNodeQuad, DispBeamColumn2d/3d)
+the reference computes.  The fixtures are data only (inputs and expected outputs).
 Usage: python tests/golden/make_golden.py
 """
 import os
@@ -129,6 +130,109 @@ def rcm_case(ref, adj):
     return ptr, flat, order
 
 
+# ---- whole elements (Brick, FourNodeQuad, DispBeamColumn2d/3d) ----------
+BRICK_CRD = np.array([[0, 0, 0], [1.1, 0.05, 0], [1.0, 1.2, 0.1], [-0.1, 0.9, 0], [0.05, 0, 1.0],
+                      [1.0, -0.1, 1.1], [1.2, 1.0, 0.9], [0, 1.1, 1.05]], float)
+QUAD_CRD = np.array([[0.0, 0.0], [1.3, 0.1], [1.1, 0.9], [-0.2, 1.2]])
+BEAM2D_CRD = np.array([[0.0, 0.0], [3000.0, 400.0]])
+BEAM3D_CRD = np.array([[0.0, 0.0, 0.0], [500.0, 300.0, 3000.0]])
+BEAM3D_VECXZ = np.array([1.0, 0.2, 0.0])
+# nodes, DOFs per node, per-DOF step scale of the random-walk paths
+ELEMENT_CASES = {"brick": (8, 3), "quad": (4, 2), "beam2d": (2, 3), "beam3d": (2, 6)}
+
+
+def element_fibers():
+    """A small asymmetric RC section: 6 x 4 concrete fibers, 3 bars on the
+    +z face and 2 on the -z face (kind 1 Steel02, 2 Concrete02)."""
+    yy, zz = np.meshgrid(np.linspace(-175, 175, 6), np.linspace(-150, 150, 4))
+    y, z = list(yy.ravel()), list(zz.ravel())
+    A, kinds = [4800.0] * 24, [2] * 24
+    for yb, zb in ((-160, 170), (0, 170), (160, 170), (-160, -170), (160, -170)):
+        y.append(float(yb))
+        z.append(float(zb))
+        A.append(510.0)
+        kinds.append(1)
+    kinds = np.array(kinds, np.int32)
+    prm = np.zeros((len(kinds), 12))
+    prm[kinds == 1] = STEEL
+    prm[kinds == 2] = CONC
+    return kinds, prm, np.array(y), np.array(z), np.array(A)
+
+
+def element_inputs(case, seed, n=10):
+    """Seeded random-walk nodal displacement path (n states) and commits."""
+    nen, ndf = ELEMENT_CASES[case]
+    rng = np.random.default_rng(1000 + 17 * seed + len(case))
+    if case in ("brick", "quad"):
+        step = np.full(nen * ndf, 1.2e-3)
+    else:
+        per = [1.0, 1.0, 2.5e-4] if ndf == 3 else [1.0, 1.0, 1.0, 2.5e-4, 2.5e-4, 2.5e-4]
+        step = np.tile(per, nen)
+    U = np.ascontiguousarray(np.cumsum(rng.normal(size=(n, nen * ndf)) * step, axis=0))
+    act = (rng.random(n) < 0.6).astype(np.int32)
+    return U, act
+
+
+def element_model(factory, case):
+    """The single-element model of `case` through the Model command API
+    (factory(ndm, ndf) -> Model): nodes 1..nen, element 1."""
+    nen, ndf = ELEMENT_CASES[case]
+    if case == "brick":
+        m = factory(3, 3)
+        for a in range(8):
+            m.node(a + 1, *BRICK_CRD[a])
+        m.nDMaterial("J2Plasticity", 1, *J2[:6])
+        m.element("stdBrick", 1, *range(1, 9), 1)
+    elif case == "quad":
+        m = factory(2, 2)
+        for a in range(4):
+            m.node(a + 1, *QUAD_CRD[a])
+        m.nDMaterial("J2Plasticity", 1, *J2[:6])
+        m.element("quad", 1, 1, 2, 3, 4, 1.3, "PlaneStrain", 1)
+    else:
+        is3d = case == "beam3d"
+        m = factory(3, 6) if is3d else factory(2, 3)
+        crd = BEAM3D_CRD if is3d else BEAM2D_CRD
+        for a in range(2):
+            m.node(a + 1, *crd[a])
+        m.uniaxialMaterial("Steel02", 1, *STEEL[:11])
+        m.uniaxialMaterial("Concrete02", 2, *CONC[:7])
+        kinds, _, y, z, A = element_fibers()
+        m.section("Fiber", 1, [(y[f], z[f], A[f], int(kinds[f])) for f in range(len(kinds))],
+                  GJ=1.0e12 if is3d else None)
+        if is3d:
+            m.geomTransf("Linear", 1, *BEAM3D_VECXZ)
+        else:
+            m.geomTransf("Linear", 1)
+        m.element("dispBeamColumn", 1, 1, 2, 1, "Legendre", 1, 5)
+    return m, nen, ndf
+
+
+def run_ref_element(ref, case, U, act):
+    """The same path through the reference's own element classes."""
+    nen, ndf = ELEMENT_CASES[case]
+    n, nd = len(U), nen * ndf
+    U = np.ascontiguousarray(U, np.float64)
+    act = np.ascontiguousarray(act, np.int32)
+    K, R = np.zeros((n, nd, nd)), np.zeros((n, nd))
+    if case == "brick":
+        rc = ref.brick_path(dp(np.ascontiguousarray(BRICK_CRD.ravel())), dp(J2), dp(np.zeros(3)), n, dp(U),
+                            ip(act), dp(K), dp(R))
+    elif case == "quad":
+        rc = ref.quad_path(dp(np.ascontiguousarray(QUAD_CRD.ravel())), dp(J2), 1.3, dp(np.zeros(2)), n, dp(U),
+                           ip(act), dp(K), dp(R))
+    else:
+        is3d = case == "beam3d"
+        kinds, prm, y, z, A = element_fibers()
+        crd = np.ascontiguousarray((BEAM3D_CRD if is3d else BEAM2D_CRD).ravel())
+        rc = ref.dispbeam_path(int(is3d), dp(crd), dp(BEAM3D_VECXZ), 5, len(kinds), ip(kinds),
+                               dp(np.ascontiguousarray(prm)), dp(y), dp(z), dp(A), 1.0e12 if is3d else -1.0, 1, n,
+                               dp(U), ip(act), dp(K), dp(R))
+    if rc != 0:
+        raise RuntimeError(f"reference {case} path: {rc} failed updates")
+    return K, R
+
+
 def main():
     ref = _oracle.load_ref()
     if ref is None:
@@ -151,6 +255,10 @@ def main():
     for i, adj in enumerate(graph_cases()):
         ptr, flat, order = rcm_case(ref, adj)
         d[f"rcm{i}_ptr"], d[f"rcm{i}_adj"], d[f"rcm{i}_order"] = ptr, flat, order
+    for case in ELEMENT_CASES:
+        U, act = element_inputs(case, 7)
+        K, R = run_ref_element(ref, case, U, act)
+        d[f"ele_{case}_U"], d[f"ele_{case}_act"], d[f"ele_{case}_K"], d[f"ele_{case}_R"] = U, act, K, R
     np.savez_compressed(OUT, **d)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(d)} arrays)")
 

@@ -86,7 +86,7 @@ class FrameTransient:
 class _TransientLoss(torch.autograd.Function):
     @staticmethod
     def forward(ctx, theta, problem):
-        th = theta.detach().to("cpu", torch.float64).numpy()
+        th = theta.detach().to("cpu", torch.float64).numpy().copy()  # own copy: theta may change in place
         loss = problem.run(th)
         ctx.problem, ctx.token, ctx.theta = problem, problem.token, th
         return torch.tensor(loss, dtype=theta.dtype, device=theta.device)

@@ -43,3 +43,45 @@ struct J2Param {
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace xg
+
+namespace xg {
+
+// Records the launches `body` issues on stream s into *graph and
+// instantiates it into *exec.  If body (or the capture itself) fails, the
+// capture is ended and the partial graph destroyed before the error
+// propagates, so the stream never stays in capture mode.
+template <class F>
+void capture_graph(hipStream_t s, F&& body, hipGraph_t* graph, hipGraphExec_t* exec) {
+  *graph = nullptr;
+  *exec = nullptr;
+  XG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    hipGraph_t partial = nullptr;
+    (void)hipStreamEndCapture(s, &partial);
+    if (partial) (void)hipGraphDestroy(partial);
+    throw;
+  }
+  XG_HIP(hipStreamEndCapture(s, graph));
+  const hipError_t e = hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(*graph);
+    *graph = nullptr;
+    *exec = nullptr;
+    XG_HIP(e);
+  }
+}
+
+// Destroys an instantiated graph after the work queued on stream s (which
+// may include launches of it) has finished.
+inline void destroy_graph(hipStream_t s, hipGraph_t& graph, hipGraphExec_t& exec) {
+  if (!graph && !exec) return;
+  if (s) (void)hipStreamSynchronize(s);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+}
+
+}  // namespace xg

@@ -649,11 +649,10 @@ void Domain::precond_setup() {
   launch_block_inverse(stream, nblk, blk_first.p, blk_size.p, rowptr.p, colind.p, vals.p, bw, binv.p, pcbad.p);
 }
 
+// Destroys the captured chunks once the stream has drained: a replay still
+// queued or running must not outlive its executable graph.
 void Domain::cg_graph_clear() {
-  for (auto& g : cg_graphs) {
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (g.graph) (void)hipGraphDestroy(g.graph);
-  }
+  for (auto& g : cg_graphs) destroy_graph(stream, g.graph, g.exec);
   cg_graphs.clear();
 }
 
@@ -690,16 +689,12 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
     eager(stream, iters, it0);
     return;
   }
+  // the launch checks of cg_iterations run once outside the capture, so a
+  // bad argument throws before the stream enters capture mode
+  cg_iterations_check(bw, sym);
   CgGraph g;
   g.key = key;
-  XG_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-  eager(stream, iters, 0);
-  XG_HIP(hipStreamEndCapture(stream, &g.graph));
-  const hipError_t e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
-    (void)hipGraphDestroy(g.graph);
-    XG_HIP(e);
-  }
+  capture_graph(stream, [&] { eager(stream, iters, 0); }, &g.graph, &g.exec);
   cg_graphs.push_back(g);
   XG_HIP(hipGraphLaunch(g.exec, stream));
 }

@@ -366,13 +366,10 @@ void Ensemble::setup() {
   ready = true;
 }
 
+// The step graphs are destroyed once the stream has drained (a replay still
+// queued must not outlive its executable graph).
 void Ensemble::drop_graph() {
-  for (int k = 0; k < 2; k++) {
-    if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
-    if (graph[k]) (void)hipGraphDestroy(graph[k]);
-    graph_exec[k] = nullptr;
-    graph[k] = nullptr;
-  }
+  for (int k = 0; k < 2; k++) destroy_graph(merged.stream, graph[k], graph_exec[k]);
   graph_dt = -1;
 }
 
@@ -419,10 +416,7 @@ int Ensemble::analyze(int nsteps, double dt) {
   if (use_graph && !graph_exec[parity]) {
     // capture both parities (capture records, it runs nothing)
     for (int k = 0; k < 2; k++) {
-      XG_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      enqueue_step(s);
-      XG_HIP(hipStreamEndCapture(s, &graph[parity]));
-      XG_HIP(hipGraphInstantiate(&graph_exec[parity], graph[parity], nullptr, nullptr, 0));
+      capture_graph(s, [&] { enqueue_step(s); }, &graph[parity], &graph_exec[parity]);
       flip();
     }
   }

@@ -84,6 +84,9 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
                    const unsigned char* bsize, double* part, void* scal, int nbm, const int* nbcol,
                    bool fuse,  // fuse: alpha / beta reductions in the update / new-direction kernels when small
                    const SymSpmv* sym = nullptr);
+// throws XG_ERR_ARG for a block width or symmetric layout cg_iterations
+// cannot launch (called before a graph capture of cg_iterations)
+void cg_iterations_check(int bw, const SymSpmv* sym);
 // whole solve in one workgroup (k_pcg_small) for systems that fit it
 bool cg_small_fits(int n, long long nnz);
 void cg_solve_small(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,

@@ -1141,6 +1141,16 @@ void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, doubl
   hipLaunchKernelGGL(k_cg_init_scal, dim3(1), dim3(256), 0, s, nb, part, tol2, maxit, (CGScal*)scal);
 }
 
+void cg_iterations_check(int bw, const SymSpmv* sym) {
+  if (bw < 1 || bw > 6) throw Error(XG_ERR_ARG, "PCG block width must be 1..6");
+  if (sym && sym->m) {
+    if (sym->m != 2 && sym->m != 3) throw Error(XG_ERR_ARG, "symmetric SpMV block size must be 2 or 3");
+    const int g = sym->lanes;
+    if (g != 4 && g != 8 && g != 16 && g != 32 && g != 64)
+      throw Error(XG_ERR_ARG, "symmetric SpMV lanes must be 4, 8, 16, 32 or 64");
+  }
+}
+
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
                    double* p, double* p2, double* q, int bw, const double* binv, const int* bfirst,
@@ -1153,7 +1163,7 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
   const int* stop = &S->converged;
   fuse = fuse && nbs <= kCgFuseMax && nbv <= kCgFuseMax;
   double* part2 = part + kCgFuseMax;
-  if (bw < 1 || bw > 6) throw Error(XG_ERR_ARG, "PCG block width must be 1..6");
+  cg_iterations_check(bw, sym);
   for (int it = 0; it < iters; it++) {
     const int par = (it0 + it) & 1;
     double* ri = par ? r1 : r0;
