@@ -25,6 +25,13 @@ J2_SAT = np.array([166.7e3, 76.9e3, 250.0, 400.0, 12.0, 500.0, 0.0])
 STEEL = np.array([420.0, 200e3, 0.01, 18.0, 0.925, 0.15, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0])
 STEEL_ISO = np.array([420.0, 200e3, 0.02, 15.0, 0.925, 0.15, 0.05, 1.0, 0.05, 1.0, 0.0, 0.0])
 CONC = np.array([-30.0, -0.002, -6.0, -0.02, 0.1, 3.0, 1500.0, 0, 0, 0, 0, 0])
+# Saturation softening so steep (delta 3000, sigma_inf 50 < sigma_0 250) that
+# the local Newton iteration on the plastic multiplier needs more than 25
+# iterations once the trial deviatoric stress norm passes about 2,500
+# (J2Plasticity.cpp:309-313): from about 3,000 on it is still far from its
+# root at the 26th iteration, while norms up to about 2,000 converge in at
+# most 22 iterations.
+J2_FAIL = np.array([166.7e3, 76.9e3, 250.0, 50.0, 3000.0, 2000.0, 0.0])
 
 
 def _p(a, t):
@@ -51,6 +58,36 @@ def j2_case(ref, prm, n, w, seed, scale):
     fn = ref.j2_3d_path if w == 6 else ref.j2_ps_path
     fn(dp(prm), n, dp(eps), ip(act), dp(s), dp(t))
     return eps, act, s, t
+
+
+def j2_fail_path(w, seed):
+    """Strain path for J2_FAIL that keeps the committed hardening variable
+    at zero (the failure needs the steep start of the saturation curve):
+    steps cycle through an elastic state (trial deviatoric stress norm
+    50-150, committed), a converging plastic one (300-1,500) and a failing
+    one (3,500-4,500), each a random direction with a small volumetric part.
+    Failures are decided far from the 25-iteration limit."""
+    rng = np.random.default_rng(seed)
+    n = 24
+    lo = np.select([np.arange(n) % 3 == 0, np.arange(n) % 3 == 1], [50.0, 300.0], 3500.0)
+    hi = np.select([np.arange(n) % 3 == 0, np.arange(n) % 3 == 1], [150.0, 1500.0], 4500.0)
+    targets = rng.uniform(lo, hi)
+    G = J2_FAIL[1]
+    eps = np.zeros((n, w))
+    for i in range(n):
+        d = rng.normal(size=w)
+        if w == 6:
+            d[:3] -= d[:3].mean()
+            # tensor norm of the engineering-shear vector: shears counted twice, halved
+            nrm = np.sqrt(np.sum(d[:3] ** 2) + 2 * np.sum((0.5 * d[3:]) ** 2))
+        else:
+            # plane strain: eps22 = 0; deviator of (e00, e11, 0) plus the shear
+            t = np.array([d[0], d[1], 0.0])
+            t -= t.mean()
+            nrm = np.sqrt(np.sum(t ** 2) + 2 * (0.5 * d[2]) ** 2)
+        eps[i] = d * targets[i] / (2 * G * nrm)
+    act = (np.arange(n) % 3 == 0).astype(np.int32)
+    return np.ascontiguousarray(eps), act
 
 
 def uniaxial_case(ref, kind, prm, n, seed, amp, shift=0.0):
@@ -173,7 +210,22 @@ def element_inputs(case, seed, n=10):
     return U, act
 
 
-def element_model(factory, case):
+def element_fail_inputs(case, seed, n=12):
+    """Nodal displacement path for the J2_FAIL elements, cycling through a
+    small elastic state (committed), a moderate plastic one and a large one
+    (strains about 0.02: trial stress norms past the failure threshold at
+    most Gauss points), so the committed hardening variable stays zero."""
+    nen, ndf = ELEMENT_CASES[case]
+    rng = np.random.default_rng(2000 + seed + len(case))
+    U = np.zeros((n, nen * ndf))
+    for k in range(n):
+        scale = (3e-4, 4e-3, 2.5e-2)[k % 3]
+        U[k] = rng.normal(size=nen * ndf) * scale
+    act = (np.arange(n) % 3 == 0).astype(np.int32)
+    return np.ascontiguousarray(U), act
+
+
+def element_model(factory, case, j2=J2):
     """The single-element model of `case` through the Model command API
     (factory(ndm, ndf) -> Model): nodes 1..nen, element 1."""
     nen, ndf = ELEMENT_CASES[case]
@@ -181,13 +233,13 @@ def element_model(factory, case):
         m = factory(3, 3)
         for a in range(8):
             m.node(a + 1, *BRICK_CRD[a])
-        m.nDMaterial("J2Plasticity", 1, *J2[:6])
+        m.nDMaterial("J2Plasticity", 1, *j2[:6])
         m.element("stdBrick", 1, *range(1, 9), 1)
     elif case == "quad":
         m = factory(2, 2)
         for a in range(4):
             m.node(a + 1, *QUAD_CRD[a])
-        m.nDMaterial("J2Plasticity", 1, *J2[:6])
+        m.nDMaterial("J2Plasticity", 1, *j2[:6])
         m.element("quad", 1, 1, 2, 3, 4, 1.3, "PlaneStrain", 1)
     else:
         is3d = case == "beam3d"
@@ -208,7 +260,7 @@ def element_model(factory, case):
     return m, nen, ndf
 
 
-def run_ref_element(ref, case, U, act):
+def run_ref_element(ref, case, U, act, j2=J2):
     """The same path through the reference's own element classes."""
     nen, ndf = ELEMENT_CASES[case]
     n, nd = len(U), nen * ndf
@@ -216,10 +268,10 @@ def run_ref_element(ref, case, U, act):
     act = np.ascontiguousarray(act, np.int32)
     K, R = np.zeros((n, nd, nd)), np.zeros((n, nd))
     if case == "brick":
-        rc = ref.brick_path(dp(np.ascontiguousarray(BRICK_CRD.ravel())), dp(J2), dp(np.zeros(3)), n, dp(U),
+        rc = ref.brick_path(dp(np.ascontiguousarray(BRICK_CRD.ravel())), dp(j2), dp(np.zeros(3)), n, dp(U),
                             ip(act), dp(K), dp(R))
     elif case == "quad":
-        rc = ref.quad_path(dp(np.ascontiguousarray(QUAD_CRD.ravel())), dp(J2), 1.3, dp(np.zeros(2)), n, dp(U),
+        rc = ref.quad_path(dp(np.ascontiguousarray(QUAD_CRD.ravel())), dp(j2), 1.3, dp(np.zeros(2)), n, dp(U),
                            ip(act), dp(K), dp(R))
     else:
         is3d = case == "beam3d"
@@ -259,6 +311,20 @@ def main():
         U, act = element_inputs(case, 7)
         K, R = run_ref_element(ref, case, U, act)
         d[f"ele_{case}_U"], d[f"ele_{case}_act"], d[f"ele_{case}_K"], d[f"ele_{case}_R"] = U, act, K, R
+    # the return-map failure path: material points and whole elements
+    d["j2fail_prm"] = J2_FAIL
+    for w, key in ((6, "j2fail_3d"), (3, "j2fail_ps")):
+        eps, act = j2_fail_path(w, 40 + w)
+        s, t = np.zeros((len(eps), w)), np.zeros((len(eps), w * w))
+        fn = ref.j2_3d_path if w == 6 else ref.j2_ps_path
+        bad = fn(dp(J2_FAIL), len(eps), dp(eps), ip(act), dp(s), dp(t))
+        d[key + "_eps"], d[key + "_act"], d[key + "_stress"], d[key + "_tangent"] = eps, act, s, t
+        d[key + "_bad"] = np.array([bad], np.int32)
+    for case in ("brick", "quad"):
+        U, act = element_fail_inputs(case, 9)
+        K, R = run_ref_element(ref, case, U, act, J2_FAIL)
+        d[f"elefail_{case}_U"], d[f"elefail_{case}_act"] = U, act
+        d[f"elefail_{case}_K"], d[f"elefail_{case}_R"] = K, R
     np.savez_compressed(OUT, **d)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {len(d)} arrays)")
 

@@ -119,3 +119,41 @@ def test_frame2d_pushover_trajectory():
     for (ug, lg), (uo, lo) in zip(*res):
         assert relerr(ug, uo) <= RTOL
         assert abs(lg - lo) <= RTOL * abs(lo)
+
+
+def test_frame3d_fused_pcg_launch_patterns():
+    """The node-blocked product that forms the new search direction on the
+    fly for six DOFs per node (k_spmv_nb_dot<L,6,true>) with the fused
+    alpha update (k_cg_update_fused<6>), taken by 3D frames when the
+    one-workgroup solve is off: fused and graph-replayed iterations equal the
+    separate launches bit for bit, and the pushover equals the oracle's to
+    1e-9."""
+    print("case: frame3d fused pcg launch patterns", flush=True)
+    fr = meshes.rc_frame_3d(2, 1, 3)
+    runs = []
+    for graph, fuse in ((True, True), (False, False), (True, False), (False, True)):
+        m = Model(3, 6)
+        meshes.build_rc_frame_3d(m, fr)
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        m.cg_config(16, graph, fuse, small=False)
+        m.test("NormUnbalance", 1e-5, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 6.0)
+        m.setup()
+        t = []
+        for step in range(10):
+            assert m.analyze(1) == 0
+            t.append(m.trial_disp().copy())
+        t.append(np.array([m.counters()["cg_iters"]], dtype=float))
+        runs.append(t)
+        m.close()
+    for other in runs[1:]:
+        for a, c in zip(runs[0], other):
+            assert np.array_equal(a, c)
+    o = _oracle.oracle_model(3, 6)
+    meshes.build_rc_frame_3d(o, fr)
+    o.test("NormUnbalance", 1e-5, 40)
+    o.integrator("DisplacementControl", fr["roof"], 1, 6.0)
+    o.setup()
+    for step in range(10):
+        assert o.analyze(1) == 0
+        assert relerr(runs[0][step], o.trial_disp()) <= RTOL, step

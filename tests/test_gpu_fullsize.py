@@ -1,0 +1,105 @@
+"""GPU parity at the full sizes of the two continuum workloads:
+
+* the 64 x 16 plane-strain FourNodeQuad J2 strip (2,176 equations) through
+  its whole 50-step DisplacementControl pushover to 0.05, against the oracle
+  step by step;
+* the 64^3 stdBrick J2 cube (811,200 equations, 63,695,790 CSR entries,
+  149,225,472 assembly-map entries: the int32 offsets at their largest):
+  numbering and CSR pattern bit-exact, and after one update on a field that
+  leaves most points plastic, every point's stress and tangent, the
+  unbalance and all CSR values against the oracle; then a PCG solve checked
+  by its true residual in the GPU's own assembled matrix (scipy product)."""
+import numpy as np
+import pytest
+
+from tests import _oracle
+from xara_amd import meshes
+from xara_amd.model import Model
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-9
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    s = np.max(np.abs(b))
+    return 0.0 if s == 0 else float(np.max(np.abs(a - b)) / s)
+
+
+def test_quad_strip_64x16_full_pushover():
+    print("case: quad strip 64x16, 50 DisplacementControl steps", flush=True)
+    mesh = meshes.quad_strip(64, 16)
+    res = []
+    for m in (Model(2, 2), _oracle.oracle_model(2, 2)):
+        meshes.build_quad_strip(m, mesh)
+        m.load(mesh["tip"], 0.0, 1.0)
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        m.test("NormUnbalance", 1e-8, 25)
+        m.integrator("DisplacementControl", mesh["tip"], 2, 0.001)
+        m.setup()
+        assert m.neq == 2176
+        traj = []
+        for k in range(1, 51):
+            assert m.analyze(1) == 0, k
+            assert abs(m.nodeDisp(mesh["tip"], 2) - 0.001 * k) <= 1e-12
+            traj.append((m.trial_disp().copy(), m.load_factor))
+        npts = 4 * len(mesh["ele_tags"])
+        res.append((traj, m.point_data("plastic", npts)[:, 6] > 0))
+        m.close()
+    (tg, pg), (to, po) = res
+    for k, ((ug, lg), (uo, lo)) in enumerate(zip(tg, to)):
+        assert relerr(ug, uo) <= RTOL, k
+        assert abs(lg - lo) <= RTOL * abs(lo), k
+    assert np.array_equal(pg, po) and pg.sum() > 100
+
+
+def cube_field(mesh):
+    x, y, z = mesh["crds"].T
+    mod = 1.0 + 0.6 * np.sin(2 * np.pi * x) * np.cos(np.pi * y)
+    U = np.zeros((len(x), 3))
+    U[:, 2] = -2e-3 * z * mod
+    U[:, 0] = 0.3 * 2e-3 * (x - 0.5) * z
+    U[:, 1] = 0.3 * 2e-3 * (y - 0.5) * z * mod
+    return U
+
+
+def test_cube_64_full_state_determination():
+    import scipy.sparse as sp
+
+    print("case: 64^3 cube numbering, update, unbalance, tangent, solve", flush=True)
+    n = 64
+    mesh = meshes.brick_cube(n)
+    U = cube_field(mesh)
+    npts = 8 * n ** 3
+    out = []
+    for m in (Model(3, 3), _oracle.oracle_model(3, 3)):
+        meshes.build_brick_cube(m, mesh)
+        m.system("GpuBlockPCG", 1e-10, 20000)
+        m.setup()
+        rowptr, colind = m.csr_pattern()
+        eq = m.node_eqns()
+        m.set_trial_disp(U)
+        assert m.update() == 0
+        m.set_load_factor(100.0)
+        B = m.form_unbalance()
+        vals = m.form_tangent()
+        out.append(dict(m=m, eq=eq, rowptr=rowptr, colind=colind, B=B, vals=vals,
+                        stress=m.point_data("stress", npts), tangent=m.point_data("tangent", npts),
+                        plastic=m.point_data("plastic", npts)[:, 6]))
+        print(f"  {type(m).__name__} done", flush=True)
+    g, o = out
+    assert g["m"].neq == 811200 and g["m"].nnz == 63695790
+    assert np.array_equal(g["eq"], o["eq"])
+    assert np.array_equal(g["rowptr"], o["rowptr"]) and np.array_equal(g["colind"], o["colind"])
+    assert (g["plastic"] > 0).mean() > 0.5
+    for key in ("stress", "tangent", "B", "vals"):
+        assert relerr(g[key], o[key]) <= RTOL, key
+    o["m"].close()
+    # the PCG solve of the assembled system, by its true residual
+    gm = g["m"]
+    A = sp.csr_matrix((g["vals"], g["colind"], g["rowptr"]), shape=(gm.neq, gm.neq))
+    X = gm.solve(g["B"])
+    r = A @ X - g["B"]
+    assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(g["B"])
+    assert gm.counters()["last_cg_relres"] <= 1e-10
+    gm.close()

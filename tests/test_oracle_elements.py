@@ -20,15 +20,16 @@ import numpy as np
 import pytest
 
 from tests import _oracle
-from tests.golden.make_golden import (ELEMENT_CASES, J2, element_inputs, element_model, run_ref_element)
+from tests.golden.make_golden import (ELEMENT_CASES, J2, J2_FAIL, element_fail_inputs, element_inputs, element_model,
+                                      run_ref_element)
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz")
 
 
-def run_oracle_element(case, U, act):
+def run_oracle_element(case, U, act, j2=J2):
     """Walk the displacement path U (n x element DOFs) with commits act
     through the oracle's single-element model; returns K (n, nd, nd), R."""
-    m, nen, ndf = element_model(_oracle.oracle_model, case)
+    m, nen, ndf = element_model(_oracle.oracle_model, case, j2)
     m.setup()
     n = len(U)
     nd = nen * ndf
@@ -67,6 +68,27 @@ def test_element_golden(case):
     assert rel(Ko, g[f"ele_{case}_K"]) <= 1e-13 and rel(Ro, g[f"ele_{case}_R"]) <= 1e-13
     # the recorded path leaves the elastic range (tangent changes along it)
     assert rel(g[f"ele_{case}_K"][-1], g[f"ele_{case}_K"][0]) > 1e-3
+
+
+@pytest.mark.parametrize("case", ["brick", "quad"])
+def test_element_return_map_failure_golden(case):
+    """Brick and FourNodeQuad over J2 whose return map fails at most Gauss
+    points of every third state (J2Plasticity.cpp:309-313): Brick::update
+    ignores the material's return code (Brick.cpp:839) and J2PlaneStrain
+    returns 0, so the elements go on with the failed points' previous trial
+    stress and tangent; K and R against the reference's."""
+    g = np.load(GOLD, allow_pickle=False)
+    U, act = g[f"elefail_{case}_U"], g[f"elefail_{case}_act"]
+    Ko, Ro = run_oracle_element(case, U, act, J2_FAIL)
+    assert rel(Ko, g[f"elefail_{case}_K"]) <= 1e-13 and rel(Ro, g[f"elefail_{case}_R"]) <= 1e-13
+
+
+@pytest.mark.parametrize("case", ["brick", "quad"])
+def test_element_return_map_failure_live_reference(ref, case):
+    U, act = element_fail_inputs(case, 3)
+    Kr, Rr = run_ref_element(ref, case, U, act, J2_FAIL)
+    Ko, Ro = run_oracle_element(case, U, act, J2_FAIL)
+    assert rel(Ko, Kr) <= 1e-13 and rel(Ro, Rr) <= 1e-13
 
 
 # ---- closed forms that pin FourNodeQuad without the reference ----

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from tests.golden.make_golden import J2, J2_SAT, STEEL, STEEL_ISO, CONC, dp, ip, graph_cases
+from tests.golden.make_golden import J2, J2_SAT, J2_FAIL, STEEL, STEEL_ISO, CONC, dp, ip, graph_cases, j2_fail_path
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz")
 
@@ -36,6 +36,30 @@ def test_j2_golden(orc, gold, name, prm, kind):
     assert close(s, gold[key + "_stress"]) and close(t, gold[key + "_tangent"])
     # the path must actually exercise the plastic branch
     assert not np.allclose(t[-1], t[0])
+
+
+@pytest.mark.parametrize("kind", ["3d", "ps"])
+def test_j2_return_map_failure_golden(orc, gold, kind):
+    """The local Newton iteration on the plastic multiplier exceeding 25
+    iterations (J2Plasticity.cpp:309-313): the integrator returns before
+    touching the trial state, so stress and tangent stay those of the
+    previous trial strain; J2ThreeDimensional::setTrialStrain passes the -1
+    on (:141), J2PlaneStrain::setTrialStrain returns 0 regardless (:133-135).
+    Every third step of the recorded path fails (far from the iteration
+    limit), the others converge, elastic and plastic."""
+    key = f"j2fail_{kind}"
+    eps = np.ascontiguousarray(gold[key + "_eps"])
+    act = np.ascontiguousarray(gold[key + "_act"])
+    n, w = eps.shape
+    s, t = np.zeros((n, w)), np.zeros((n, w * w))
+    fn = orc.j2_3d_path if kind == "3d" else orc.j2_ps_path
+    bad = fn(dp(gold["j2fail_prm"]), n, dp(eps), ip(act), dp(s), dp(t))
+    assert close(s, gold[key + "_stress"]) and close(t, gold[key + "_tangent"])
+    assert bad == int(gold[key + "_bad"][0]) == (n // 3 if kind == "3d" else 0)
+    fail = np.arange(2, n, 3)
+    assert np.array_equal(s[fail], s[fail - 1]) and np.array_equal(t[fail], t[fail - 1])
+    ok = np.arange(1, n, 3)
+    assert not np.allclose(t[ok], t[ok - 1])  # the converged steps are plastic
 
 
 @pytest.mark.parametrize("name,kind,prm", [("steel", 1, STEEL), ("steeliso", 1, STEEL_ISO), ("conc", 2, CONC)])
@@ -85,6 +109,20 @@ def test_j2_live_reference(orc, ref, seed):
             f(dp(prm), n, dp(eps), ip(act), dp(s), dp(t))
             out.append((s, t))
         assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_j2_return_map_failure_live_reference(orc, ref, seed):
+    for w, fo, fr in ((6, orc.j2_3d_path, ref.j2_3d_path), (3, orc.j2_ps_path, ref.j2_ps_path)):
+        eps, act = j2_fail_path(w, seed)
+        n = len(eps)
+        out = []
+        for f in (fo, fr):
+            s, t = np.zeros((n, w)), np.zeros((n, w * w))
+            bad = f(dp(J2_FAIL), n, dp(eps), ip(act), dp(s), dp(t))
+            out.append((s, t, bad))
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+        assert out[0][2] == out[1][2] == (n // 3 if w == 6 else 0)
 
 
 @pytest.mark.parametrize("kind,prm,amp,shift", [(1, STEEL, 0.015, 0.0), (1, STEEL_ISO, 0.02, 0.001),
