@@ -143,7 +143,11 @@ int  xg_nodal_loads(xg_domain* d, int n, const int* tags, const double* P /* n*n
    (UniformExcitation.cpp:132, PathSeries.cpp:334).  dir is 0-based.         */
 int  xg_uniform_excitation(xg_domain* d, int dir, int npts, double dt,
                            const double* accel, double factor);
-/* `rayleigh alphaM betaK betaKinit betaKcomm` (only alphaM, betaKcomm used) */
+/* `rayleigh alphaM betaK betaKinit betaKcomm` (Domain::setRayleighDampingFactors,
+   Domain.cpp:1858-1873; Element::getDamp / getRayleighDampingForces,
+   Element.cpp:181-200,283-323): betaK on the current tangent, betaKcomm on
+   the tangent of the last commit; betaKinit must be 0 (XG_ERR_ARG).  The
+   ensemble and the adjoint take alphaM only. */
 int  xg_rayleigh(xg_domain* d, double alphaM, double betaK, double betaKinit,
                  double betaKcomm);
 

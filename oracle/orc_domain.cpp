@@ -38,6 +38,9 @@ struct Dom;
 struct OElement {
   int tag = 0;
   std::vector<int> nodes;  // node indices (tag order positions)
+  // committed tangent for betaKcomm damping (Element::Kc, Element.cpp:167-176):
+  // formed when the factors are set, refreshed by every commitState (:74-80)
+  std::vector<double> Kc;
   virtual ~OElement() {}
   virtual int ndof() const = 0;
   virtual int update(Dom& d) = 0;
@@ -329,6 +332,20 @@ static std::vector<int> rcm_number(const std::vector<std::vector<int>>& adj) {
   return result;  // result[i] = vertex tag at position i
 }
 
+// Element::setRayleighDampingFactors (Element.cpp:110-178): with betaKcomm
+// != 0 an element without a committed tangent takes its current tangent,
+// without it the stored one is dropped.
+static void rayleigh_kc_init(Dom& d) {
+  for (auto* e : d.eles) {
+    if (d.betaKcomm == 0.0) {
+      e->Kc.clear();
+    } else if (e->Kc.empty()) {
+      e->Kc.assign((size_t)e->ndof() * e->ndof(), 0.0);
+      e->tangent(d, e->Kc.data());
+    }
+  }
+}
+
 static int setup(Dom& d) {
   if (d.nodemap.empty()) return fail(XG_ERR_STATE, "no nodes");
   d.nodes.clear();
@@ -411,6 +428,7 @@ static int setup(Dom& d) {
   for (auto* e : d.eles)
     if (dynamic_cast<OBeam*>(e)) e->update(d);
   d.ready = true;
+  rayleigh_kc_init(d);
   return 0;
 }
 
@@ -461,10 +479,20 @@ static int form_tangent(Dom& d) {
     K.assign(nd * nd, 0.0);
     e->tangent(d, K.data());
     if (d.integ == 3) {
-      // c1*K + c2*C + c3*M with Rayleigh C = alphaM M + betaKcomm Kc
+      // Newmark::formEleTangent (Newmark.cpp:262-289): T = c1 K, T += c2 C,
+      // T += c3 M, with C = Element::getDamp (Element.cpp:181-200):
+      // alphaM M, then + betaK K, + betaKcomm Kc
       M.assign(nd * nd, 0.0);
       e->mass(d, M.data());
-      for (int i = 0; i < nd * nd; i++) K[i] = f.cK * K[i] + f.cM * M[i] + f.cC * d.alphaM * M[i];
+      for (int i = 0; i < nd * nd; i++) {
+        double c = 0.0;
+        if (d.alphaM != 0.0) c = d.alphaM * M[i];
+        if (d.betaK != 0.0) c = c + d.betaK * K[i];
+        if (d.betaKcomm != 0.0) c = c + d.betaKcomm * e->Kc[i];
+        double t = f.cK * K[i];
+        t = t + f.cC * c;
+        K[i] = t + f.cM * M[i];
+      }
     }
     element_id(d, e, id);
     for (int i = 0; i < nd; i++) {
@@ -510,16 +538,38 @@ static int form_unbalance(Dom& d) {
     R.assign(nd, 0.0);
     e->resist(d, R.data());
     if (d.integ == 3) {
-      // getResistingForceIncInertia: + M a (lumped) + alphaM M v
+      // getResistingForceIncInertia: + M a (lumped), then + the Rayleigh
+      // damping force D v (Element::getRayleighDampingForces,
+      // Element.cpp:283-323: D = alphaM M, + betaK K, + betaKcomm Kc)
       M.assign(nd * nd, 0.0);
       e->mass(d, M.data());
       int nn = (int)e->nodes.size();
+      std::vector<double> v(nd);
       for (int a = 0; a < nn; a++)
         for (int j = 0; j < d.ndf; j++) {
           const NodeRec& nr = d.nodes[e->nodes[a]];
           int i = a * d.ndf + j;
-          R[i] += M[i * nd + i] * nr.A[j] + d.alphaM * M[i * nd + i] * nr.V[j];
+          R[i] += M[i * nd + i] * nr.A[j];
+          v[i] = nr.V[j];
         }
+      if (d.alphaM != 0.0 || d.betaK != 0.0 || d.betaKcomm != 0.0) {
+        std::vector<double> Kt;
+        if (d.betaK != 0.0) {
+          Kt.assign((size_t)nd * nd, 0.0);
+          e->tangent(d, Kt.data());
+        }
+        for (int i = 0; i < nd; i++) {
+          double fi = 0.0;
+          for (int j = 0; j < nd; j++) {
+            double c = 0.0;
+            if (d.alphaM != 0.0) c = d.alphaM * M[i * nd + j];
+            if (d.betaK != 0.0) c = c + d.betaK * Kt[i * nd + j];
+            if (d.betaKcomm != 0.0) c = c + d.betaKcomm * e->Kc[i * nd + j];
+            fi += c * v[j];
+          }
+          R[i] += fi;
+        }
+      }
     }
     element_id(d, e, id);
     for (int i = 0; i < nd; i++)
@@ -621,7 +671,12 @@ static void commit_domain(Dom& d) {
       n.Vc[j] = n.V[j];
       n.Ac[j] = n.A[j];
     }
-  for (auto* e : d.eles) e->commit();
+  for (auto* e : d.eles) {
+    // Element::commitState (Element.cpp:74-80), called first by each
+    // element's own commitState: Kc = getTangentStiff()
+    if (!e->Kc.empty()) e->tangent(d, e->Kc.data());
+    e->commit();
+  }
 }
 
 // Domain::revertToLastCommit (Domain.cpp:1925-1939): nodes and elements
@@ -1273,11 +1328,15 @@ int orc_uniform_excitation(xg_domain* d, int dir, int npts, double dt, const dou
   d->gm_accel.assign(accel, accel + npts);
   return 0;
 }
+// Domain::setRayleighDampingFactors (Domain.cpp:1858-1873) on every element
+// and node.  Initial-stiffness damping (betaKinit) is not supported.
 int orc_rayleigh(xg_domain* d, double alphaM, double betaK, double betaKinit, double betaKcomm) {
+  if (betaKinit != 0.0) return fail(XG_ERR_ARG, "rayleigh: betaKinit (initial-stiffness damping) is not supported");
   d->alphaM = alphaM;
   d->betaK = betaK;
-  d->betaKinit = betaKinit;
+  d->betaKinit = 0.0;
   d->betaKcomm = betaKcomm;
+  if (d->ready) rayleigh_kc_init(*d);
   return 0;
 }
 int orc_system(xg_domain* d, int kind, double rtol, int maxit) {
@@ -1431,6 +1490,27 @@ int orc_element_tangent(xg_domain* d, int ele_tag, double* K, int* ndof) {
   if (it == d->elemap.end()) return fail(XG_ERR_ARG, "element not found");
   it->second->tangent(*d, K);
   *ndof = it->second->ndof();
+  return 0;
+}
+// Element::getDamp (Element.cpp:181-200) of one element: alphaM M, then
+// + betaK K, + betaKcomm Kc -- the matrix the transient tangent and the
+// damping force use.  Oracle-only (test infrastructure).
+int orc_element_damp(xg_domain* d, int ele_tag, double* D, int* ndof) {
+  auto it = d->elemap.find(ele_tag);
+  if (it == d->elemap.end()) return fail(XG_ERR_ARG, "element not found");
+  OElement* e = it->second.get();
+  const int nd = e->ndof();
+  std::vector<double> M((size_t)nd * nd, 0.0), K((size_t)nd * nd, 0.0);
+  e->mass(*d, M.data());
+  if (d->betaK != 0.0) e->tangent(*d, K.data());
+  for (int i = 0; i < nd * nd; i++) {
+    double c = 0.0;
+    if (d->alphaM != 0.0) c = d->alphaM * M[i];
+    if (d->betaK != 0.0) c = c + d->betaK * K[i];
+    if (d->betaKcomm != 0.0) c = c + d->betaKcomm * e->Kc[i];
+    D[i] = c;
+  }
+  *ndof = nd;
   return 0;
 }
 int orc_element_resisting_force(xg_domain* d, int ele_tag, double* R, int* ndof) {

@@ -195,40 +195,29 @@ static int walk_element(Domain* dom, Element* e, int nen, int ndf, int n, const 
   return bad;
 }
 
-// stdBrick (Brick.cpp) over J2 (prm = K G sig0 sigInf delta H eta), nodes
-// crd (8 x 3), body force b (3).
-int ref_brick_path(const double* crd, const double* prm, const double* b, int n, const double* U,
-                   const int* action, double* K, double* R) {
-  Domain* dom = new Domain();
+// Builders of one element of each kind in a fresh Domain (nodes 1..nen),
+// the element added last (Domain::addElement sets its domain, which runs the
+// first update of the frame elements).
+static Element* build_brick(Domain* dom, const double* crd, const double* prm, const double* b) {
   for (int a = 0; a < 8; a++) dom->addNode(new Node(a + 1, 3, crd[3 * a], crd[3 * a + 1], crd[3 * a + 2]));
   J2ThreeDimensional mat(1, prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], 0.0);
   Brick* e = new Brick(1, 1, 2, 3, 4, 5, 6, 7, 8, mat, b[0], b[1], b[2]);
   dom->addElement(e);
-  return walk_element(dom, e, 8, 3, n, U, action, K, R);
+  return e;
 }
 
-// quad (FourNodeQuad.cpp) over J2PlaneStrain, nodes crd (4 x 2), thickness,
-// body force b (2).
-int ref_quad_path(const double* crd, const double* prm, double thk, const double* b, int n, const double* U,
-                  const int* action, double* K, double* R) {
-  Domain* dom = new Domain();
+static Element* build_quad(Domain* dom, const double* crd, const double* prm, double thk, const double* b) {
   for (int a = 0; a < 4; a++) dom->addNode(new Node(a + 1, 2, crd[2 * a], crd[2 * a + 1]));
   J2PlaneStrain mat(1, prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], 0.0);
   std::array<int, 4> nodes{1, 2, 3, 4};
   FourNodeQuad* e = new FourNodeQuad(1, nodes, mat, thk, 0.0, 0.0, b[0], b[1]);
   dom->addElement(e);
-  return walk_element(dom, e, 4, 2, n, U, action, K, R);
+  return e;
 }
 
-// dispBeamColumn (DispBeamColumn2d.cpp / DispBeamColumn3d.cpp) with
-// `nip` Gauss-Legendre sections (FrameQuadrature<GaussLegendre<1, nip>>,
-// nip = 2..5) of one fiber section (fibers as ref_fiber_section_path),
-// geomTransf Linear (vecxz used in 3D), nodes crd (2 x ndm).
-int ref_dispbeam_path(int is3d, const double* crd, const double* vecxz, int nip, int nfib, const int* kinds,
-                      const double* prm, const double* y, const double* z, const double* A, double GJ,
-                      int compute_centroid, int n, const double* U, const int* action, double* K, double* R) {
-  Domain* dom = new Domain();
-  const int ndf = is3d ? 6 : 3;
+static Element* build_dispbeam(Domain* dom, int is3d, const double* crd, const double* vecxz, int nip, int nfib,
+                               const int* kinds, const double* prm, const double* y, const double* z,
+                               const double* A, double GJ, int compute_centroid) {
   for (int a = 0; a < 2; a++) {
     if (is3d) dom->addNode(new Node(a + 1, 6, crd[3 * a], crd[3 * a + 1], crd[3 * a + 2]));
     else dom->addNode(new Node(a + 1, 3, crd[2 * a], crd[2 * a + 1]));
@@ -256,7 +245,7 @@ int ref_dispbeam_path(int is3d, const double* crd, const double* vecxz, int nip,
     case 3: bi = new FrameQuadrature<GaussLegendre<1, 3>>(); break;
     case 4: bi = new FrameQuadrature<GaussLegendre<1, 4>>(); break;
     case 5: bi = new FrameQuadrature<GaussLegendre<1, 5>>(); break;
-    default: return -1;
+    default: return nullptr;
   }
   std::vector<SectionForceDeformation*> secs(nip, sec);
   Element* e;
@@ -270,7 +259,86 @@ int ref_dispbeam_path(int is3d, const double* crd, const double* vecxz, int nip,
     e = new DispBeamColumn2d(1, 1, 2, nip, secs.data(), *bi, tr);
   }
   dom->addElement(e);
-  return walk_element(dom, e, 2, ndf, n, U, action, K, R);
+  return e;
+}
+
+// stdBrick (Brick.cpp) over J2 (prm = K G sig0 sigInf delta H eta), nodes
+// crd (8 x 3), body force b (3).
+int ref_brick_path(const double* crd, const double* prm, const double* b, int n, const double* U,
+                   const int* action, double* K, double* R) {
+  Domain* dom = new Domain();
+  Element* e = build_brick(dom, crd, prm, b);
+  return walk_element(dom, e, 8, 3, n, U, action, K, R);
+}
+
+// quad (FourNodeQuad.cpp) over J2PlaneStrain, nodes crd (4 x 2), thickness,
+// body force b (2).
+int ref_quad_path(const double* crd, const double* prm, double thk, const double* b, int n, const double* U,
+                  const int* action, double* K, double* R) {
+  Domain* dom = new Domain();
+  Element* e = build_quad(dom, crd, prm, thk, b);
+  return walk_element(dom, e, 4, 2, n, U, action, K, R);
+}
+
+// dispBeamColumn (DispBeamColumn2d.cpp / DispBeamColumn3d.cpp) with
+// `nip` Gauss-Legendre sections (FrameQuadrature<GaussLegendre<1, nip>>,
+// nip = 2..5) of one fiber section (fibers as ref_fiber_section_path),
+// geomTransf Linear (vecxz used in 3D), nodes crd (2 x ndm).
+int ref_dispbeam_path(int is3d, const double* crd, const double* vecxz, int nip, int nfib, const int* kinds,
+                      const double* prm, const double* y, const double* z, const double* A, double GJ,
+                      int compute_centroid, int n, const double* U, const int* action, double* K, double* R) {
+  Domain* dom = new Domain();
+  Element* e = build_dispbeam(dom, is3d, crd, vecxz, nip, nfib, kinds, prm, y, z, A, GJ, compute_centroid);
+  if (!e) return -1;
+  return walk_element(dom, e, 2, is3d ? 6 : 3, n, U, action, K, R);
+}
+
+// Rayleigh damping matrix of one element along a displacement path
+// (Element::setRayleighDampingFactors right after the element joins the
+// domain, Element.cpp:110-178, then Element::getDamp, :181-200, at every
+// state; commitState refreshes the committed tangent, :74-80).  kind 1
+// brick, 2 quad (J2 prm, crd as above), 3 / 4 dispBeamColumn2d / 3d over the
+// fibers of ref_dispbeam_path with 5 Legendre points.  D: n x nd x nd.
+int ref_element_damp_path(int kind, const double* crd, const double* vecxz, const double* j2, int nfib,
+                          const int* kinds, const double* prm, const double* y, const double* z, const double* A,
+                          double alphaM, double betaK, double betaKc, int n, const double* U, const int* action,
+                          double* D) {
+  Domain* dom = new Domain();
+  const double b[3] = {0.0, 0.0, 0.0};
+  Element* e = nullptr;
+  int nen = 2, ndf = 3;
+  if (kind == 1) {
+    e = build_brick(dom, crd, j2, b);
+    nen = 8;
+  } else if (kind == 2) {
+    e = build_quad(dom, crd, j2, 1.3, b);
+    nen = 4;
+    ndf = 2;
+  } else {
+    const int is3d = kind == 4;
+    e = build_dispbeam(dom, is3d, crd, vecxz, 5, nfib, kinds, prm, y, z, A, is3d ? 1.0e12 : -1.0, 1);
+    ndf = is3d ? 6 : 3;
+  }
+  if (!e) return -1;
+  e->setRayleighDampingFactors(alphaM, betaK, 0.0, betaKc);
+  const int nd = nen * ndf;
+  int bad = 0;
+  Vector u(ndf);
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < nen; a++) {
+      for (int j = 0; j < ndf; j++) u(j) = U[(size_t)i * nd + a * ndf + j];
+      dom->getNode(a + 1)->setTrialDisp(u);
+    }
+    if (e->update() != 0) bad++;
+    const Matrix& c = e->getDamp();
+    for (int r = 0; r < nd; r++)
+      for (int k = 0; k < nd; k++) D[(size_t)i * nd * nd + r * nd + k] = c(r, k);
+    if (action[i] == 1) {
+      for (int a = 0; a < nen; a++) dom->getNode(a + 1)->commitState();
+      e->commitState();
+    }
+  }
+  return bad;
 }
 
 }  // extern "C"

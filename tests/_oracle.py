@@ -34,6 +34,7 @@ def load():
             "orc_rcm": [C.c_int, P_I, P_I, P_I],
             "orc_ddm_transient": [C.c_void_p, C.c_int, C.c_double, P_D, P_D, C.c_int, P_I, C.c_int, P_D, P_D],
             "orc_material_jvp": [C.c_int, P_D, P_D, C.c_int, C.c_double, P_D, P_D, C.c_double, P_D, P_I, P_D],
+            "orc_element_damp": [C.c_void_p, C.c_int, P_D, P_I],
         }.items():
             f = getattr(lib, name)
             f.argtypes = args
@@ -100,6 +101,8 @@ class Ref:
             "ref_quad_path": [P_D, P_D, C.c_double, P_D, C.c_int, P_D, P_I, P_D, P_D],
             "ref_dispbeam_path": [C.c_int, P_D, P_D, C.c_int, C.c_int, P_I, P_D, P_D, P_D, P_D, C.c_double,
                                   C.c_int, C.c_int, P_D, P_I, P_D, P_D],
+            "ref_element_damp_path": [C.c_int, P_D, P_D, P_D, C.c_int, P_I, P_D, P_D, P_D, P_D, C.c_double,
+                                      C.c_double, C.c_double, C.c_int, P_D, P_I, P_D],
         }.items():
             f = getattr(lib, name)
             f.argtypes = args

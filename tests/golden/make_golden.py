@@ -225,6 +225,27 @@ def element_fail_inputs(case, seed, n=12):
     return np.ascontiguousarray(U), act
 
 
+DAMP = (0.0, 2.0e-3, 3.0e-3)  # alphaM, betaK, betaKcomm of the damping paths
+
+
+def run_ref_damp(ref, case, U, act, alphaM=DAMP[0], betaK=DAMP[1], betaKc=DAMP[2]):
+    """Element::getDamp of the reference's element along the path, the
+    Rayleigh factors set right after the element joins its domain."""
+    nen, ndf = ELEMENT_CASES[case]
+    n, nd = len(U), nen * ndf
+    D = np.zeros((n, nd, nd))
+    kinds, prm, y, z, A = element_fibers()
+    kind = {"brick": 1, "quad": 2, "beam2d": 3, "beam3d": 4}[case]
+    crd = {"brick": BRICK_CRD, "quad": QUAD_CRD, "beam2d": BEAM2D_CRD, "beam3d": BEAM3D_CRD}[case]
+    rc = ref.element_damp_path(kind, dp(np.ascontiguousarray(crd.ravel())), dp(BEAM3D_VECXZ), dp(J2), len(kinds),
+                               ip(kinds), dp(np.ascontiguousarray(prm)), dp(y), dp(z), dp(A), alphaM, betaK,
+                               betaKc, n, dp(np.ascontiguousarray(U)), ip(np.ascontiguousarray(act, np.int32)),
+                               dp(D))
+    if rc != 0:
+        raise RuntimeError(f"reference {case} damping path: {rc} failed updates")
+    return D
+
+
 def element_model(factory, case, j2=J2):
     """The single-element model of `case` through the Model command API
     (factory(ndm, ndf) -> Model): nodes 1..nen, element 1."""
@@ -311,6 +332,10 @@ def main():
         U, act = element_inputs(case, 7)
         K, R = run_ref_element(ref, case, U, act)
         d[f"ele_{case}_U"], d[f"ele_{case}_act"], d[f"ele_{case}_K"], d[f"ele_{case}_R"] = U, act, K, R
+    # Rayleigh damping matrices (betaK on the current, betaKcomm on the
+    # committed tangent) along the element paths
+    for case in ELEMENT_CASES:
+        d[f"damp_{case}_D"] = run_ref_damp(ref, case, d[f"ele_{case}_U"], d[f"ele_{case}_act"])
     # the return-map failure path: material points and whole elements
     d["j2fail_prm"] = J2_FAIL
     for w, key in ((6, "j2fail_3d"), (3, "j2fail_ps")):

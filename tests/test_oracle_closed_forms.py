@@ -400,10 +400,15 @@ def test_newton_elastic_one_step_equals_linear_solve():
     assert np.allclose(got, x, rtol=1e-10, atol=1e-14)
 
 
-def test_newmark_linear_matches_textbook_recursion():
+@pytest.mark.parametrize("alphaM,betaK,betaKc", [(0.4, 0.0, 0.0), (0.3, 2e-3, 0.0), (0.0, 0.0, 3e-3),
+                                                 (0.2, 1e-3, 1.5e-3)])
+def test_newmark_linear_matches_textbook_recursion(alphaM, betaK, betaKc):
     """Elastic 2D frame under ground acceleration: the oracle's Newmark
-    (average acceleration, mass-proportional damping) equals the textbook
-    recursion on the assembled K and M (Chopra, Table 5.4.2)."""
+    (average acceleration, Rayleigh damping C = alphaM M + betaK K +
+    betaKcomm Kc, where an elastic frame has Kc = K) equals the textbook
+    recursion on the assembled K and M (Chopra, Table 5.4.2).  A damping
+    term dropped from the tangent or from the resisting force, or taken
+    with the wrong factor, moves the trajectory far beyond the bar."""
     from xara_amd import ground_motion
     fr = meshes.rc_frame_2d(3, 2)
     m = _oracle.oracle_model(2, 3)
@@ -418,10 +423,10 @@ def test_newmark_linear_matches_textbook_recursion():
     mass = np.zeros((len(fr["floor_nodes"]), 3))
     mass[:, 0] = mass[:, 1] = 40.0
     m.masses(fr["floor_nodes"], mass)
-    dt, alphaM = 0.01, 0.4
+    dt = 0.01
     ag = ground_motion.kanai_tajimi(3, 200, dt, 0.3)
     m._check(m._b.uniform_excitation(m._h, 0, len(ag), dt, ag.ctypes.data_as(__import__("xara_amd")._lib.P_D), 1.0))
-    m._check(m._b.rayleigh(m._h, alphaM, 0.0, 0.0, 0.0))
+    m._check(m._b.rayleigh(m._h, alphaM, betaK, 0.0, betaKc))
     m.integrator("LoadControl", 0.0)
     m.test("NormDispIncr", 1e-12, 10)
     m.setup()
@@ -442,7 +447,7 @@ def test_newmark_linear_matches_textbook_recursion():
                 Mv[eq[i, j]] = 40.0
             rvec[eq[i, 0]] = 1.0
     M = np.diag(Mv)
-    C = alphaM * M
+    C = alphaM * M + (betaK + betaKc) * Kd
     g, b = 0.5, 0.25
     u = np.zeros(n); v = np.zeros(n); a = np.zeros(n)
     Khat = Kd + g / (b * dt) * C + 1 / (b * dt * dt) * M

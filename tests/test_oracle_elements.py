@@ -20,8 +20,8 @@ import numpy as np
 import pytest
 
 from tests import _oracle
-from tests.golden.make_golden import (ELEMENT_CASES, J2, J2_FAIL, element_fail_inputs, element_inputs, element_model,
-                                      run_ref_element)
+from tests.golden.make_golden import (DAMP, ELEMENT_CASES, J2, J2_FAIL, element_fail_inputs, element_inputs,
+                                      element_model, run_ref_damp, run_ref_element)
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.npz")
 
@@ -68,6 +68,52 @@ def test_element_golden(case):
     assert rel(Ko, g[f"ele_{case}_K"]) <= 1e-13 and rel(Ro, g[f"ele_{case}_R"]) <= 1e-13
     # the recorded path leaves the elastic range (tangent changes along it)
     assert rel(g[f"ele_{case}_K"][-1], g[f"ele_{case}_K"][0]) > 1e-3
+
+
+def run_oracle_damp(case, U, act, alphaM=DAMP[0], betaK=DAMP[1], betaKc=DAMP[2]):
+    """The oracle's Rayleigh damping matrix (alphaM M + betaK K + betaKcomm
+    Kc) of the single element along the path: the factors set before
+    set-up, so Kc starts from the tangent at set-up."""
+    import ctypes as C
+    m, nen, ndf = element_model(_oracle.oracle_model, case)
+    m.rayleigh(alphaM, betaK, 0.0, betaKc)
+    m.setup()
+    orc = _oracle.load()
+    nd = nen * ndf
+    D = np.zeros((len(U), nd, nd))
+    k = C.c_int(0)
+    for i in range(len(U)):
+        m.set_trial_disp(U[i].reshape(nen, ndf))
+        m.update()
+        Di = np.zeros(nd * nd)
+        assert orc.element_damp(m._h, 1, Di.ctypes.data_as(C.POINTER(C.c_double)), C.byref(k)) == 0
+        D[i] = Di.reshape(nd, nd)
+        if act[i]:
+            m.commit()
+    m.close()
+    return D
+
+
+@pytest.mark.parametrize("case", sorted(ELEMENT_CASES))
+def test_rayleigh_damping_matrix_golden(case):
+    """Element::getDamp (Element.cpp:181-200) with betaK on the current
+    tangent and betaKcomm on the tangent stored at the last commit
+    (Element.cpp:74-80, 167-176), against the reference's along a path
+    with commits; the committed part must lag the current one."""
+    g = np.load(GOLD, allow_pickle=False)
+    U, act = g[f"ele_{case}_U"], g[f"ele_{case}_act"]
+    D = run_oracle_damp(case, U, act)
+    assert rel(D, g[f"damp_{case}_D"]) <= 1e-13
+    # a slip that used the current tangent for the committed part fails here
+    Kt = g[f"ele_{case}_K"]
+    assert rel(D, (DAMP[1] + DAMP[2]) * Kt) > 1e-6
+
+
+@pytest.mark.parametrize("case", sorted(ELEMENT_CASES))
+def test_rayleigh_damping_matrix_live_reference(ref, case):
+    U, act = element_inputs(case, 4)
+    assert rel(run_oracle_damp(case, U, act), run_ref_damp(ref, case, U, act)) <= 1e-13
+    assert rel(run_oracle_damp(case, U, act, 0.0, 0.0, 5e-3), run_ref_damp(ref, case, U, act, 0.0, 0.0, 5e-3)) <= 1e-13
 
 
 @pytest.mark.parametrize("case", ["brick", "quad"])

@@ -358,9 +358,12 @@ int xg_uniform_excitation(xg_domain* h, int dir, int npts, double dt, const doub
 int xg_rayleigh(xg_domain* h, double alphaM, double betaK, double betaKinit, double betaKcomm) {
   return guard([&] {
     auto& d = D(h);
-    if (betaK != 0.0 || betaKinit != 0.0 || betaKcomm != 0.0)
-      throw xg::Error(XG_ERR_ARG, "rayleigh: only mass-proportional damping (alphaM) is supported");
+    if (betaKinit != 0.0)
+      throw xg::Error(XG_ERR_ARG, "rayleigh: betaKinit (initial-stiffness damping) is not supported");
     d.alphaM = alphaM;
+    d.betaK = betaK;
+    d.betaKc = betaKcomm;
+    if (d.ready) d.rayleigh_kc_init();
     return 0;
   });
 }
@@ -868,6 +871,8 @@ int xg_adjoint_loss(xg_domain* h, int n, const int* node_tags, int dof) {
 int xg_adjoint_forward(xg_domain* h, int nsteps, double dt, int ckpt, double* loss) {
   return guard([&] {
     auto& d = D(h);
+    if (d.betaK != 0.0 || d.betaKc != 0.0)
+      throw xg::Error(XG_ERR_ARG, "adjoint: stiffness-proportional damping is not differentiated (rayleigh alphaM only)");
     const double L = ADJ(h).forward(d, nsteps, dt, ckpt);
     if (loss) *loss = L;
     return 0;

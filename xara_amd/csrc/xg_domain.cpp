@@ -561,7 +561,10 @@ void Domain::setup() {
   lambda = 0;
   time = 0;
   dc_ready = false;
+  ray_conn.clear();
+  Kc.release();
   ready = true;
+  rayleigh_kc_init();
 }
 
 // Per-equation nodal mass, influence vector of the UniformExcitation
@@ -628,6 +631,7 @@ int Domain::update_checked() {
 
 void Domain::form_unbalance() {
   for (auto& g : groups) g->resid(*this, Re.p + g->re_off);
+  if (stiffness_damping()) rayleigh_forces();
   if (integ == 3)
     launch_assemble_rhs_dyn(stream, num.neq, rhs_ptr.p, rhs_idx.p, Re.p, lambda, Peq.p, Meq.p, Req.p, ag, alphaM,
                             eq2dof.p, V.p, A.p, B.p);
@@ -635,8 +639,44 @@ void Domain::form_unbalance() {
     launch_assemble_rhs(stream, num.neq, rhs_ptr.p, rhs_idx.p, Re.p, lambda, Peq.p, B.p);
 }
 
+// Element::getRayleighDampingForces (Element.cpp:283-323), added to each
+// element's resisting force by getResistingForceIncInertia: (betaK K +
+// betaKc Kc) v with K the tangent of the current trial state.
+void Domain::rayleigh_forces() {
+  if (betaK != 0.0)
+    for (auto& g : groups) g->tangent(*this, Ke.p + g->ke_off);
+  if (ray_conn.size() != groups.size()) {
+    ray_conn.clear();
+    for (auto& g : groups) {
+      ray_conn.emplace_back(new DVec<int>());
+      ray_conn.back()->upload(g->conn, stream);
+    }
+  }
+  for (size_t k = 0; k < groups.size(); k++) {
+    Group& g = *groups[k];
+    if (g.nd != g.nen * ndf) throw Error(XG_ERR_ARG, "rayleigh: element DOFs differ from the nodes' DOFs");
+    launch_rayleigh_force(stream, g.nele(), g.nen, ndf, ray_conn[k]->p, Ke.p + g.ke_off,
+                          betaKc != 0.0 ? Kc.p + g.ke_off : nullptr, betaK, betaKc, V.p, Re.p + g.re_off);
+  }
+}
+
+// Element::setRayleighDampingFactors (Element.cpp:167-176): with betaKc the
+// committed tangent starts as the tangent of the current state.
+void Domain::rayleigh_kc_init() {
+  if (betaKc == 0.0) {
+    Kc.release();
+    return;
+  }
+  if (Kc.n == (size_t)ke_total && Kc.p) return;
+  Kc.resize(ke_total);
+  for (auto& g : groups) g->tangent(*this, Kc.p + g->ke_off);
+  XG_HIP(hipStreamSynchronize(stream));
+}
+
 void Domain::form_tangent() {
   for (auto& g : groups) g->tangent(*this, Ke.p + g->ke_off);
+  // Newmark::formEleTangent: K + c2 (betaK K + betaKc Kc) per element entry
+  if (stiffness_damping()) launch_ke_damp(stream, ke_total, nm_c2, betaK, betaKc, Kc.p, Ke.p);
   launch_assemble_csr(stream, (long long)num.colind.size(), asm_ptr.p, asm_idx.p, Ke.p, vals.p);
   sym_stale = true;
   // Newmark::formNodTangent: c2 C + c3 M of the nodal masses (C = alphaM M)
@@ -748,6 +788,9 @@ void Domain::incr_disp(const double* dUdev, double scale) {
 }
 
 void Domain::commit() {
+  // Element::commitState (Element.cpp:74-80) first: Kc = getTangentStiff()
+  if (betaKc != 0.0 && Kc.p)
+    for (auto& g : groups) g->tangent(*this, Kc.p + g->ke_off);
   for (auto& g : groups) g->commit(*this);
   Uc.copy_from(U, stream);
   Vc.copy_from(V, stream);

@@ -152,9 +152,19 @@ struct Domain {
   bool dc_ready = false;
   DVec<double> phat, dUhat, dUbar, dU;
   double lambda = 0, time = 0, dt = 0;
-  // transient (Newmark, UniformExcitation + PathSeries, mass-proportional damping)
+  // transient (Newmark, UniformExcitation + PathSeries, Rayleigh damping)
   double nm_gamma = 0.5, nm_beta = 0.25, nm_c2 = 0, nm_c3 = 0;
   double alphaM = 0;
+  // stiffness-proportional Rayleigh factors on the current tangent (betaK)
+  // and on the tangent of the last commit (betaKc, stored per element in
+  // Kc with the layout of Ke, Element.cpp:74-80,167-176); ray_conn: each
+  // group's connectivity on the device for the damping force
+  double betaK = 0, betaKc = 0;
+  DVec<double> Kc;
+  std::vector<std::unique_ptr<DVec<int>>> ray_conn;
+  bool stiffness_damping() const { return integ == 3 && (betaK != 0.0 || betaKc != 0.0); }
+  void rayleigh_kc_init();
+  void rayleigh_forces();
   int gm_dir = -1;
   double gm_dt = 0, gm_factor = 1, ag = 0;
   std::vector<double> gm_accel;

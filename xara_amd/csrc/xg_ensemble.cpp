@@ -38,6 +38,8 @@ void Ensemble::add(const Domain& d) {
     for (int j = 0; j < 3; j++)
       if (kv.second.P[j] != 0.0) throw Error(XG_ERR_ARG, "ensemble: nodal loads are not supported");
   if (d.gm_dir < 0 || d.gm_accel.empty()) throw Error(XG_ERR_ARG, "ensemble: models need a UniformExcitation");
+  if (d.betaK != 0.0 || d.betaKc != 0.0)
+    throw Error(XG_ERR_ARG, "ensemble: members take mass-proportional damping only (rayleigh alphaM)");
   auto m = std::make_unique<ModelDef>();
   m->nodes = d.nodedefs;
   m->steel = d.steeldefs;

@@ -26,6 +26,12 @@ void launch_quad_resid(hipStream_t s, int nele, const int* conn, const double* c
 void launch_quad_tangent(hipStream_t s, int nele, const int* conn, const double* crd, const double* thk,
                          long long npts, const double* D, double* Ke);
 
+// ---- Rayleigh stiffness-proportional damping (xg_rayleigh.hip) ----
+void launch_ke_damp(hipStream_t s, long long n, double c2, double betaK, double betaKc, const double* Kc,
+                    double* Ke);
+void launch_rayleigh_force(hipStream_t s, int nele, int nen, int ndf, const int* conn, const double* Kt,
+                           const double* Kc, double betaK, double betaKc, const double* V, double* Re);
+
 // ---- assembly and PCG (xg_solver.hip) ----
 void launch_assemble_csr(hipStream_t s, long long nnz, const int* ptr, const int* idx, const double* Ke,
                          double* vals);
