@@ -161,3 +161,77 @@ def test_unsupported_objects_are_refused():
                         ({}, {"constraints": [{"name": 1, "node": 1, "dof": 1, "ref_value": 0.5}]})]:
         with pytest.raises(XaraError):
             model_io.load_json(_oracle.oracle_model(2, 2), doc(props, dict(base, **geom)))
+
+
+# ---- a document the reference itself wrote ----
+# tests/golden/fmk_dmg_model.json is the reference's own `print -json` output
+# held by its test tree (tests/Compatibility/fmk-dmg/model.json): a 2D steel
+# moment frame of 1,133 nodes whose elements (ElasticBeam2d,
+# ComponentElement2d, ZeroLength), materials (Steel01, Bilin,
+# ElasticMaterial) and P-Delta transformations lie off the GPU path.
+FMK = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "fmk_dmg_model.json")
+
+
+def fmk_doc():
+    with open(FMK) as f:
+        return json.load(f)
+
+
+def test_reference_document_refused_by_name():
+    """The whole document: the first object off the path (uniaxialMaterial
+    1, a Steel01) is refused with its name and type, nothing skipped."""
+    with pytest.raises(XaraError, match="uniaxialMaterial 1 of type Steel01"):
+        model_io.load_json(_oracle.oracle_model(2, 3), fmk_doc())
+
+
+def test_reference_document_nodes_and_transformations():
+    """Its node and crdTransformation sections: 1,133 nodes with their
+    coordinates and masses, the LinearCrdTransf2d read, the
+    PDeltaCrdTransf2d ones and every element type refused by name.  The
+    coordinates are checked through an elastic dispBeamColumn between two
+    of the document's nodes, whose tangent must equal the one built from
+    the coordinates directly."""
+    d = fmk_doc()["StructuralAnalysisModel"]
+    nodes = d["geometry"]["nodes"]
+    transfs = d["properties"]["crdTransformations"]
+    linear = [t for t in transfs if t["type"] == "LinearCrdTransf2d"]
+    assert len(nodes) == 1133 and len(linear) == 1
+    sub = {"StructuralAnalysisModel": {"properties": {"crdTransformations": linear},
+                                       "geometry": {"nodes": nodes}}}
+    m = model_io.load_json(_oracle.oracle_model(2, 3), sub)
+    assert m.num_nodes == 1133
+    # an elastic fiber element between two document nodes
+    a, b = nodes[1], nodes[200]
+    ya = np.array([-150.0, 150.0])
+    for mm in (m,):
+        mm.uniaxialMaterial("Elastic", 1, 29000.0)
+        mm.section("Fiber", 1, [(y, 0.0, 40.0, 1) for y in ya])
+        mm.element("dispBeamColumn", 1, int(a["name"]), int(b["name"]), int(linear[0]["name"]), "Legendre", 1, 5)
+    m.setup()
+    assert m.neq == 3 * 1133
+    K = m.element_tangent(1)
+    direct = _oracle.oracle_model(2, 3)
+    direct.node(1, *[float(x) for x in a["crd"]])
+    direct.node(2, *[float(x) for x in b["crd"]])
+    direct.uniaxialMaterial("Elastic", 1, 29000.0)
+    direct.section("Fiber", 1, [(y, 0.0, 40.0, 1) for y in ya])
+    direct.geomTransf("Linear", 1)
+    direct.element("dispBeamColumn", 1, 1, 2, 1, "Legendre", 1, 5)
+    direct.setup()
+    Kd = direct.element_tangent(1)
+    assert np.array_equal(K, Kd) and np.abs(Kd).max() > 0
+    for t in transfs:
+        if t["type"] == "PDeltaCrdTransf2d":
+            bad = {"StructuralAnalysisModel": {"properties": {"crdTransformations": [t]},
+                                               "geometry": {"nodes": nodes[:2]}}}
+            with pytest.raises(XaraError, match=f"geomTransf {t['name']} of type PDeltaCrdTransf2d"):
+                model_io.load_json(_oracle.oracle_model(2, 3), bad)
+    seen = set()
+    for e in d["geometry"]["elements"]:
+        if e["type"] in seen:
+            continue
+        seen.add(e["type"])
+        one = {"StructuralAnalysisModel": {"properties": {}, "geometry": {"nodes": nodes, "elements": [e]}}}
+        with pytest.raises(XaraError, match=f"element {e['name']} of type {e['type']}"):
+            model_io.load_json(_oracle.oracle_model(2, 3), one)
+    assert seen == {"ElasticBeam2d", "ComponentElement2d", "ZeroLength"}

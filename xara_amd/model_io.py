@@ -107,7 +107,8 @@ def load_json(model, doc):
         model.fix_many(ft, np.array([fix[t] for t in ft], np.int32))
 
     # ---- materials ----
-    for m in props.get("nDMaterials", []):
+    # (older printers wrote the key as "ndMaterials")
+    for m in props.get("nDMaterials", props.get("ndMaterials", [])):
         t = m.get("type")
         if t != "J2Plasticity":
             raise _err(f"nDMaterial {m.get('name')} of type {t} is not on the GPU path")
