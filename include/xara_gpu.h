@@ -231,6 +231,14 @@ int  xg_update(xg_domain* d);
    SparseGenRowLinSOE::addA (SparseGenRowLinSOE.cpp:237-291): K into the CSR
    values; copied to vals (nnz) when vals != NULL.                            */
 int  xg_form_tangent(xg_domain* d, double* vals);
+/* LinearSOE::addA (SparseGenRowLinSOE::addA, SparseGenRowLinSOE.cpp:237-291)
+   for contributions other than the element tangents -- the DOF_Group
+   blocks of TransientIntegrator::formTangent (TransientIntegrator.cpp:88-95)
+   -- after xg_form_tangent: nblk blocks of bs x bs values (row-major, the
+   factor already applied) added at the CSR entries of their equation
+   numbers id (nblk*bs, negative ids skipped).  An entry outside the
+   pattern is XG_ERR_ARG.                                                    */
+int  xg_add_a(xg_domain* d, int nblk, int bs, const double* m, const int* id);
 /* StaticIntegrator::formUnbalance (StaticIntegrator.cpp:51-70): B = -sum R_e
    + lambda P; copied to B (neq) when B != NULL.                              */
 int  xg_form_unbalance(xg_domain* d, double* B);

@@ -1513,6 +1513,23 @@ int orc_element_damp(xg_domain* d, int ele_tag, double* D, int* ndof) {
   *ndof = nd;
   return 0;
 }
+// LinearSOE::addA into the assembled CSR (SparseGenRowLinSOE.cpp:237-291)
+int orc_add_a(xg_domain* d, int nblk, int bs, const double* m, const int* id) {
+  if (!d->ready) return fail(XG_ERR_STATE, "setup first");
+  for (int b = 0; b < nblk; b++)
+    for (int i = 0; i < bs; i++) {
+      const int r = id[(size_t)b * bs + i];
+      if (r < 0) continue;
+      for (int j = 0; j < bs; j++) {
+        const int c = id[(size_t)b * bs + j];
+        if (c < 0) continue;
+        const long p = csr_pos(*d, r, c);
+        if (p >= d->rowptr[r + 1] || d->colind[p] != c) return fail(XG_ERR_ARG, "addA: entry outside the CSR pattern");
+        d->Aval[p] += m[((size_t)b * bs + i) * bs + j];
+      }
+    }
+  return 0;
+}
 int orc_element_resisting_force(xg_domain* d, int ele_tag, double* R, int* ndof) {
   auto it = d->elemap.find(ele_tag);
   if (it == d->elemap.end()) return fail(XG_ERR_ARG, "element not found");

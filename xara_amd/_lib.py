@@ -56,6 +56,7 @@ SIGNATURES = {
     "uniform_excitation": (C.c_int, [VP, C.c_int, C.c_int, C.c_double, P_D, C.c_double]),
     "rayleigh": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_double]),
     "system": (C.c_int, [VP, C.c_int, C.c_double, C.c_int]),
+    "add_a": (C.c_int, [VP, C.c_int, C.c_int, P_D, P_I]),
     "numberer": (C.c_int, [VP, C.c_int]),
     "spmv_config": (C.c_int, [VP, C.c_int, C.c_int]),
     "cg_config": (C.c_int, [VP, C.c_int, C.c_int]),

@@ -1,7 +1,9 @@
 // extern "C" boundary of the library (declared in include/xara_gpu.h).
 // Every entry point converts exceptions into an int status and records the
 // message for xg_last_error().
+#include <algorithm>
 #include <cstring>
+#include <map>
 #include <string>
 
 #include <memory>
@@ -552,6 +554,46 @@ int xg_form_unbalance(xg_domain* h, double* B) {
     d.form_unbalance();
     if (B) d.B.download(B, d.num.neq, d.stream);
     else XG_HIP(hipStreamSynchronize(d.stream));
+    return 0;
+  });
+}
+
+int xg_add_a(xg_domain* h, int nblk, int bs, const double* m, const int* id) {
+  return guard([&] {
+    auto& d = R(h);
+    if (nblk < 0 || bs < 1 || (nblk > 0 && (!m || !id))) throw xg::Error(XG_ERR_ARG, "addA: bad block arguments");
+    // positions of the entries, duplicates summed in call order
+    std::map<long long, double> acc;
+    const int neq = d.num.neq;
+    for (int b = 0; b < nblk; b++)
+      for (int i = 0; i < bs; i++) {
+        const int r = id[(size_t)b * bs + i];
+        if (r < 0) continue;
+        if (r >= neq) throw xg::Error(XG_ERR_ARG, "addA: equation number out of range");
+        const int* cb = d.num.colind.data() + d.num.rowptr[r];
+        const int* ce = d.num.colind.data() + d.num.rowptr[r + 1];
+        for (int j = 0; j < bs; j++) {
+          const int c = id[(size_t)b * bs + j];
+          if (c < 0) continue;
+          const int* it = std::lower_bound(cb, ce, c);
+          if (it == ce || *it != c) throw xg::Error(XG_ERR_ARG, "addA: entry outside the CSR pattern");
+          acc[(long long)(it - d.num.colind.data())] += m[((size_t)b * bs + i) * bs + j];
+        }
+      }
+    if (acc.empty()) return 0;
+    std::vector<long long> pos;
+    std::vector<double> val;
+    for (auto& kv : acc) {
+      pos.push_back(kv.first);
+      val.push_back(kv.second);
+    }
+    xg::DVec<long long> dpos;
+    xg::DVec<double> dval;
+    dpos.upload(pos, d.stream);
+    dval.upload(val, d.stream);
+    xg::launch_add_diag(d.stream, (int)pos.size(), dpos.p, dval.p, 1.0, d.vals.p);
+    d.sym_stale = true;
+    XG_HIP(hipStreamSynchronize(d.stream));
     return 0;
   });
 }
