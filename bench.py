@@ -46,23 +46,66 @@ def synthetic_field(mesh):
     return U
 
 
+def host_cpu():
+    """CPU model and physical core count of this host (lscpu / cpuinfo)."""
+    model, cores = "unknown", None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+        import subprocess
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True).stdout
+        pairs = {ln for ln in out.splitlines() if ln and not ln.startswith("#")}
+        cores = len(pairs) or None
+    except OSError:
+        pass
+    return model, cores
+
+
 def cpu_baseline(n_sample, seconds):
+    """The oracle's update of an n_sample^3 cube with the same synthetic
+    field, built here with -O3 -march=native and OpenMP over elements, timed
+    on one thread (the reference's serial Domain::update) and on every CPU
+    this process may run on; `value` is the all-threads rate."""
+    import shutil
+    import tempfile
+
     from tests import _oracle
     from xara_amd import meshes
-    mesh = meshes.brick_cube(n_sample)
-    o = _oracle.oracle_model(3, 3)
-    meshes.build_brick_cube(o, mesh, pressure=0.0)
-    o.setup()
-    o.set_trial_disp(synthetic_field(mesh))
-    o.update()
-    reps = 1
-    ms = o.time_op("update", reps)
-    reps = max(1, int(seconds * 1000.0 / max(ms, 1e-3)))
-    ms = o.time_op("update", reps)
-    npts = 8 * n_sample ** 3
-    return dict(value=npts / (ms * 1e-3), unit="points/s", cores=1, kind="port",
-                sample=f"{reps} serial updates of a {n_sample}^3 stdBrick J2 cube ({npts} points) "
-                       f"with the same synthetic field, oracle/liboracle.so (-O2), one thread")
+    tmp = tempfile.mkdtemp(prefix="xara_cpu_")
+    try:
+        path = os.path.join(tmp, "liboracle_native.so")
+        t0 = time.time()
+        _oracle.build_native(path)
+        build_s = time.time() - t0
+        b = _oracle.load_native(path)
+        mesh = meshes.brick_cube(n_sample)
+        o = _oracle.oracle_model(3, 3, binding=b)
+        meshes.build_brick_cube(o, mesh, pressure=0.0)
+        o.setup()
+        o.set_trial_disp(synthetic_field(mesh))
+        npts = 8 * n_sample ** 3
+        threads = len(os.sched_getaffinity(0))
+
+        def rate(nt, budget):
+            b.set_threads(nt)
+            o.update()
+            ms = o.time_op("update", 1)
+            reps = max(2, int(budget * 1000.0 / max(ms, 1e-3)))
+            ms = o.time_op("update", reps)
+            return npts / (ms * 1e-3), reps
+
+        one, reps1 = rate(1, seconds / 2)
+        allc, repsn = rate(threads, seconds / 2)
+        o.close()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    model, phys = host_cpu()
+    return dict(value=allc, unit="points/s", cores=threads, kind="port", one_thread=one,
+                physical_cores=phys, cpu=model, build=_oracle.NATIVE_FLAGS, build_s=round(build_s, 1),
+                sample=f"oracle update of a {n_sample}^3 stdBrick J2 cube ({npts} points) with the same synthetic "
+                       f"field: {repsn} updates on {threads} threads (OpenMP over elements), {reps1} on one thread")
 
 
 def main():
@@ -71,7 +114,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=64, help="cube edge in elements")
-    ap.add_argument("--cpu-n", type=int, default=24, help="cube edge of the CPU sample")
+    ap.add_argument("--cpu-n", type=int, default=32, help="cube edge of the CPU sample")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",

@@ -20,6 +20,9 @@
 #include <memory>
 #include <string>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "../include/xara_gpu.h"
 
@@ -450,7 +453,14 @@ static long csr_pos(const Dom& d, int row, int col) {
 // ---------------------------------------------------------------------------
 static int domain_update(Dom& d) {
   int res = 0;
-  for (auto* e : d.eles) res += e->update(d);
+  // Domain::update is a serial element loop (Domain.cpp:1988-2006); the
+  // elements are independent, so a build with OpenMP (the all-cores CPU
+  // baseline) splits the loop over threads with the same per-element results
+  const long ne = (long)d.eles.size();
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : res) schedule(static)
+#endif
+  for (long i = 0; i < ne; i++) res += d.eles[i]->update(d);
   d.counters.updates++;
   for (auto* e : d.eles) d.counters.material_points += e->npoints();
   return res;
@@ -1492,6 +1502,18 @@ int orc_element_tangent(xg_domain* d, int ele_tag, double* K, int* ndof) {
   *ndof = it->second->ndof();
   return 0;
 }
+// Threads of the element loop in an OpenMP build (the CPU baseline); returns
+// the count in effect, 1 in a serial build.
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 // Element::getDamp (Element.cpp:181-200) of one element: alphaM M, then
 // + betaK K, + betaKcomm Kc -- the matrix the transient tangent and the
 // damping force use.  Oracle-only (test infrastructure).

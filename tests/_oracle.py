@@ -80,9 +80,32 @@ def material_jvp(kind, prm, h, kon, strain, dh, dtheta, dstrain):
     return hout, kout.value, dhout
 
 
-def oracle_model(ndm, ndf=None):
+def oracle_model(ndm, ndf=None, binding=None):
     from xara_amd.model import Model
-    return Model(ndm, ndf, _binding=load())
+    return Model(ndm, ndf, _binding=binding or load())
+
+
+def build_native(path):
+    """The oracle built for timing on the host it runs on: -O3
+    -march=native, OpenMP over the element loop of Domain::update."""
+    src = os.path.join(ROOT, "oracle", "orc_domain.cpp")
+    subprocess.run(["g++", "-O3", "-march=native", "-fopenmp", "-std=c++17", "-fPIC", "-shared", "-o", path, src],
+                   check=True)
+
+
+NATIVE_FLAGS = "g++ -O3 -march=native -fopenmp"
+
+
+def load_native(path):
+    """Binding of an oracle built by build_native, with set_threads(n)."""
+    from xara_amd import _lib
+    lib = C.CDLL(path)
+    b = _lib.bind(lib, "orc_", path)
+    f = lib.orc_set_threads
+    f.argtypes = [C.c_int]
+    f.restype = C.c_int
+    b.set_threads = f
+    return b
 
 
 class Ref:
