@@ -260,15 +260,20 @@ def test_pcg_graph_replay_matches_eager_launches():
 
 @pytest.mark.parametrize("system", ["GpuPCG", "GpuBlockPCG"])
 def test_single_workgroup_pcg_matches_launch_per_step(system):
-    """The whole-solve kernel of small systems (one workgroup, one launch)
-    runs the same iteration as the kernel-per-step PCG: the same converged
-    trajectories and iteration counts within rounding's reach, over a frame
-    pushover (block width 3) and a quad strip (2)."""
+    """The whole-solve kernels of small systems (one workgroup, one launch)
+    against the kernel-per-step PCG: the same converged trajectories over a
+    2D frame pushover (3 DOFs per node, half bandwidth 11) and a quad strip
+    (2, 19), both taken by the band-Cholesky-preconditioned solve, which
+    then needs at most three iterations per solve; and over a 3x3x3 3D
+    frame (half bandwidth 77 > 63, taken by the (block-)Jacobi
+    one-workgroup solve), whose iteration counts agree within rounding's
+    reach."""
     fr = meshes.rc_frame_2d(3, 2)
     conn, _ = meshes.frame_elements(fr)
     theta = meshes.default_theta(len(conn))
     mesh = meshes.quad_strip(16, 4)
-    for case in ("frame", "quad"):
+    fr3 = meshes.rc_frame_3d(3, 3, 3)
+    for case in ("frame", "quad", "frame3d"):
         print(f"case: small pcg {system} {case}", flush=True)
         res = []
         for small in (True, False):
@@ -279,11 +284,15 @@ def test_single_workgroup_pcg_matches_launch_per_step(system):
                 P[:, 0] = 1.0
                 m.loads(fr["floor_nodes"], P)
                 node, dof, du, tol = fr["roof"], 1, 6.0, 1e-5
-            else:
+            elif case == "quad":
                 m = Model(2, 2)
                 meshes.build_quad_strip(m, mesh)
                 m.load(mesh["tip"], 0.0, 1.0)
                 node, dof, du, tol = mesh["tip"], 2, 0.001, 1e-8
+            else:
+                m = Model(3, 6)
+                meshes.build_rc_frame_3d(m, fr3)
+                node, dof, du, tol = fr3["roof"], 1, 6.0, 1e-5
             m.system(system, 1e-12, 100000)
             m.cg_config(16, True, True, small=small)
             m.test("NormUnbalance", tol, 40)
@@ -293,13 +302,17 @@ def test_single_workgroup_pcg_matches_launch_per_step(system):
             for step in range(6):
                 assert m.analyze(1) == 0
                 t.append(m.trial_disp().copy())
-            res.append((np.array(t), m.counters()["cg_iters"]))
+            c = m.counters()
+            res.append((np.array(t), c["cg_iters"], c["solves"]))
             m.close()
-        (a, ia), (b, ib) = res
+        (a, ia, sa), (b, ib, sb) = res
         assert relerr(a, b) <= 1e-9
-        # the sums' order differs, and on an ill-conditioned frame rounding
-        # moves the iteration at which a solve crosses its tolerance
-        assert abs(ia - ib) <= 0.1 * ib
+        if case == "frame3d":
+            # the sums' order differs, and on an ill-conditioned frame rounding
+            # moves the iteration at which a solve crosses its tolerance
+            assert abs(ia - ib) <= 0.1 * ib
+        else:
+            assert ia <= 3 * sa and ib > 3 * sb
 
 
 def test_revert_to_start_and_to_last_commit():

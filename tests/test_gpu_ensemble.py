@@ -27,7 +27,7 @@ def define(m, case, tol=1e-9, maxit=30):
     nst, nbay, seed, pga = case
     fr = meshes.rc_frame_2d(nst, nbay)
     ag = ground_motion.kanai_tajimi(seed, 400, DT, pga)
-    meshes.define_transient_rc_frame_2d(m, fr, ag, DT, tol=tol, maxit=maxit)
+    meshes.define_transient_rc_frame_2d(m, fr, ag, DT, mass=60.0, tol=tol, maxit=maxit)
     return fr
 
 

@@ -593,6 +593,7 @@ int xg_add_a(xg_domain* h, int nblk, int bs, const double* m, const int* id) {
     dval.upload(val, d.stream);
     xg::launch_add_diag(d.stream, (int)pos.size(), dpos.p, dval.p, 1.0, d.vals.p);
     d.sym_stale = true;
+    d.pc_done = nullptr;
     XG_HIP(hipStreamSynchronize(d.stream));
     return 0;
   });

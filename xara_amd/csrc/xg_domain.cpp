@@ -557,6 +557,8 @@ void Domain::setup() {
   // the PCG's block-symmetric layout depends only on the pattern: built here
   // with the other index maps rather than in the first solve
   if (spmv_format != XG_SPMV_CSR) sym_prepare(true);
+  band_order();
+  pc_done = nullptr;
   XG_HIP(hipStreamSynchronize(s));
   lambda = 0;
   time = 0;
@@ -679,14 +681,91 @@ void Domain::form_tangent() {
   if (stiffness_damping()) launch_ke_damp(stream, ke_total, nm_c2, betaK, betaKc, Kc.p, Ke.p);
   launch_assemble_csr(stream, (long long)num.colind.size(), asm_ptr.p, asm_idx.p, Ke.p, vals.p);
   sym_stale = true;
+  pc_done = nullptr;
   // Newmark::formNodTangent: c2 C + c3 M of the nodal masses (C = alphaM M)
   if (integ == 3) launch_add_diag(stream, nmass_eq, mass_pos.p, mass_val.p, nm_c3 + nm_c2 * alphaM, vals.p);
 }
 
-void Domain::precond_setup() {
-  const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
-  pcbad.zero(stream);
-  launch_block_inverse(stream, nblk, blk_first.p, blk_size.p, rowptr.p, colind.p, vals.p, bw, binv.p, pcbad.p);
+Domain::CgBufs Domain::cg_bufs() {
+  return CgBufs{r0.p, r1.p, z.p, p.p, p2.p, q.p, part.p, binv.p, pcbad.p, scal.p, band_L.p, band_Lr.p};
+}
+
+// The preconditioner of the current matrix into w: the band-Cholesky factor
+// for a system the one-workgroup band solve takes, else the (block-)Jacobi
+// inverses.  Formed once per matrix and buffer set.
+void Domain::precond_setup(const CgBufs& w) {
+  const bool band = band_active();
+  const void* key = band ? (const void*)w.bandL : (const void*)w.binv;
+  if (pc_done == key) return;
+  XG_HIP(hipMemsetAsync(w.pcbad, 0, sizeof(int), stream));
+  if (band) {
+    band_factor(stream, num.neq, band_bw, rowptr.p, colind.p, vals.p, band_pinv.p, w.bandL, w.bandLr, w.pcbad);
+  } else {
+    const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
+    launch_block_inverse(stream, nblk, blk_first.p, blk_size.p, rowptr.p, colind.p, vals.p, bw, w.binv, w.pcbad);
+  }
+  pc_done = key;
+}
+
+// Half bandwidth of the pattern under an equation order (pinv: position of
+// each equation).
+static int band_width(const Numbering& num, const std::vector<int>& pinv) {
+  int bw = 0;
+  for (int r = 0; r < num.neq; r++)
+    for (int q = num.rowptr[r]; q < num.rowptr[r + 1]; q++)
+      bw = std::max(bw, std::abs(pinv[r] - pinv[num.colind[q]]));
+  return bw;
+}
+
+void Domain::band_order() {
+  band_bw = -1;
+  band_pinv.release();
+  band_L.release();
+  band_Lr.release();
+  const int n = num.neq;
+  if (n == 0 || n > kBandMaxEqn) return;
+  std::vector<int> ident(n);
+  for (int i = 0; i < n; i++) ident[i] = i;
+  int best = band_width(num, ident);
+  std::vector<int> pinv = ident;
+  // nodes sorted along the longest extent of the model, then the other
+  // axes, then DOF: the band of a slender mesh is its short side
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < nnode; i++)
+    for (int k = 0; k < ndm; k++) {
+      lo[k] = std::min(lo[k], crd_h[(size_t)i * ndm + k]);
+      hi[k] = std::max(hi[k], crd_h[(size_t)i * ndm + k]);
+    }
+  int ax[3] = {0, 1, 2};
+  std::sort(ax, ax + ndm, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+  std::vector<int> nodes(nnode);
+  for (int i = 0; i < nnode; i++) nodes[i] = i;
+  std::stable_sort(nodes.begin(), nodes.end(), [&](int a, int b) {
+    for (int k = 0; k < ndm; k++) {
+      const double xa = crd_h[(size_t)a * ndm + ax[k]], xb = crd_h[(size_t)b * ndm + ax[k]];
+      if (xa != xb) return xa < xb;
+    }
+    return false;
+  });
+  std::vector<int> geo(n, -1);
+  int k = 0;
+  for (int nd : nodes)
+    for (int j = 0; j < ndf; j++) {
+      const int e = num.eq[(size_t)nd * ndf + j];
+      if (e >= 0) geo[e] = k++;
+    }
+  if (k == n) {
+    const int g = band_width(num, geo);
+    if (g < best) {
+      best = g;
+      pinv = geo;
+    }
+  }
+  if (best > 63) return;
+  band_bw = best;
+  band_pinv.upload(pinv, stream);
+  band_L.resize((size_t)n * (best + 1));
+  band_Lr.resize((size_t)n * (best + 1));
 }
 
 // Destroys the captured chunks once the stream has drained: a replay still
@@ -703,22 +782,22 @@ void Domain::cg_graph_clear() {
 // small systems.  Graphs hold kernels only, one per chunk length and set of
 // launch arguments (at most kCgGraphs; further ones are launched one by
 // one).
-void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
+void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x, const CgBufs& w, bool graphs) {
   const int n = num.neq, nbm = spmv_nbm();
   const SymSpmv* sym = sym_prepare(true);
   auto eager = [&](hipStream_t s, int k, int i0) {
-    cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, r0.p, r1.p, z.p, p.p, p2.p, q.p, bw, binv.p,
-                  bfirst.p, bsize.p, part.p, scal.p, nbm, nbcol.p, cg_fuse_on, sym);
+    cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, w.r0, w.r1, w.z, w.p, w.p2, w.q, bw, w.binv,
+                  bfirst.p, bsize.p, w.part, w.scal, nbm, nbcol.p, cg_fuse_on, sym);
   };
-  if (!cg_graph_on || (iters & 1) || (it0 & 1)) {
+  if (!graphs || (iters & 1) || (it0 & 1)) {
     eager(stream, iters, it0);
     return;
   }
   const std::vector<uintptr_t> key = {
       (uintptr_t)iters,   (uintptr_t)L,       (uintptr_t)n,       (uintptr_t)bw,      (uintptr_t)nbm, (uintptr_t)cg_fuse_on,
-      (uintptr_t)x,       (uintptr_t)rowptr.p, (uintptr_t)colind.p, (uintptr_t)vals.p, (uintptr_t)r0.p,
-      (uintptr_t)r1.p,    (uintptr_t)z.p,     (uintptr_t)p.p,     (uintptr_t)p2.p,    (uintptr_t)q.p,     (uintptr_t)binv.p,
-      (uintptr_t)bfirst.p, (uintptr_t)bsize.p, (uintptr_t)part.p, (uintptr_t)scal.p,  (uintptr_t)nbcol.p,
+      (uintptr_t)x,       (uintptr_t)rowptr.p, (uintptr_t)colind.p, (uintptr_t)vals.p, (uintptr_t)w.r0,
+      (uintptr_t)w.r1,    (uintptr_t)w.z,     (uintptr_t)w.p,     (uintptr_t)w.p2,    (uintptr_t)w.q,     (uintptr_t)w.binv,
+      (uintptr_t)bfirst.p, (uintptr_t)bsize.p, (uintptr_t)w.part, (uintptr_t)w.scal,  (uintptr_t)nbcol.p,
       (uintptr_t)(sym ? sym->val : nullptr), (uintptr_t)(sym ? sym->lanes : 0)};
   for (auto& g : cg_graphs)
     if (g.key == key) {
@@ -739,27 +818,33 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x) {
   XG_HIP(hipGraphLaunch(g.exec, stream));
 }
 
-int Domain::solve(const double* b, double* x) {
+int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs) {
   const int n = num.neq;
   if (n == 0) return 0;
   const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
-  precond_setup();
+  precond_setup(w);
   int bad = 0;
-  pcbad.download(&bad, 1, stream);
+  XG_HIP(hipMemcpyAsync(&bad, w.pcbad, sizeof(int), hipMemcpyDeviceToHost, stream));
+  XG_HIP(hipStreamSynchronize(stream));
   if (bad) return XG_BAD_LINEAR_SOLVE;
   int conv = 0, it = 0, brk = 0;
   double rr = 0, bb = 0;
-  if (cg_small_on && cg_small_fits(n, (long long)num.colind.size())) {
+  if (band_active()) {
+    // one workgroup, one launch, preconditioned by the band's exact factor
+    cg_solve_band(stream, n, rowptr.p, colind.p, vals.p, b, x, w.r0, w.z, w.p, w.q, w.r1, band_bw, band_pinv.p,
+                  w.bandL, w.bandLr, rtol * rtol, maxit, w.scal);
+    cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);
+  } else if (cg_small_on && cg_small_fits(n, (long long)num.colind.size())) {
     // the whole solve in one workgroup, one launch
-    cg_solve_small(stream, n, rowptr.p, colind.p, vals.p, b, x, r0.p, z.p, p.p, q.p, bw, binv.p, bfirst.p, bsize.p,
-                   rtol * rtol, maxit, scal.p);
-    cg_read_scal(stream, scal.p, &conv, &it, &rr, &bb, &brk);
+    cg_solve_small(stream, n, rowptr.p, colind.p, vals.p, b, x, w.r0, w.z, w.p, w.q, bw, w.binv, bfirst.p, bsize.p,
+                   rtol * rtol, maxit, w.scal);
+    cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);
   } else {
-    cg_start(stream, n, b, x, r0.p, z.p, p.p, bw, binv.p, bfirst.p, bsize.p, part.p, rtol * rtol, maxit, scal.p);
+    cg_start(stream, n, b, x, w.r0, w.z, w.p, bw, w.binv, bfirst.p, bsize.p, w.part, rtol * rtol, maxit, w.scal);
     int issued = 0, next = cg_chunk;
     const int L = lanes();
     while (true) {
-      cg_read_scal(stream, scal.p, &conv, &it, &rr, &bb, &brk);
+      cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);
       if (conv) break;
       // maxit + 1 iterations at most: the fused iteration tests the previous
       // update's convergence at its start, so the last test comes one
@@ -770,7 +855,7 @@ int Domain::solve(const double* b, double* x) {
       // host round trips on long solves; iterations issued past
       // convergence return at once
       const int chunk = std::min(next, maxit + 1 - issued);
-      cg_chunk_run(chunk, issued, L, bw, x);
+      cg_chunk_run(chunk, issued, L, bw, x, w, graphs);
       issued += chunk;
       next = std::min(2 * next, 8 * cg_chunk);
     }

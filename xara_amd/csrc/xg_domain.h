@@ -183,7 +183,33 @@ struct Domain {
   void update();
   void form_unbalance();
   void form_tangent();
-  int solve(const double* b, double* x);  // device pointers, returns <0 on failure
+  // the buffers one PCG solve works in: the domain's own (cg_bufs()) or a
+  // caller's workspace of the same sizes
+  struct CgBufs {
+    double *r0, *r1, *z, *p, *p2, *q, *part, *binv;
+    int* pcbad;
+    void* scal;
+    double *bandL, *bandLr;  // band-Cholesky factor (band_active())
+  };
+  // band-Cholesky preconditioner of small systems (taken by the one-
+  // workgroup solve, cg_small): the half bandwidth in the band order
+  // (the numbering, or the nodes sorted along the model's longest extent
+  // when that is narrower), -1 when neither is 63 wide or less or the
+  // system is larger than kBandMaxEqn
+  static constexpr int kBandMaxEqn = 16384;
+  int band_bw = -1;
+  DVec<int> band_pinv;
+  DVec<double> band_L, band_Lr;
+  bool band_active() const { return cg_small_on && band_bw >= 0; }
+  void band_order();
+  // the preconditioner's buffers last formed from the current matrix
+  // (nullptr: the matrix changed since), so repeated solves of one matrix
+  // form it once
+  const void* pc_done = nullptr;
+  CgBufs cg_bufs();
+  int solve(const double* b, double* x) { return solve_with(b, x, cg_bufs(), cg_graph_on); }  // device pointers
+  // returns < 0 on failure; graphs: replay chunks from captured graphs
+  int solve_with(const double* b, double* x, const CgBufs& w, bool graphs);
   void incr_disp(const double* dU_dev, double scale = 1.0);
   void commit();
   void revert();
@@ -200,8 +226,8 @@ struct Domain {
   int integ_update(const double* x);
   int update_checked();  // update(); < 0 if an element update failed (AnalysisModel::updateDomain)
   int run_test(int& iter, std::vector<double>& norms, int& numIncr);
-  void precond_setup();
-  void cg_chunk_run(int iters, int it0, int L, int bw, double* x);
+  void precond_setup(const CgBufs& w);
+  void cg_chunk_run(int iters, int it0, int L, int bw, double* x, const CgBufs& w, bool graphs);
   void cg_graph_clear();
   int lanes();
   std::vector<int> node_block_columns();

@@ -24,17 +24,23 @@ namespace xg {
 
 template <int NT>
 __device__ __forceinline__ double block_sum_t(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
+  // wavefront butterfly (every lane ends with the same bits: each step adds
+  // the same two operands in either order), then the NT / 64 wave sums in
+  // wave order: one barrier pair instead of a log2(NT)-level LDS tree
 #pragma unroll
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] += sh[t + s];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if constexpr (NT == 64) {
+    return v;
+  } else {
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) sh[t >> 6] = v;
     __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) r += sh[w];
+    __syncthreads();
+    return r;
   }
-  const double r = sh[0];
-  __syncthreads();
-  return r;
 }
 
 __device__ __forceinline__ double path_accel(const EnsDev& E, int m, double t) {

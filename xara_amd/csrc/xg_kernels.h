@@ -93,6 +93,14 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
 // throws XG_ERR_ARG for a block width or symmetric layout cg_iterations
 // cannot launch (called before a graph capture of cg_iterations)
 void cg_iterations_check(int bw, const SymSpmv* sym);
+// band-Cholesky preconditioner (half bandwidth bw <= 63 in the band order
+// pinv): factor L (column rows) and Lr (rows), n*(bw+1) doubles each
+void band_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* colind, const double* vals,
+                 const int* pinv, double* L, double* Lr, int* bad);
+// whole band-preconditioned PCG solve in one workgroup; t: n doubles of work
+void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
+                   double* x, double* r, double* z, double* p, double* q, double* t, int bw, const int* pinv,
+                   const double* L, const double* Lr, double tol2, int maxit, void* scal);
 // whole solve in one workgroup (k_pcg_small) for systems that fit it
 bool cg_small_fits(int n, long long nnz);
 void cg_solve_small(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,

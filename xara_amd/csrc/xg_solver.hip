@@ -102,21 +102,27 @@ __global__ void k_add_diag(int n, const long long* __restrict__ pos, const doubl
 }
 
 // ---------------------------------------------------------------------------
-// Block reductions (deterministic tree)
+// Block reductions (deterministic: fixed butterfly and wave order)
 // ---------------------------------------------------------------------------
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
+  // wavefront butterfly (every lane ends with the same bits: each step adds
+  // the same two operands in either order), then the NT / 64 wave sums in
+  // wave order: one barrier pair instead of a log2(NT)-level LDS tree
 #pragma unroll
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] += sh[t + s];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if constexpr (NT == 64) {
+    return v;
+  } else {
+    const int t = threadIdx.x;
+    if ((t & 63) == 0) sh[t >> 6] = v;
     __syncthreads();
+    double r = sh[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; w++) r += sh[w];
+    __syncthreads();
+    return r;
   }
-  double r = sh[0];
-  __syncthreads();
-  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -828,6 +834,227 @@ __global__ __launch_bounds__(kSmallPcgThreads) void k_pcg_small(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Band-Cholesky preconditioner of small, narrow-banded systems.  In the band
+// order k = pinv[e] of the equations (the numbering itself, or a coordinate
+// ordering when that is narrower) the matrix has half bandwidth bw <= 63, so
+// one 64-lane wavefront holds a whole pivot window: lane i works on row
+// j + i.  Layouts (ld = bw + 1):
+//   L  before the factorization, row k holds A(k, k-bw .. k) at offsets
+//      0 .. bw (the diagonal at bw); after it, row j holds column j of the
+//      factor, L(j + i, j) at offset i;
+//   Lr the factor by rows: L(k, k - i) at offset bw - i.
+// The factorization runs once per assembled matrix; every application of
+// the preconditioner is a forward and a backward sweep by wave 0 of the
+// solving workgroup.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__global__ __launch_bounds__(256) void k_band_fill(long long n, double* __restrict__ a) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) a[i] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_band_gather(int n, int bw, const int* __restrict__ rowptr,
+                                                     const int* __restrict__ colind, const double* __restrict__ vals,
+                                                     const int* __restrict__ pinv, double* __restrict__ L) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int rk = pinv[r], ld = bw + 1;
+  for (int q = rowptr[r]; q < rowptr[r + 1]; q++) {
+    const int ck = pinv[colind[q]];
+    if (ck <= rk && rk - ck <= bw) L[(long long)rk * ld + (ck - rk + bw)] = vals[q];
+  }
+}
+
+// Right-looking band Cholesky by one wavefront: at pivot j lane i holds
+// row j + i of the reduced window in w[0 .. BW] (w[c] = A'(j + i, j + c)).
+// A non-positive pivot sets *bad (the matrix is not positive definite).
+template <int BW>
+__global__ __launch_bounds__(64) void k_band_factor(int n, int bw, double* __restrict__ L, double* __restrict__ Lr,
+                                                    int* __restrict__ bad) {
+  const int lane = threadIdx.x;
+  const int ld = bw + 1;
+  const bool act = lane <= bw;
+  double w[BW + 1];
+#pragma unroll
+  for (int c = 0; c <= BW; c++) w[c] = 0.0;
+  if (act && lane < n) {
+    const double* row = L + (long long)lane * ld + (bw - lane);
+#pragma unroll
+    for (int c = 0; c <= BW; c++)
+      if (c <= lane) w[c] = row[c];
+  }
+  int fail = 0;
+  for (int j = 0; j < n; j++) {
+    // the row entering the window after this pivot, loaded ahead
+    const int r = j + 1 + bw;
+    const double nrow = (act && r < n) ? L[(long long)r * ld + lane] : 0.0;
+    const double a00 = lane_d(w[0], 0);
+    if (!(a00 > 0.0)) fail = 1;
+    const double d = sqrt(a00 > 0.0 ? a00 : 1.0);
+    const double l = lane == 0 ? d : (act ? w[0] / d : 0.0);
+    if (act && j + lane < n) {
+      L[(long long)j * ld + lane] = l;
+      Lr[(long long)(j + lane) * ld + (bw - lane)] = l;
+    }
+#pragma unroll
+    for (int c = 1; c <= BW; c++) w[c] -= l * lane_d(l, c);
+#pragma unroll
+    for (int c = 0; c < BW; c++) w[c] = __shfl_down(w[c + 1], 1, 64);
+    w[BW] = 0.0;
+#pragma unroll
+    for (int c = 0; c <= BW; c++) {
+      const double v = lane_d(nrow, c);
+      if (lane == bw) w[c] = v;
+    }
+  }
+  if (lane == 0 && fail) bad[0] = 1;
+}
+
+// t <- (L L^T)^-1 t in band order, by the calling wavefront (lanes 0..63):
+// forward sweep with the columns of L, backward sweep with its rows.
+__device__ void band_sweeps(int n, int bw, const double* __restrict__ L, const double* __restrict__ Lr,
+                            double* __restrict__ t) {
+  const int lane = threadIdx.x & 63;
+  const int ld = bw + 1;
+  const bool act = lane <= bw;
+  // forward: lane i holds the pending right-hand side of row j + i
+  double rw = (act && lane < n) ? t[lane] : 0.0;
+  double lc = (act && lane < n) ? L[lane] : 0.0;
+  for (int j = 0; j < n; j++) {
+    const double lnext = (act && j + 1 + lane < n) ? L[(long long)(j + 1) * ld + lane] : 0.0;
+    const double inr = (j + 1 + bw < n) ? t[j + 1 + bw] : 0.0;
+    const double yj = lane_d(rw, 0) / lane_d(lc, 0);
+    if (lane == 0) t[j] = yj;
+    rw -= lc * yj;
+    rw = __shfl_down(rw, 1, 64);
+    if (lane == bw) rw = inr;
+    lc = lnext;
+  }
+  // backward: lane k holds the pending value of row i - k
+  double yw = (act && n - 1 - lane >= 0) ? t[n - 1 - lane] : 0.0;
+  double lr = (act && n - 1 - lane >= 0) ? Lr[(long long)(n - 1) * ld + (bw - lane)] : 0.0;
+  for (int i = n - 1; i >= 0; i--) {
+    const double lnext = (act && i - 1 - lane >= 0) ? Lr[(long long)(i - 1) * ld + (bw - lane)] : 0.0;
+    const double inr = (i - 1 - bw >= 0) ? t[i - 1 - bw] : 0.0;
+    const double xi = lane_d(yw, 0) / lane_d(lr, 0);
+    if (lane == 0) t[i] = xi;
+    yw -= lr * xi;
+    yw = __shfl_down(yw, 1, 64);
+    if (lane == bw) yw = inr;
+    lr = lnext;
+  }
+}
+
+// The whole PCG solve of a band-preconditioned system in one workgroup:
+// the product and the vector updates over all threads, the preconditioner
+// by wave 0 between barriers (t: a band-ordered work vector).  With the
+// exact factor the first iteration meets the tolerance; the test comes
+// before the next application, so a converged solve applies it once.
+__global__ __launch_bounds__(kSmallPcgThreads) void k_pcg_band(
+    int n, const int* __restrict__ rowptr, const int* __restrict__ colind, const double* __restrict__ vals,
+    const double* __restrict__ b, double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+    double* __restrict__ p, double* __restrict__ q, double* __restrict__ t, int bw, const int* __restrict__ pinv,
+    const double* __restrict__ L, const double* __restrict__ Lr, double tol2, int maxit, CGScal* __restrict__ S) {
+  __shared__ double sh[kSmallPcgThreads / 64];
+  const int tid = threadIdx.x;
+  constexpr int NT = kSmallPcgThreads;
+  auto precond = [&]() {
+    for (int i = tid; i < n; i += NT) t[pinv[i]] = r[i];
+    __syncthreads();
+    if (tid < 64) band_sweeps(n, bw, L, Lr, t);
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) z[i] = t[pinv[i]];
+    __syncthreads();
+  };
+  for (int i = tid; i < n; i += NT) {
+    x[i] = 0.0;
+    r[i] = b[i];
+  }
+  __syncthreads();
+  precond();
+  double l1 = 0.0, l2 = 0.0;
+  for (int i = tid; i < n; i += NT) {
+    p[i] = z[i];
+    l1 += r[i] * z[i];
+    l2 += r[i] * r[i];
+  }
+  double rz = block_sum_1024(l1, sh);
+  const double bb = block_sum_1024(l2, sh);
+  double rr = bb, pq = 0.0, alpha = 0.0, beta = 0.0;
+  int it = 0, brk = 0;
+  bool done = bb == 0.0;
+  while (!done) {
+    double lpq = 0.0;
+    for (int i = tid; i < n; i += NT) {
+      const int rb = rowptr[i], re = rowptr[i + 1];
+      double s = 0.0;
+      for (int k0 = rb; k0 < re; k0 += 16) {
+        int c[16];
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+          const int k = min(k0 + u, re - 1);
+          c[u] = colind[k];
+          v[u] = vals[k];
+        }
+        double pc[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) pc[u] = p[c[u]];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+          if (k0 + u < re) s += v[u] * pc[u];
+      }
+      q[i] = s;
+      lpq += p[i] * s;
+    }
+    pq = block_sum_1024(lpq, sh);
+    if (pq == 0.0 || !isfinite(pq)) {
+      brk = 1;
+      break;
+    }
+    alpha = rz / pq;
+    l2 = 0.0;
+    for (int i = tid; i < n; i += NT) {
+      x[i] += alpha * p[i];
+      const double ri = r[i] - alpha * q[i];
+      r[i] = ri;
+      l2 += ri * ri;
+    }
+    rr = block_sum_1024(l2, sh);
+    it++;
+    if (rr <= tol2 * bb || it >= maxit) break;
+    precond();
+    l1 = 0.0;
+    for (int i = tid; i < n; i += NT) l1 += r[i] * z[i];
+    const double rzn = block_sum_1024(l1, sh);
+    beta = rzn / rz;
+    rz = rzn;
+    for (int i = tid; i < n; i += NT) p[i] = z[i] + beta * p[i];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    S->rzp[0] = S->rzp[1] = rz;
+    S->pq = pq;
+    S->rr = rr;
+    S->bb = bb;
+    S->alpha = alpha;
+    S->beta = beta;
+    S->tol2 = tol2;
+    S->maxit = maxit;
+    S->iterp[0] = S->iterp[1] = it;
+    S->fresh = 0;
+    S->breakdown = brk;
+    S->converged = 1;
+  }
+}
+
 // Jacobi / nodal block-Jacobi inverse from the assembled CSR matrix.
 // One thread per node block: gathers the m x m diagonal block (m <= M, the
 // nodal DOF count), inverts it by Gauss-Jordan with partial pivoting and
@@ -1200,6 +1427,28 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
       hipLaunchKernelGGL(k_cg_newp, dim3(nbv), dim3(256), 0, s, n, S, z, p);
     }
   }
+}
+
+void band_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* colind, const double* vals,
+                 const int* pinv, double* L, double* Lr, int* bad) {
+  if (n <= 0) return;
+  if (bw < 0 || bw > 63) throw Error(XG_ERR_ARG, "band preconditioner: half bandwidth must be 0..63");
+  const long long sz = (long long)n * (bw + 1);
+  const int fg = (int)std::min<long long>((sz + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_band_fill, dim3(fg), dim3(256), 0, s, sz, L);
+  hipLaunchKernelGGL(k_band_fill, dim3(fg), dim3(256), 0, s, sz, Lr);
+  hipLaunchKernelGGL(k_band_gather, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, bw, rowptr, colind, vals, pinv, L);
+  if (bw <= 15) hipLaunchKernelGGL(k_band_factor<15>, dim3(1), dim3(64), 0, s, n, bw, L, Lr, bad);
+  else if (bw <= 31) hipLaunchKernelGGL(k_band_factor<31>, dim3(1), dim3(64), 0, s, n, bw, L, Lr, bad);
+  else if (bw <= 47) hipLaunchKernelGGL(k_band_factor<47>, dim3(1), dim3(64), 0, s, n, bw, L, Lr, bad);
+  else hipLaunchKernelGGL(k_band_factor<63>, dim3(1), dim3(64), 0, s, n, bw, L, Lr, bad);
+}
+
+void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
+                   double* x, double* r, double* z, double* p, double* q, double* t, int bw, const int* pinv,
+                   const double* L, const double* Lr, double tol2, int maxit, void* scal) {
+  hipLaunchKernelGGL(k_pcg_band, dim3(1), dim3(kSmallPcgThreads), 0, s, n, rowptr, colind, vals, b, x, r, z, p, q, t,
+                     bw, pinv, L, Lr, tol2, maxit, (CGScal*)scal);
 }
 
 bool cg_small_fits(int n, long long nnz) { return n > 0 && n <= 1024 && nnz <= 32768; }

@@ -188,13 +188,28 @@ def build_rc_frame_2d(model, fr, ele0=1):
     model.elements("dispBeamColumn", np.arange(ele0 + nc, ele0 + nc + nb), fr["beams"], 1, 1, 5)
 
 
-def define_transient_rc_frame_2d(model, fr, ag, dt, mass=60.0, alphaM=0.35, tol=1e-8, maxit=25):
+GRAVITY = 9810.0  # mm / s^2
+
+
+def gravity_masses(fr, floor_load=100.0, bay=6000.0):
+    """Lumped floor masses from the gravity load on the beams: each floor
+    node carries floor_load (N/mm) over half of each adjacent bay, divided
+    by g (N, mm, s: masses in tonnes)."""
+    x = dict(zip(fr["node_tags"].tolist(), fr["crds"][:, 0].tolist()))
+    xs = np.array([x[int(t)] for t in fr["floor_nodes"]])
+    edge = (xs == fr["crds"][:, 0].min()) | (xs == fr["crds"][:, 0].max())
+    trib = np.where(edge, 0.5 * bay, bay)
+    return floor_load * trib / GRAVITY
+
+
+def define_transient_rc_frame_2d(model, fr, ag, dt, mass=None, alphaM=0.35, tol=1e-8, maxit=25):
     """Plane RC frame under a UniformExcitation record in the x direction:
-    lumped floor masses (x and y), mass-proportional damping, Newmark
-    average acceleration, NormDispIncr test."""
+    lumped floor masses (x and y; from the gravity load on the beams unless
+    a uniform mass is given), mass-proportional damping, Newmark average
+    acceleration, NormDispIncr test."""
     build_rc_frame_2d(model, fr)
     m = np.zeros((len(fr["floor_nodes"]), 3))
-    m[:, 0] = m[:, 1] = mass
+    m[:, 0] = m[:, 1] = gravity_masses(fr) if mass is None else mass
     model.masses(fr["floor_nodes"], m)
     model.uniformExcitation(1, ag, dt)
     model.rayleigh(alphaM)
