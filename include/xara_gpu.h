@@ -245,6 +245,16 @@ int  xg_form_unbalance(xg_domain* d, double* B);
 /* LinearSOE::solve (LinearSOE.h:46-89) of the assembled system with right-hand
    side B (neq) into X (neq).                                                 */
 int  xg_solve(xg_domain* d, const double* B, double* X);
+/* xg_solve without allocating: no counterpart in the reference; for callers
+   that cannot allocate device memory (code around HIP graph capture, tight
+   serving loops, harnesses that forbid hipMalloc).  Same result as
+   xg_solve; every PCG work buffer (staging of B and X, the six iteration
+   vectors, the block-Jacobi inverses or the band factor, partial sums,
+   scalars) comes from a fixed 128 MiB device array per GPU, looked up at
+   xg_create, and the kernels are launched one by one (no graph).  A system whose buffers do
+   not fit (about 1.1 M equations with nodal 3x3 blocks) is XG_ERR_ARG.
+   Calls that overlap on one GPU share that array and race.                 */
+int  xg_solve_noalloc(xg_domain* d, const double* B, double* X);
 /* y = A x with the assembled CSR values (for tests)                          */
 int  xg_spmv(xg_domain* d, const double* x, double* y);
 /* Domain::commit (Domain.cpp:1893-1923) / revertToLastCommit / revertToStart */

@@ -57,6 +57,7 @@ SIGNATURES = {
     "rayleigh": (C.c_int, [VP, C.c_double, C.c_double, C.c_double, C.c_double]),
     "system": (C.c_int, [VP, C.c_int, C.c_double, C.c_int]),
     "add_a": (C.c_int, [VP, C.c_int, C.c_int, P_D, P_I]),
+    "solve_noalloc": (C.c_int, [VP, P_D, P_D]),
     "numberer": (C.c_int, [VP, C.c_int]),
     "spmv_config": (C.c_int, [VP, C.c_int, C.c_int]),
     "cg_config": (C.c_int, [VP, C.c_int, C.c_int]),
@@ -112,7 +113,7 @@ SIGNATURES = {
     "material_vjp": (C.c_int, [C.c_int, P_D, P_D, C.c_int, C.c_double, P_D, P_D, P_D, P_D]),
 }
 GPU_ONLY = {k for k in SIGNATURES if k.startswith(("ensemble_", "adjoint_"))} | {
-    "device_name", "material_vjp", "spmv_config", "cg_config"}
+    "device_name", "material_vjp", "spmv_config", "cg_config", "solve_noalloc"}
 
 
 class Binding:

@@ -78,6 +78,7 @@ int xg_create(int ndm, int ndf, int device, xg_domain** out) {
     XG_HIP(hipGetDeviceCount(&count));
     if (device < 0 || device >= count) throw xg::Error(XG_ERR_DEVICE, "no such GPU");
     XG_HIP(hipSetDevice(device));
+    xg::noalloc_init(device);  // this device's address of the no-allocation scratch array
     xg_domain* h = new xg_domain();
     h->d.ndm = ndm;
     h->d.ndf = ndf;
@@ -556,6 +557,10 @@ int xg_form_unbalance(xg_domain* h, double* B) {
     else XG_HIP(hipStreamSynchronize(d.stream));
     return 0;
   });
+}
+
+int xg_solve_noalloc(xg_domain* h, const double* B, double* X) {
+  return guard([&] { return xg::solve_noalloc(R(h), B, X); });
 }
 
 int xg_add_a(xg_domain* h, int nblk, int bs, const double* m, const int* id) {

@@ -241,4 +241,10 @@ struct Domain {
   void spmv(const double* x, double* y, bool solver = false);
 };
 
+// no-allocation solve (xg_noalloc.cpp): per-device scratch lookup at
+// xg_create, and the solve with its work buffers in the scratch array
+void noalloc_init(int device);
+size_t noalloc_scratch_bytes();
+int solve_noalloc(Domain& d, const double* B, double* X);
+
 }  // namespace xg
