@@ -387,6 +387,9 @@ def quad_main(args):
         wall = time.perf_counter() - t0
         c = m.counters()
         res[name] = (wall, c["newton_iters"] - c0["newton_iters"], np.array(traj))
+        if name == "gpu":
+            solver = {"solves": c["solves"] - c0["solves"], "pcg_iters": c["cg_iters"] - c0["cg_iters"],
+                      "band_bw": c["band_bw"]}
         print(f"bench: quad {name} {wall:.2f}s newton={res[name][1]}", file=sys.stderr, flush=True)
     (gw, gi, gt), (cw, ci, ct) = res["gpu"], res["cpu"]
     rel = float(np.max(np.abs(gt - ct)) / np.max(np.abs(ct))) if len(gt) == len(ct) else None
@@ -395,7 +398,7 @@ def quad_main(args):
         "steps": nsteps, "warmup": 0, "ms_per_step": 1e3 * gw / nsteps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "FourNodeQuad J2 plane-strain strip 64x16, DisplacementControl pushover to 0.05",
-                   "newton_iters": gi, "system": args.system, "max_rel_diff_vs_oracle": rel},
+                   "newton_iters": gi, "system": args.system, **solver, "max_rel_diff_vs_oracle": rel},
         "cpu_baseline": {"value": ci / cw, "unit": "iterations/s", "cores": 1, "kind": "port",
                          "sample": "the whole pushover, oracle direct solve"},
     }

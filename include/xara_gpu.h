@@ -75,6 +75,8 @@ typedef struct xg_counters {
   double    time_tangent_ms;
   double    time_unbalance_ms;
   double    time_solve_ms;
+  long long band_bw;          /* half bandwidth of the band-Cholesky
+                                 preconditioner, -1 when it is not used    */
 } xg_counters;
 
 /* ---- lifetime -------------------------------------------------------------- */

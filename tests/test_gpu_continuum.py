@@ -262,18 +262,20 @@ def test_pcg_graph_replay_matches_eager_launches():
 def test_single_workgroup_pcg_matches_launch_per_step(system):
     """The whole-solve kernels of small systems (one workgroup, one launch)
     against the kernel-per-step PCG: the same converged trajectories over a
-    2D frame pushover (3 DOFs per node, half bandwidth 11) and a quad strip
-    (2, 19), both taken by the band-Cholesky-preconditioned solve, which
-    then needs at most three iterations per solve; and over a 3x3x3 3D
-    frame (half bandwidth 77 > 63, taken by the (block-)Jacobi
-    one-workgroup solve), whose iteration counts agree within rounding's
-    reach."""
+    2D frame pushover (3 DOFs per node, half bandwidth 11) and quad strips
+    of half bandwidths 13, 43 and 59 (the band factor's column-frame kernel
+    up to 47, its shifting-window kernel above), all
+    taken by the band-Cholesky-preconditioned solve, which then needs at
+    most three iterations per solve; and over a 3x3x3 3D frame (half
+    bandwidth 77 > 63, taken by the (block-)Jacobi one-workgroup solve),
+    whose iteration counts agree within rounding's reach."""
     fr = meshes.rc_frame_2d(3, 2)
     conn, _ = meshes.frame_elements(fr)
     theta = meshes.default_theta(len(conn))
-    mesh = meshes.quad_strip(16, 4)
+    strips = {"quad": (meshes.quad_strip(16, 4), 13), "quad19": (meshes.quad_strip(40, 19, ly=10.0), 43),
+              "quad27": (meshes.quad_strip(40, 27, ly=14.0), 59)}
     fr3 = meshes.rc_frame_3d(3, 3, 3)
-    for case in ("frame", "quad", "frame3d"):
+    for case in ("frame", "quad", "quad19", "quad27", "frame3d"):
         print(f"case: small pcg {system} {case}", flush=True)
         res = []
         for small in (True, False):
@@ -284,7 +286,8 @@ def test_single_workgroup_pcg_matches_launch_per_step(system):
                 P[:, 0] = 1.0
                 m.loads(fr["floor_nodes"], P)
                 node, dof, du, tol = fr["roof"], 1, 6.0, 1e-5
-            elif case == "quad":
+            elif case in strips:
+                mesh = strips[case][0]
                 m = Model(2, 2)
                 meshes.build_quad_strip(m, mesh)
                 m.load(mesh["tip"], 0.0, 1.0)
@@ -303,6 +306,8 @@ def test_single_workgroup_pcg_matches_launch_per_step(system):
                 assert m.analyze(1) == 0
                 t.append(m.trial_disp().copy())
             c = m.counters()
+            if case in strips and small:
+                assert c["band_bw"] == strips[case][1], c["band_bw"]
             res.append((np.array(t), c["cg_iters"], c["solves"]))
             m.close()
         (a, ia, sa), (b, ib, sb) = res

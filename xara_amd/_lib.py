@@ -29,6 +29,7 @@ class Counters(C.Structure):
         ("time_tangent_ms", C.c_double),
         ("time_unbalance_ms", C.c_double),
         ("time_solve_ms", C.c_double),
+        ("band_bw", C.c_longlong),
     ]
 
 

@@ -711,7 +711,9 @@ int xg_element_resisting_force(xg_domain* h, int ele_tag, double* R, int* ndof) 
 
 int xg_get_counters(xg_domain* h, xg_counters* c) {
   return guard([&] {
-    *c = D(h).counters;
+    auto& d = D(h);
+    *c = d.counters;
+    c->band_bw = d.band_active() ? d.band_bw : -1;
     return 0;
   });
 }

@@ -687,7 +687,7 @@ void Domain::form_tangent() {
 }
 
 Domain::CgBufs Domain::cg_bufs() {
-  return CgBufs{r0.p, r1.p, z.p, p.p, p2.p, q.p, part.p, binv.p, pcbad.p, scal.p, band_L.p, band_Lr.p};
+  return CgBufs{r0.p, r1.p, z.p, p.p, p2.p, q.p, part.p, binv.p, pcbad.p, scal.p, band_A.p, band_L.p, band_Lr.p};
 }
 
 // The preconditioner of the current matrix into w: the band-Cholesky factor
@@ -699,7 +699,8 @@ void Domain::precond_setup(const CgBufs& w) {
   if (pc_done == key) return;
   XG_HIP(hipMemsetAsync(w.pcbad, 0, sizeof(int), stream));
   if (band) {
-    band_factor(stream, num.neq, band_bw, rowptr.p, colind.p, vals.p, band_pinv.p, w.bandL, w.bandLr, w.pcbad);
+    band_factor(stream, num.neq, band_bw, rowptr.p, colind.p, vals.p, band_pinv.p, w.bandA, w.bandL, w.bandLr,
+                w.pcbad);
   } else {
     const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
     launch_block_inverse(stream, nblk, blk_first.p, blk_size.p, rowptr.p, colind.p, vals.p, bw, w.binv, w.pcbad);
@@ -720,6 +721,7 @@ static int band_width(const Numbering& num, const std::vector<int>& pinv) {
 void Domain::band_order() {
   band_bw = -1;
   band_pinv.release();
+  band_A.release();
   band_L.release();
   band_Lr.release();
   const int n = num.neq;
@@ -764,6 +766,7 @@ void Domain::band_order() {
   if (best > 63) return;
   band_bw = best;
   band_pinv.upload(pinv, stream);
+  band_A.resize((size_t)n * (best + 1));
   band_L.resize((size_t)n * (best + 1));
   band_Lr.resize((size_t)n * (best + 1));
 }

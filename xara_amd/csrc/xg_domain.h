@@ -189,7 +189,7 @@ struct Domain {
     double *r0, *r1, *z, *p, *p2, *q, *part, *binv;
     int* pcbad;
     void* scal;
-    double *bandL, *bandLr;  // band-Cholesky factor (band_active())
+    double *bandA, *bandL, *bandLr;  // band matrix and its Cholesky factor (band_active())
   };
   // band-Cholesky preconditioner of small systems (taken by the one-
   // workgroup solve, cg_small): the half bandwidth in the band order
@@ -199,7 +199,7 @@ struct Domain {
   static constexpr int kBandMaxEqn = 16384;
   int band_bw = -1;
   DVec<int> band_pinv;
-  DVec<double> band_L, band_Lr;
+  DVec<double> band_A, band_L, band_Lr;
   bool band_active() const { return cg_small_on && band_bw >= 0; }
   void band_order();
   // the preconditioner's buffers last formed from the current matrix

@@ -94,9 +94,10 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
 // cannot launch (called before a graph capture of cg_iterations)
 void cg_iterations_check(int bw, const SymSpmv* sym);
 // band-Cholesky preconditioner (half bandwidth bw <= 63 in the band order
-// pinv): factor L (column rows) and Lr (rows), n*(bw+1) doubles each
+// pinv): the band matrix A, its factor L by columns and Lr by rows, n*(bw+1)
+// doubles each
 void band_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* colind, const double* vals,
-                 const int* pinv, double* L, double* Lr, int* bad);
+                 const int* pinv, double* A, double* L, double* Lr, int* bad);
 // whole band-preconditioned PCG solve in one workgroup; t: n doubles of work
 void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
                    double* x, double* r, double* z, double* p, double* q, double* t, int bw, const int* pinv,

@@ -60,7 +60,7 @@ size_t noalloc_scratch_bytes() { return kScratchBytes; }
 static size_t needed(const Domain& d, int bw) {
   const size_t n = (size_t)d.num.neq;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-  const size_t band = d.band_active() ? 2 * al(n * (d.band_bw + 1) * sizeof(double)) : 0;
+  const size_t band = d.band_active() ? 3 * al(n * (d.band_bw + 1) * sizeof(double)) : 0;
   return 8 * al(n * sizeof(double)) + al(n * bw * sizeof(double)) + al(cg_part_size(d.num.neq) * sizeof(double)) +
          al(cg_scal_bytes()) + al(sizeof(int)) + band;
 }
@@ -100,8 +100,9 @@ int solve_noalloc(Domain& d, const double* B, double* X) {
   w.part = c.take<double>(cg_part_size(n));
   w.scal = c.take<char>(cg_scal_bytes());
   w.pcbad = c.take<int>(1);
-  w.bandL = w.bandLr = nullptr;
+  w.bandA = w.bandL = w.bandLr = nullptr;
   if (d.band_active()) {
+    w.bandA = c.take<double>((size_t)n * (d.band_bw + 1));
     w.bandL = c.take<double>((size_t)n * (d.band_bw + 1));
     w.bandLr = c.take<double>((size_t)n * (d.band_bw + 1));
   }

@@ -838,16 +838,25 @@ __global__ __launch_bounds__(kSmallPcgThreads) void k_pcg_small(
 // Band-Cholesky preconditioner of small, narrow-banded systems.  In the band
 // order k = pinv[e] of the equations (the numbering itself, or a coordinate
 // ordering when that is narrower) the matrix has half bandwidth bw <= 63, so
-// one 64-lane wavefront holds a whole pivot window: lane i works on row
-// j + i.  Layouts (ld = bw + 1):
-//   L  before the factorization, row k holds A(k, k-bw .. k) at offsets
-//      0 .. bw (the diagonal at bw); after it, row j holds column j of the
-//      factor, L(j + i, j) at offset i;
-//   Lr the factor by rows: L(k, k - i) at offset bw - i.
+// one 64-lane wavefront holds a whole pivot window.  Layouts (ld = bw + 1):
+//   A  the matrix: row k holds A(k, k-bw .. k) at offsets 0 .. bw (the
+//      diagonal at bw);
+//   L  the factor by columns: row j holds U(j + i, j) at offset i;
+//   Lr the factor by rows: row k holds U(k, k - i) at offset i;
+// where A = U D^2 U^T (U = L D^-1 unit lower, D = diag L); in both, offset
+// 0 holds 1 / L(j, j)^2.  The sweeps then divide nowhere on their chain.
 // The factorization runs once per assembled matrix; every application of
 // the preconditioner is a forward and a backward sweep by wave 0 of the
-// solving workgroup.
+// solving workgroup, fed through LDS by the other waves.
 // ---------------------------------------------------------------------------
+// A workgroup barrier that orders LDS only: unlike __syncthreads it does not
+// wait for this wave's outstanding global stores, which no other wave reads.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ double lane_d(double v, int l) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, l);
@@ -862,13 +871,13 @@ __global__ __launch_bounds__(256) void k_band_fill(long long n, double* __restri
 
 __global__ __launch_bounds__(256) void k_band_gather(int n, int bw, const int* __restrict__ rowptr,
                                                      const int* __restrict__ colind, const double* __restrict__ vals,
-                                                     const int* __restrict__ pinv, double* __restrict__ L) {
+                                                     const int* __restrict__ pinv, double* __restrict__ A) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
   const int rk = pinv[r], ld = bw + 1;
   for (int q = rowptr[r]; q < rowptr[r + 1]; q++) {
     const int ck = pinv[colind[q]];
-    if (ck <= rk && rk - ck <= bw) L[(long long)rk * ld + (ck - rk + bw)] = vals[q];
+    if (ck <= rk && rk - ck <= bw) A[(long long)rk * ld + (ck - rk + bw)] = vals[q];
   }
 }
 
@@ -876,7 +885,8 @@ __global__ __launch_bounds__(256) void k_band_gather(int n, int bw, const int* _
 // row j + i of the reduced window in w[0 .. BW] (w[c] = A'(j + i, j + c)).
 // A non-positive pivot sets *bad (the matrix is not positive definite).
 template <int BW>
-__global__ __launch_bounds__(64) void k_band_factor(int n, int bw, double* __restrict__ L, double* __restrict__ Lr,
+__global__ __launch_bounds__(64) void k_band_factor(int n, int bw, const double* __restrict__ A,
+                                                    double* __restrict__ L, double* __restrict__ Lr,
                                                     int* __restrict__ bad) {
   const int lane = threadIdx.x;
   const int ld = bw + 1;
@@ -885,7 +895,7 @@ __global__ __launch_bounds__(64) void k_band_factor(int n, int bw, double* __res
 #pragma unroll
   for (int c = 0; c <= BW; c++) w[c] = 0.0;
   if (act && lane < n) {
-    const double* row = L + (long long)lane * ld + (bw - lane);
+    const double* row = A + (long long)lane * ld + (bw - lane);
 #pragma unroll
     for (int c = 0; c <= BW; c++)
       if (c <= lane) w[c] = row[c];
@@ -894,14 +904,16 @@ __global__ __launch_bounds__(64) void k_band_factor(int n, int bw, double* __res
   for (int j = 0; j < n; j++) {
     // the row entering the window after this pivot, loaded ahead
     const int r = j + 1 + bw;
-    const double nrow = (act && r < n) ? L[(long long)r * ld + lane] : 0.0;
+    const double nrow = (act && r < n) ? A[(long long)r * ld + lane] : 0.0;
     const double a00 = lane_d(w[0], 0);
     if (!(a00 > 0.0)) fail = 1;
     const double d = sqrt(a00 > 0.0 ? a00 : 1.0);
-    const double l = lane == 0 ? d : (act ? w[0] / d : 0.0);
+    const double rd = 1.0 / d;
+    const double l = lane == 0 ? d : (act ? w[0] * rd : 0.0);
     if (act && j + lane < n) {
-      L[(long long)j * ld + lane] = l;
-      Lr[(long long)(j + lane) * ld + (bw - lane)] = l;
+      const double o = lane == 0 ? rd * rd : l * rd;
+      L[(long long)j * ld + lane] = o;
+      Lr[(long long)(j + lane) * ld + lane] = o;
     }
 #pragma unroll
     for (int c = 1; c <= BW; c++) w[c] -= l * lane_d(l, c);
@@ -917,39 +929,316 @@ __global__ __launch_bounds__(64) void k_band_factor(int n, int bw, double* __res
   if (lane == 0 && fail) bad[0] = 1;
 }
 
-// t <- (L L^T)^-1 t in band order, by the calling wavefront (lanes 0..63):
-// forward sweep with the columns of L, backward sweep with its rows.
-__device__ void band_sweeps(int n, int bw, const double* __restrict__ L, const double* __restrict__ Lr,
-                            double* __restrict__ t) {
+// 1 / sqrt(a) for a > 0: the hardware estimate and two Newton steps
+// (y <- y (1.5 - a y^2 / 2)), accurate to rounding.
+__device__ __forceinline__ double rsqrt_nr(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  const double h = 0.5 * a;
+  y = y * fma(-h, y * y, 1.5);
+  y = y * fma(-h, y * y, 1.5);
+  return y;
+}
+
+// Band Cholesky over four wavefronts, for bw <= 47.  Lane l holds the window
+// row congruent to l mod 64, so no value moves between lanes; each row's
+// entries sit in registers indexed by column, in a frame of 128 columns
+// (j0 - 48 .. j0 + 79 for the chunk of 16 pivots starting at j0) that moves
+// 16 columns on after each chunk, so every register index is a
+// compile-time constant.  The frame is dealt out in panels of 4 columns:
+// wave v holds frame columns rho with (rho / 4) mod 4 = v, does every update
+// of them in pivot order, and factors its panels itself -- four pivots in a
+// row within one wavefront, the multipliers broadcast by readlane.  It then
+// puts the panel's four multiplier vectors in LDS (rotated so that distance
+// m from each pivot row is element m); after one barrier every wave applies
+// the rank-4 update to its own columns, the owner of the next panel first
+// to that panel's columns, which it then factors.  Entries above a row's
+// diagonal receive updates that are never read.  After each chunk the 16
+// finished lanes take the rows 64 further on, staged in LDS two chunks
+// ahead (each thread loads its share at the start of a chunk and stores it
+// to LDS at the end).  Outputs as k_band_factor: U by columns (L) and by
+// rows (Lr), 1 / L(j, j)^2 at offset 0.
+template <int BW, int WV>
+__device__ __forceinline__ void band_factor_wave(int n, int bw, const double* __restrict__ A,
+                                                 double* __restrict__ L, double* __restrict__ Lr,
+                                                 int* __restrict__ bad, double (*__restrict__ sA)[16 * 64],
+                                                 double (*__restrict__ shl)[4][64], int nch) {
+  constexpr int NS = 32, OFF = 48;
+  static_assert(BW <= 44, "the column frame holds half bandwidths up to 44");
+  // frame column rho lives in w[loc(rho)] of wave own(rho)
+  auto own = [](int rho) { return (rho >> 2) & 3; };
+  auto loc = [](int rho) { return ((rho >> 4) << 2) | (rho & 3); };
   const int lane = threadIdx.x & 63;
   const int ld = bw + 1;
-  const bool act = lane <= bw;
-  // forward: lane i holds the pending right-hand side of row j + i
-  double rw = (act && lane < n) ? t[lane] : 0.0;
-  double lc = (act && lane < n) ? L[lane] : 0.0;
-  for (int j = 0; j < n; j++) {
-    const double lnext = (act && j + 1 + lane < n) ? L[(long long)(j + 1) * ld + lane] : 0.0;
-    const double inr = (j + 1 + bw < n) ? t[j + 1 + bw] : 0.0;
-    const double yj = lane_d(rw, 0) / lane_d(lc, 0);
-    if (lane == 0) t[j] = yj;
-    rw -= lc * yj;
-    rw = __shfl_down(rw, 1, 64);
-    if (lane == bw) rw = inr;
-    lc = lnext;
+  double w[NS];
+  int row = lane, fail = 0;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int rho = 16 * (s >> 2) + 4 * WV + (s & 3);
+    const int col = rho - OFF, e = col - lane + bw;
+    w[s] = (col >= 0 && e >= 0 && e <= bw && lane < n) ? A[(long long)lane * ld + e] : 0.0;
   }
-  // backward: lane k holds the pending value of row i - k
-  double yw = (act && n - 1 - lane >= 0) ? t[n - 1 - lane] : 0.0;
-  double lr = (act && n - 1 - lane >= 0) ? Lr[(long long)(n - 1) * ld + (bw - lane)] : 0.0;
-  for (int i = n - 1; i >= 0; i--) {
-    const double lnext = (act && i - 1 - lane >= 0) ? Lr[(long long)(i - 1) * ld + (bw - lane)] : 0.0;
-    const double inr = (i - 1 - bw >= 0) ? t[i - 1 - bw] : 0.0;
-    const double xi = lane_d(yw, 0) / lane_d(lr, 0);
-    if (lane == 0) t[i] = xi;
-    yw -= lr * xi;
-    yw = __shfl_down(yw, 1, 64);
-    if (lane == bw) yw = inr;
-    lr = lnext;
+  // factor the panel of pivots jp .. jp + 3 at frame columns rho0 .. rho0 + 3
+  // (this wave's), its multipliers into buf
+  auto panel = [&](int jp, int rho0, double (*buf)[64]) {
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int j = jp + b;
+      if (j < n) {
+        const int p = j & 63;
+        const double a = w[loc(rho0 + b)];
+        const double a00 = lane_d(a, p);
+        if (!(a00 > 0.0)) fail = 1;
+        const double rd = rsqrt_nr(a00 > 0.0 ? a00 : 1.0);
+        const int i = row - j;
+        const bool inwin = (unsigned)i <= (unsigned)bw && row < n;
+        const double l = inwin ? a * rd : 0.0;
+        buf[b][(lane - p) & 63] = l;
+#pragma unroll
+        for (int d = 1; b + d < 4; d++) w[loc(rho0 + b + d)] -= l * lane_d(l, (p + d) & 63);
+        if (inwin) {
+          const double o = i == 0 ? rd * rd : l * rd;
+          L[(long long)j * ld + i] = o;
+          Lr[(long long)row * ld + i] = o;
+        }
+      } else {
+        buf[b][lane] = 0.0;
+      }
+    }
+  };
+  // rank-4 update of this wave's columns in [lo, hi] by the panel at frame
+  // columns rho0 .. rho0 + 3 (multipliers B, the lanes' own lr)
+  auto apply = [&](int rho0, int lo, int hi, const double (*B)[64], const double* lr) {
+#pragma unroll
+    for (int rho = lo; rho <= hi; rho++) {
+      if (own(rho) == WV) {
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int m = rho - (rho0 + b);
+          if (m >= 1 && m <= BW + 3) w[loc(rho)] -= lr[b] * B[b][m];
+        }
+      }
+    }
+  };
+  // the rows chunk c's finished lanes take, 16c + 64 .. 16c + 79: this
+  // thread's four of the 16 x 64 staged values
+  const int tid = threadIdx.x;
+  double pre[4];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int e = tid + 256 * k, t = e >> 6, q = e & 63, R = 16 * c + 64 + t;
+      pre[k] = (c < nch && R < n && q <= bw) ? A[(long long)R * ld + q] : 0.0;
+    }
+  };
+  auto store = [&](int c) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) sA[c % 3][tid + 256 * k] = pre[k];
+  };
+  fetch(0);
+  store(0);
+  fetch(1);
+  store(1);
+  if (WV == 0) panel(0, OFF, shl[0]);
+  lds_barrier();
+  for (int c = 0; c < nch; c++) {
+    const int j0 = 16 * c;
+    fetch(c + 2);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int jp = j0 + 4 * q, rho0 = OFF + 4 * q;
+      const double(*B)[64] = shl[q & 1];
+      double lr[4];
+#pragma unroll
+      for (int b = 0; b < 4; b++) lr[b] = B[b][(lane - ((jp + b) & 63)) & 63];
+      const int hi = rho0 + 3 + BW;
+      if (q < 3) {
+        if (own(rho0 + 4) == WV) {
+          apply(rho0, rho0 + 4, rho0 + 7, B, lr);
+          if (jp + 4 < n) panel(jp + 4, rho0 + 4, shl[(q + 1) & 1]);
+          apply(rho0, rho0 + 8, hi, B, lr);
+        } else {
+          apply(rho0, rho0 + 4, hi, B, lr);
+        }
+      } else {
+        apply(rho0, rho0 + 4, hi, B, lr);
+        store(c + 2);
+        // the lanes finished in this chunk take their next rows; the frame
+        // moves 16 columns (4 of this wave's slots) on
+        const bool sw = row >= j0 && row < j0 + 16;
+        if (sw) row += 64;
+        const int t = row - (j0 + 64);
+        const double* S = sA[c % 3];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          const int rho = 16 * (s >> 2) + 4 * WV + (s & 3);
+          if (rho >= 112 - BW) {
+            const int e = rho - 112 - t + bw;
+            const bool ok = sw && e >= 0 && e <= bw;
+            const double v = S[ok ? t * 64 + e : 0];
+            if (ok) w[s] = v;
+          }
+        }
+#pragma unroll
+        for (int s = 0; s < NS - 4; s++) w[s] = w[s + 4];
+#pragma unroll
+        for (int s = NS - 4; s < NS; s++) w[s] = 0.0;
+        if (WV == 0 && j0 + 16 < n) panel(j0 + 16, OFF, shl[0]);
+      }
+      lds_barrier();
+    }
   }
+  if (lane == 0 && fail) bad[0] = 1;
+}
+
+template <int BW>
+__global__ __launch_bounds__(256) void k_band_factor_w(int n, int bw, const double* __restrict__ A,
+                                                      double* __restrict__ L, double* __restrict__ Lr,
+                                                      int* __restrict__ bad) {
+  __shared__ double sA[3][16 * 64];
+  __shared__ double shl[2][4][64];
+  const int nch = (n + 15) >> 4;
+  switch (threadIdx.x >> 6) {
+    case 0: band_factor_wave<BW, 0>(n, bw, A, L, Lr, bad, sA, shl, nch); break;
+    case 1: band_factor_wave<BW, 1>(n, bw, A, L, Lr, bad, sA, shl, nch); break;
+    case 2: band_factor_wave<BW, 2>(n, bw, A, L, Lr, bad, sA, shl, nch); break;
+    default: band_factor_wave<BW, 3>(n, bw, A, L, Lr, bad, sA, shl, nch); break;
+  }
+}
+
+// Sweeps of the band factor, (U D^2 U^T)^-1 t, in blocks of 64 rows.  Lane
+// l of wave 0 holds the pending value of the window row congruent to l mod
+// 64, so no value moves between lanes: at pivot j the pending value of lane
+// j mod 64 is final (U has a unit diagonal), it is broadcast by readlane and
+// every other lane subtracts its entry of U times it -- one readlane and one
+// FMA on the chain per row.  The finished lanes of each group of CH steps
+// (16 when bw <= 48: a finished lane's entries stay zero for the rest of its
+// group) write their value (scaled by D^-2 after the forward sweep) and take
+// the row 64 further on.  The entries of each block, the t values the
+// lanes take next and, forward, the block's D^-2 are staged in LDS by waves
+// 1..15 two blocks ahead, as a 64 x 64 tile already rotated so that at step
+// u lane l reads element u * 64 + l (zero at distance 0 and past bw or n).
+constexpr int kBandTile = 64 * 64 + 128;  // doubles per staged block
+
+// Waves 1..15 stage block b of the factor M (L for the forward sweep, Lr for
+// the backward), the t values of rows tr0 .. tr0 + 63 and, forward, the
+// block's D^-2 into S.  Step u, lane l reads the entry at distance
+// (l - u) mod 64 (forward) or (u - l) mod 64 (backward) of row 64 b + u.
+__device__ __forceinline__ void band_stage(int n, int bw, const double* __restrict__ M,
+                                           const double* __restrict__ t, int b, int tr0, bool fwd,
+                                           double* __restrict__ S) {
+  constexpr int NL = kSmallPcgThreads - 64;
+  constexpr int K = (kBandTile + NL - 1) / NL;
+  const int ld = bw + 1;
+  const int e0 = threadIdx.x - 64;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int e = e0 + k * NL;
+    double x = 0.0;
+    if (e < 64 * 64) {
+      const int u = e >> 6, l = e & 63, row = 64 * b + u;
+      const int i = fwd ? ((l - u) & 63) : ((u - l) & 63);
+      if (i >= 1 && i <= bw && row < n) x = M[(long long)row * ld + i];
+    } else if (e < 64 * 64 + 64) {
+      const int row = tr0 + (e - 64 * 64);
+      if (row >= 0 && row < n) x = t[row];
+    } else if (e < kBandTile) {
+      const int row = 64 * b + (e - 64 * 64 - 64);
+      if (fwd && row < n) x = M[(long long)row * ld];
+    }
+    v[k] = x;
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    const int e = e0 + k * NL;
+    if (e < kBandTile) S[e] = v[k];
+  }
+}
+
+// One staged block of a sweep by wave 0: 64 steps (forward: u = 0 .. 63,
+// backward: u = 63 .. 0); rw the lanes' pending values; returns the block's
+// finished values (lane l: row 64 b + l), scaled by D^-2 when forward.
+template <bool FWD, int CH>
+__device__ __forceinline__ double band_block(const double* __restrict__ S, double& rw) {
+  const int lane = threadIdx.x & 63;
+  const double tn = S[64 * 64 + lane];
+  const double d2 = FWD ? S[64 * 64 + 64 + lane] : 1.0;
+  double out = 0.0;
+#pragma unroll 1
+  for (int c = 0; c < 4; c++) {
+    const int u0 = FWD ? 16 * c : 63 - 16 * c;
+    double lv[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) lv[k] = S[(FWD ? u0 + k : u0 - k) * 64 + lane];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int u = FWD ? u0 + k : u0 - k;
+      rw = fma(-lv[k], lane_d(rw, u), rw);
+      if (CH == 1 || k % CH == CH - 1) {
+        // the group's lanes are final: keep them, take the next rows
+        const int g0 = FWD ? u - (CH - 1) : u;
+        if (lane >= g0 && lane < g0 + CH) {
+          out = FWD ? rw * d2 : rw;
+          rw = tn;
+        }
+      }
+    }
+  }
+  return out;
+}
+
+// t <- (U D^2 U^T)^-1 t in band order.  Called by all kSmallPcgThreads threads
+// of the workgroup (it holds barriers); stage: 3 * kBandTile doubles of LDS.
+__device__ __attribute__((noinline)) void band_sweeps(int n, int bw, const double* __restrict__ L,
+                                                      const double* __restrict__ Lr, double* __restrict__ t,
+                                                      double* __restrict__ stage) {
+  const int tid = threadIdx.x, lane = tid & 63;
+  const bool w0 = tid < 64;
+  const int nb = (n + 63) >> 6;
+  // forward: L y = t; block b's staged t values are rows 64 (b + 1) + l
+  double rw = 0.0;
+  if (w0) {
+    rw = lane < n ? t[lane] : 0.0;
+  } else {
+    band_stage(n, bw, L, t, 0, 64, true, stage);
+    if (nb > 1) band_stage(n, bw, L, t, 1, 128, true, stage + kBandTile);
+  }
+  lds_barrier();
+  for (int b = 0; b < nb; b++) {
+    if (w0) {
+      double* S = stage + (b % 3) * kBandTile;
+      const double y = bw <= 48 ? band_block<true, 16>(S, rw) : band_block<true, 1>(S, rw);
+      if (64 * b + lane < n) t[64 * b + lane] = y;
+    } else if (b + 2 < nb) {
+      band_stage(n, bw, L, t, b + 2, 64 * (b + 3), true, stage + ((b + 2) % 3) * kBandTile);
+    }
+    lds_barrier();
+  }
+  // the backward staging reads the forward sweep's t from global memory
+  __syncthreads();
+  // backward: L^T x = y, blocks from the last; block b's staged t values are
+  // rows 64 (b - 1) + l
+  rw = 0.0;
+  if (w0) {
+    const int r = 64 * (nb - 1) + lane;
+    rw = r < n ? t[r] : 0.0;
+  } else {
+    band_stage(n, bw, Lr, t, nb - 1, 64 * (nb - 2), false, stage);
+    if (nb > 1) band_stage(n, bw, Lr, t, nb - 2, 64 * (nb - 3), false, stage + kBandTile);
+  }
+  __syncthreads();
+  for (int q = 0; q < nb; q++) {
+    const int b = nb - 1 - q;
+    if (w0) {
+      double* S = stage + (q % 3) * kBandTile;
+      const double x = bw <= 48 ? band_block<false, 16>(S, rw) : band_block<false, 1>(S, rw);
+      if (64 * b + lane < n) t[64 * b + lane] = x;
+    } else if (q + 2 < nb) {
+      band_stage(n, bw, Lr, t, b - 2, 64 * (b - 3), false, stage + ((q + 2) % 3) * kBandTile);
+    }
+    lds_barrier();
+  }
+  __syncthreads();
 }
 
 // The whole PCG solve of a band-preconditioned system in one workgroup:
@@ -963,13 +1252,13 @@ __global__ __launch_bounds__(kSmallPcgThreads) void k_pcg_band(
     double* __restrict__ p, double* __restrict__ q, double* __restrict__ t, int bw, const int* __restrict__ pinv,
     const double* __restrict__ L, const double* __restrict__ Lr, double tol2, int maxit, CGScal* __restrict__ S) {
   __shared__ double sh[kSmallPcgThreads / 64];
+  __shared__ double stage[3 * kBandTile];
   const int tid = threadIdx.x;
   constexpr int NT = kSmallPcgThreads;
   auto precond = [&]() {
     for (int i = tid; i < n; i += NT) t[pinv[i]] = r[i];
     __syncthreads();
-    if (tid < 64) band_sweeps(n, bw, L, Lr, t);
-    __syncthreads();
+    band_sweeps(n, bw, L, Lr, t, stage);
     for (int i = tid; i < n; i += NT) z[i] = t[pinv[i]];
     __syncthreads();
   };
@@ -1430,18 +1719,22 @@ void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const in
 }
 
 void band_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* colind, const double* vals,
-                 const int* pinv, double* L, double* Lr, int* bad) {
+                 const int* pinv, double* A, double* L, double* Lr, int* bad) {
   if (n <= 0) return;
   if (bw < 0 || bw > 63) throw Error(XG_ERR_ARG, "band preconditioner: half bandwidth must be 0..63");
   const long long sz = (long long)n * (bw + 1);
   const int fg = (int)std::min<long long>((sz + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_band_fill, dim3(fg), dim3(256), 0, s, sz, A);
   hipLaunchKernelGGL(k_band_fill, dim3(fg), dim3(256), 0, s, sz, L);
   hipLaunchKernelGGL(k_band_fill, dim3(fg), dim3(256), 0, s, sz, Lr);
-  hipLaunchKernelGGL(k_band_gather, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, bw, rowptr, colind, vals, pinv, L);
-  if (bw <= 15) hipLaunchKernelGGL(k_band_factor<15>, dim3(1), dim3(64), 0, s, n, bw, L, Lr, bad);
-  else if (bw <= 31) hipLaunchKernelGGL(k_band_factor<31>, dim3(1), dim3(64), 0, s, n, bw, L, Lr, bad);
-  else if (bw <= 47) hipLaunchKernelGGL(k_band_factor<47>, dim3(1), dim3(64), 0, s, n, bw, L, Lr, bad);
-  else hipLaunchKernelGGL(k_band_factor<63>, dim3(1), dim3(64), 0, s, n, bw, L, Lr, bad);
+  hipLaunchKernelGGL(k_band_gather, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, bw, rowptr, colind, vals, pinv, A);
+  const dim3 g(1), b(256);
+  if (bw <= 15) hipLaunchKernelGGL(k_band_factor_w<15>, g, b, 0, s, n, bw, A, L, Lr, bad);
+  else if (bw <= 23) hipLaunchKernelGGL(k_band_factor_w<23>, g, b, 0, s, n, bw, A, L, Lr, bad);
+  else if (bw <= 31) hipLaunchKernelGGL(k_band_factor_w<31>, g, b, 0, s, n, bw, A, L, Lr, bad);
+  else if (bw <= 39) hipLaunchKernelGGL(k_band_factor_w<39>, g, b, 0, s, n, bw, A, L, Lr, bad);
+  else if (bw <= 44) hipLaunchKernelGGL(k_band_factor_w<44>, g, b, 0, s, n, bw, A, L, Lr, bad);
+  else hipLaunchKernelGGL(k_band_factor<63>, dim3(1), dim3(64), 0, s, n, bw, A, L, Lr, bad);
 }
 
 void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
