@@ -85,6 +85,41 @@ def test_brick_assembly_parity():
     assert relerr(g.element_resisting_force(tag), o.element_resisting_force(tag)) <= RTOL
 
 
+def test_brick_tangent_into_csr_matches_element_matrices(monkeypatch):
+    """formTangent of a stdBrick-only model straight into the CSR values
+    (k_brick_tangent_csr: no element matrices, the J2 blocks in closed form)
+    against the element-matrix path (k_brick_tangent's BaT D Bb + the gather
+    assembly, XG_BRICK_TANGENT_CSR=0): the same values to rounding
+    (1e-13 of the largest entry) on a plastic state of a 7^3 cube, and again
+    after a commit and a further update."""
+    print("case: brick tangent into CSR, 7^3", flush=True)
+    n = 7
+    mesh = meshes.brick_cube(n)
+    U = random_field(mesh, 4e-3, 3)
+    out = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("XG_BRICK_TANGENT_CSR", env)
+        m = Model(3, 3)
+        meshes.build_brick_cube(m, mesh)
+        m.system("GpuBlockPCG", 1e-12, 1000)
+        m.setup()
+        m.set_trial_disp(U)
+        assert m.update() == 0
+        plastic = float((m.point_data("plastic", 8 * n ** 3)[:, 6] > 0).mean())
+        K1 = m.form_tangent()
+        m.commit()
+        m.set_trial_disp(1.5 * U)
+        assert m.update() == 0
+        K2 = m.form_tangent()
+        out.append((K1, K2, plastic))
+        m.close()
+    (a1, a2, pa), (b1, b2, pb) = out
+    assert pa == pb and pa > 0.2
+    assert relerr(a1, b1) <= 1e-13
+    assert relerr(a2, b2) <= 1e-13
+    assert relerr(a1, a2) > 1e-6
+
+
 @pytest.mark.parametrize("system,fmt", [("GpuPCG", "auto"), ("GpuBlockPCG", "auto"), ("GpuBlockPCG", "sym")])
 def test_pcg_solve_matches_direct(system, fmt):
     print(f"case: pcg {system} {fmt}", flush=True)

@@ -203,6 +203,59 @@ __global__ __launch_bounds__(128) void k_brick_resid(
   for (int i = 0; i < 24; i++) Re[e * 24 + i] = r[i];
 }
 
+// Block (J, K) of the element tangent, sum over the 8 Gauss points of
+// BJ^T (D dvol) BK (Brick::formResidAndTangent, Brick.cpp:970-1004), from
+// the points' tangents D (upper triangle), shape derivatives shp[g][dir][node]
+// and volume weights dv staged by the caller.
+__device__ __forceinline__ void brick_block(int J, int K, const double (*D)[21], const double (*shp)[3][8],
+                                            const double* dv, double k9[3][3]) {
+#pragma unroll
+  for (int p = 0; p < 3; p++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) k9[p][q] = 0.0;
+  for (int g = 0; g < 8; g++) {
+    const double dvol = dv[g];
+    double dd[6][6];
+    {
+      int idx = 0;
+#pragma unroll
+      for (int a = 0; a < 6; a++)
+#pragma unroll
+        for (int b = a; b < 6; b++) {
+          const double v = D[g][idx++] * dvol;
+          dd[a][b] = v;
+          dd[b][a] = v;
+        }
+    }
+    const double jx = shp[g][0][J], jy = shp[g][1][J], jz = shp[g][2][J];
+    const double kx = shp[g][0][K], ky = shp[g][1][K], kz = shp[g][2][K];
+    // BJ^T rows (computeB, Brick.cpp:1047-1076):
+    //   p=0: [jx 0 0 jy 0 jz], p=1: [0 jy 0 jx jz 0], p=2: [0 0 jz 0 jy jx]
+    double bt[3][6] = {{jx, 0, 0, jy, 0, jz}, {0, jy, 0, jx, jz, 0}, {0, 0, jz, 0, jy, jx}};
+    double btd[3][6];
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int q = 0; q < 6; q++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc += bt[p][k] * dd[k][q];
+        btd[p][q] = acc;
+      }
+    // BK columns: q=0: [kx 0 0 ky 0 kz], q=1: [0 ky 0 kx kz 0], q=2: [0 0 kz 0 ky kx]
+    const double bk[6][3] = {{kx, 0, 0}, {0, ky, 0}, {0, 0, kz}, {ky, kx, 0}, {0, kz, ky}, {kz, 0, kx}};
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+      for (int q = 0; q < 3; q++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) acc += btd[p][k] * bk[k][q];
+        k9[p][q] += acc;
+      }
+  }
+}
+
 constexpr int kBrickTanElems = 4;  // one 64-lane wavefront per element
 
 // Tangent stiffness K_e = sum_g B^T (D dvol) B (Brick.cpp:925-1018,
@@ -253,53 +306,145 @@ __global__ __launch_bounds__(256) void k_brick_tangent(
   __syncthreads();
   if (!active) return;
   const int J = lane >> 3, K = lane & 7;
-  double k9[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  for (int g = 0; g < 8; g++) {
-    const double dvol = s_dv[le][g];
-    double dd[6][6];
-    {
-      int idx = 0;
-#pragma unroll
-      for (int a = 0; a < 6; a++)
-#pragma unroll
-        for (int b = a; b < 6; b++) {
-          const double v = s_D[le][g][idx++] * dvol;
-          dd[a][b] = v;
-          dd[b][a] = v;
-        }
-    }
-    const double jx = s_shp[le][g][0][J], jy = s_shp[le][g][1][J], jz = s_shp[le][g][2][J];
-    const double kx = s_shp[le][g][0][K], ky = s_shp[le][g][1][K], kz = s_shp[le][g][2][K];
-    // BJ^T rows (computeB, Brick.cpp:1047-1076):
-    //   p=0: [jx 0 0 jy 0 jz], p=1: [0 jy 0 jx jz 0], p=2: [0 0 jz 0 jy jx]
-    double bt[3][6] = {{jx, 0, 0, jy, 0, jz}, {0, jy, 0, jx, jz, 0}, {0, 0, jz, 0, jy, jx}};
-    double btd[3][6];
-#pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-      for (int q = 0; q < 6; q++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < 6; k++) acc += bt[p][k] * dd[k][q];
-        btd[p][q] = acc;
-      }
-    // BK columns: q=0: [kx 0 0 ky 0 kz], q=1: [0 ky 0 kx kz 0], q=2: [0 0 kz 0 ky kx]
-    const double bk[6][3] = {{kx, 0, 0}, {0, ky, 0}, {0, 0, kz}, {ky, kx, 0}, {0, kz, ky}, {kz, 0, kx}};
-#pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-      for (int q = 0; q < 3; q++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < 6; k++) acc += btd[p][k] * bk[k][q];
-        k9[p][q] += acc;
-      }
-  }
+  double k9[3][3];
+  brick_block(J, K, s_D[le], s_shp[le], s_dv[le], k9);
   double* out = Ke + e * 576;
 #pragma unroll
   for (int p = 0; p < 3; p++)
 #pragma unroll
     for (int q = 0; q < 3; q++) out[(3 * J + p) * 24 + 3 * K + q] = k9[p][q];
+}
+
+// formTangent of a stdBrick-only model straight into the CSR values, without
+// element matrices: one wavefront per node block row I (equations 3I ..
+// 3I+2, every node's DOFs all free or all fixed).  The elements around the
+// node are taken eight at a time in ascending tag order; lane (s, g) stages
+// element s's shape derivatives at Gauss point g, the point's stress
+// normal n and its tangent coefficients in LDS, then lane (s, b) forms the
+// element's block (a, b), a the node's local index, and puts it at its
+// neighbour's position; each CSR value then sums its elements in tag order,
+// the order of the element-by-element assembly (exact zeros stand for the
+// elements that do not touch it).  The block BaT D Bb of the J2 tangent
+// D = K 1(x)1 + 2G Idev + c2 n(x)n + c3 (Idev - n(x)n) (J2Plasticity.cpp:
+// 361-391) is formed in closed form,
+//   lam ga gb^T + mu ((ga . gb) I + gb ga^T) + gam (N ga)(N gb)^T,
+// lam = K - (2G + c3)/3, mu = G + c3/2, gam = c2 - c3, ga the shape
+// derivatives of node a, N the normal as a symmetric 3x3 tensor: 57 FMAs per
+// point instead of the 324 of the product with the 6x6 matrix.  Table (per
+// block row I, entries fz_ptr[I] .. fz_ptr[I+1]): the element, the node's
+// local index in it, and for each of its 8 nodes the neighbour position in
+// row I (255: a fixed node).
+constexpr int kFzMaxNb = 32;  // neighbour blocks per row the LDS holds
+constexpr int kFzPt = 33;     // staged doubles per (element, point): shp 24, coefficients 3, n 6
+__global__ __launch_bounds__(64) void k_brick_tangent_csr(
+    int nbr, const int* __restrict__ rowptr, const int* __restrict__ fz_ptr, const int* __restrict__ fz_ele,
+    const unsigned char* __restrict__ fz_a, const unsigned char* __restrict__ fz_tgt,
+    const int* __restrict__ conn, const int* __restrict__ mat, const J2Param* __restrict__ params,
+    const double* __restrict__ crd, long long npts, const double* __restrict__ sig,
+    const double* __restrict__ tc, double* __restrict__ vals) {
+  // staging [s][g][kFzPt]; reused for the blocks by neighbour, acc[k][s][9]
+  __shared__ double lds[kFzMaxNb * 8 * 9];
+  static_assert(8 * 8 * kFzPt <= kFzMaxNb * 8 * 9, "staging fits the block buffer");
+  const int lane = threadIdx.x, s = lane >> 3, g = lane & 7;
+  // workgroups are dealt to the 8 XCDs round-robin: give each XCD one
+  // contiguous range of rows, so that the rows sharing an element are
+  // mostly formed under one L2
+  const int per = (nbr + 7) / 8;
+  const int I0 = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  const int I1 = min(nbr, ((int)blockIdx.x & 7) * per + per);
+  for (int I = I0; I < I1; I += (int)gridDim.x >> 3) {
+    const int b0 = rowptr[3 * I], len = rowptr[3 * I + 1] - b0, nnb = len / 3;
+    const int p0 = fz_ptr[I], ne = fz_ptr[I + 1] - p0;
+    double run[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // output o = lane + 64 i, o = p * len + k
+    for (int pass = 0; pass * 8 < ne; pass++) {
+      const int es = pass * 8 + s;
+      const bool vs = es < ne;
+      const int e = vs ? fz_ele[p0 + es] : 0;
+      if (vs) {
+        double xl[3][8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          const int nd = conn[8LL * e + k];
+#pragma unroll
+          for (int d = 0; d < 3; d++) xl[d][k] = crd[3LL * nd + d];
+        }
+        const long long pt = 8LL * e + g;
+        double st[6], nn[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) st[c] = sig[c * npts + pt];
+        const double c2 = tc[pt], c3 = tc[npts + pt];
+        const J2Param pr = params[mat[e]];
+        double xn[3], shp[4][8];
+        brick_gauss(g, xn);
+        const double dvol = 1.0 * dev_shp3d(xn, shp, xl);
+        j2_stress_normal(st, nn);
+        double* o = lds + (s * 8 + g) * kFzPt;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+          for (int k = 0; k < 8; k++) o[d * 8 + k] = shp[d][k];
+        o[24] = (pr.K - (2.0 * pr.G + c3) * (1.0 / 3.0)) * dvol;
+        o[25] = (pr.G + 0.5 * c3) * dvol;
+        o[26] = (c2 - c3) * dvol;
+#pragma unroll
+        for (int c = 0; c < 6; c++) o[27 + c] = nn[c];
+      }
+      __syncthreads();
+      double k9[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      if (vs) {
+        const int a = fz_a[p0 + es];
+        for (int gg = 0; gg < 8; gg++) {
+          const double* q = lds + (s * 8 + gg) * kFzPt;
+          const double ga[3] = {q[a], q[8 + a], q[16 + a]};
+          const double gb[3] = {q[g], q[8 + g], q[16 + g]};
+          const double lam = q[24], mu = q[25], gam = q[26];
+          // N rows: (n00 n01 n20), (n01 n11 n12), (n20 n12 n22)
+          const double n00 = q[27], n11 = q[28], n22 = q[29], n01 = q[30], n12 = q[31], n20 = q[32];
+          const double va[3] = {n00 * ga[0] + n01 * ga[1] + n20 * ga[2], n01 * ga[0] + n11 * ga[1] + n12 * ga[2],
+                                n20 * ga[0] + n12 * ga[1] + n22 * ga[2]};
+          const double vb[3] = {n00 * gb[0] + n01 * gb[1] + n20 * gb[2], n01 * gb[0] + n11 * gb[1] + n12 * gb[2],
+                                n20 * gb[0] + n12 * gb[1] + n22 * gb[2]};
+          const double dot = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+#pragma unroll
+          for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+              double v = lam * ga[p] * gb[r] + mu * gb[p] * ga[r] + gam * va[p] * vb[r];
+              if (p == r) v += mu * dot;
+              k9[p][r] += v;
+            }
+        }
+      }
+      __syncthreads();
+      for (int i = lane; i < nnb * 72; i += 64) lds[i] = 0.0;
+      __syncthreads();
+      const int tg = vs ? fz_tgt[8LL * (p0 + es) + g] : 255;
+      if (tg != 255) {
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+          for (int q = 0; q < 3; q++) lds[(tg * 8 + s) * 9 + 3 * p + q] = k9[p][q];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        const int o = lane + 64 * i;
+        if (o < 3 * len) {
+          const int p = o / len, k = o - p * len, j = k / 3, q = k - 3 * j;
+          double r = run[i];
+#pragma unroll
+          for (int s2 = 0; s2 < 8; s2++) r += lds[(j * 8 + s2) * 9 + 3 * p + q];
+          run[i] = r;
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      const int o = lane + 64 * i;
+      if (o < 3 * len) vals[b0 + o] = run[i];
+    }
+  }
 }
 
 // commitState of every point (J2Plasticity.cpp:516-522): n <- n+1.
@@ -333,6 +478,16 @@ void launch_brick_tangent(hipStream_t s, int nele, const int* conn, const int* m
   if (nele <= 0) return;
   hipLaunchKernelGGL(k_brick_tangent, dim3(ceil_div(nele, kBrickTanElems)), dim3(256), 0, s, nele,
                      conn, mat, prm, crd, npts, sig, tc, Ke);
+}
+
+void launch_brick_tangent_csr(hipStream_t s, int nbr, const int* rowptr, const int* fz_ptr, const int* fz_ele,
+                              const unsigned char* fz_a, const unsigned char* fz_tgt, const int* conn,
+                              const int* mat, const J2Param* prm, const double* crd, long long npts,
+                              const double* sig, const double* tc, double* vals) {
+  if (nbr <= 0) return;
+  const int grid = 8 * std::min((nbr + 7) / 8, 1 << 17);  // a multiple of the 8 XCDs
+  hipLaunchKernelGGL(k_brick_tangent_csr, dim3(grid), dim3(64), 0, s, nbr, rowptr, fz_ptr, fz_ele, fz_a, fz_tgt,
+                     conn, mat, prm, crd, npts, sig, tc, vals);
 }
 
 void launch_j2_commit(hipStream_t s, long long npts, int ncomp, const double* ept, const double* xit,

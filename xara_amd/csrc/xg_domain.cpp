@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "xg_frame.h"
@@ -72,6 +73,12 @@ struct J2Group : Group {
   void tangent(Domain& d, double* Ke) override {
     if (brick) launch_brick_tangent(d.stream, nele(), dconn.p, dmat.p, dprm.p, d.crd.p, np(), sig.p, D.p, Ke);
     else launch_quad_tangent(d.stream, nele(), dconn.p, d.crd.p, dthk.p, np(), D.p, Ke);
+  }
+  bool tangent_csr(Domain& d) override {
+    if (!brick || !d.fz_ok) return false;
+    launch_brick_tangent_csr(d.stream, d.num.neq / 3, d.rowptr.p, d.fz_ptr.p, d.fz_ele.p, d.fz_a.p, d.fz_tgt.p,
+                             dconn.p, dmat.p, dprm.p, d.crd.p, np(), sig.p, D.p, d.vals.p);
+    return true;
   }
   void commit(Domain& d) override { launch_j2_commit(d.stream, np(), ncomp(), ept.p, xit.p, epn.p, xin.p); }
   void revert(Domain&) override {}  // J2Plasticity::revertToLastCommit is a no-op (J2Plasticity.cpp:524)
@@ -219,6 +226,75 @@ int Domain::lanes() {
   if (mean > 128) return 16;
   if (mean > 24) return 8;
   return 4;
+}
+
+// Tables of k_brick_tangent_csr for a model of stdBrick elements alone whose
+// nodes have all 3 DOFs free or none (the node-blocked CSR view exists):
+// per node block row I, the elements containing its node in ascending tag
+// order (the group's order), the node's local index in each, and for each
+// element node the position of its block in row I (255 for a fixed node).
+// Off when another group is present, a row has more than kBrickCsrMaxNb
+// neighbour blocks, or XG_BRICK_TANGENT_CSR=0 is set (the element-matrix
+// path, for comparison).
+void Domain::fused_tangent_tables() {
+  fz_ok = false;
+  fz_ptr.release();
+  fz_ele.release();
+  fz_a.release();
+  fz_tgt.release();
+  const char* env = std::getenv("XG_BRICK_TANGENT_CSR");
+  if (env && std::strcmp(env, "0") == 0) return;
+  if (groups.size() != 1 || nb_m != 3 || ndf != 3 || num.neq == 0) return;
+  auto* jg = dynamic_cast<J2Group*>(groups[0].get());
+  if (!jg || !jg->brick) return;
+  const int nbr = num.neq / 3;
+  const std::vector<int>& rp = num.rowptr;
+  const std::vector<int>& ci = num.colind;
+  std::vector<int> node_of(nbr, -1);
+  for (int n = 0; n < nnode; n++)
+    if (num.eq[(size_t)n * 3] >= 0) node_of[num.eq[(size_t)n * 3] / 3] = n;
+  std::vector<int> cnt(nnode + 1, 0);
+  const int ne = jg->nele();
+  for (int e = 0; e < ne; e++)
+    for (int a = 0; a < 8; a++) cnt[jg->conn[8 * (size_t)e + a] + 1]++;
+  for (int n = 0; n < nnode; n++) cnt[n + 1] += cnt[n];
+  std::vector<int> ele_at(cnt[nnode]), loc_at(cnt[nnode]), fill(cnt.begin(), cnt.end() - 1);
+  for (int e = 0; e < ne; e++)  // ascending e: each node's list in tag order
+    for (int a = 0; a < 8; a++) {
+      const int n = jg->conn[8 * (size_t)e + a];
+      ele_at[fill[n]] = e;
+      loc_at[fill[n]++] = a;
+    }
+  std::vector<int> ptr(nbr + 1, 0), ele;
+  std::vector<unsigned char> loc, tgt;
+  for (int I = 0; I < nbr; I++) {
+    const int n = node_of[I];
+    if (n < 0) return;
+    const int b0 = rp[3 * I], len = rp[3 * I + 1] - b0, nnb = len / 3;
+    if (nnb > kBrickCsrMaxNb) return;
+    for (int k = cnt[n]; k < cnt[n + 1]; k++) {
+      const int e = ele_at[k];
+      ele.push_back(e);
+      loc.push_back((unsigned char)loc_at[k]);
+      for (int b = 0; b < 8; b++) {
+        const int eqb = num.eq[(size_t)jg->conn[8 * (size_t)e + b] * 3];
+        unsigned char t = 255;
+        if (eqb >= 0) {
+          const int* c0 = &ci[b0];
+          const int* hit = std::lower_bound(c0, c0 + len, eqb);
+          if (hit == c0 + len || *hit != eqb) return;  // not in the pattern: leave the element path on
+          t = (unsigned char)((hit - c0) / 3);
+        }
+        tgt.push_back(t);
+      }
+    }
+    ptr[I + 1] = (int)ele.size();
+  }
+  fz_ptr.upload(ptr, stream);
+  fz_ele.upload(ele, stream);
+  fz_a.upload(loc, stream);
+  fz_tgt.upload(tgt, stream);
+  fz_ok = true;
 }
 
 // Node-blocked view of the CSR pattern for k_spmv_nb_dot: with M = ndf, it
@@ -553,6 +629,7 @@ void Domain::setup() {
   scal.resize(cg_scal_bytes());
   dscalar.resize(4);
   for (auto& g : groups) g->setup(*this);
+  fused_tangent_tables();
   setup_transient();
   // the PCG's block-symmetric layout depends only on the pattern: built here
   // with the other index maps rather than in the first solve
@@ -676,10 +753,15 @@ void Domain::rayleigh_kc_init() {
 }
 
 void Domain::form_tangent() {
-  for (auto& g : groups) g->tangent(*this, Ke.p + g->ke_off);
-  // Newmark::formEleTangent: K + c2 (betaK K + betaKc Kc) per element entry
-  if (stiffness_damping()) launch_ke_damp(stream, ke_total, nm_c2, betaK, betaKc, Kc.p, Ke.p);
-  launch_assemble_csr(stream, (long long)num.colind.size(), asm_ptr.p, asm_idx.p, Ke.p, vals.p);
+  // a stdBrick-only model without stiffness-proportional damping: element
+  // tangents formed and summed into the CSR values in one pass
+  const bool fused = groups.size() == 1 && !stiffness_damping() && groups[0]->tangent_csr(*this);
+  if (!fused) {
+    for (auto& g : groups) g->tangent(*this, Ke.p + g->ke_off);
+    // Newmark::formEleTangent: K + c2 (betaK K + betaKc Kc) per element entry
+    if (stiffness_damping()) launch_ke_damp(stream, ke_total, nm_c2, betaK, betaKc, Kc.p, Ke.p);
+    launch_assemble_csr(stream, (long long)num.colind.size(), asm_ptr.p, asm_idx.p, Ke.p, vals.p);
+  }
   sym_stale = true;
   pc_done = nullptr;
   // Newmark::formNodTangent: c2 C + c3 M of the nodal masses (C = alphaM M)

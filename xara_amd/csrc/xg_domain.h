@@ -31,6 +31,9 @@ struct Group {
   virtual void update(Domain& d) = 0;
   virtual void resid(Domain& d, double* Re) = 0;
   virtual void tangent(Domain& d, double* Ke) = 0;
+  // formTangent of a model made of this group alone straight into the CSR
+  // values, where the group has such a kernel; false if it has not
+  virtual bool tangent_csr(Domain&) { return false; }
   virtual void mass_diag(Domain&, std::vector<double>&) {}  // lumped element mass per element DOF
   virtual void commit(Domain& d) = 0;
   virtual void revert(Domain& d) = 0;
@@ -119,6 +122,13 @@ struct Domain {
   DVec<double> r0, r1, z, p, p2, q, part, binv, tmpv;
   DVec<int> bfirst, blk_first;
   DVec<unsigned char> bsize, blk_size;
+  // stdBrick-only models: per node block row, the elements around the node
+  // (ascending tag), the node's local index in each, and each element
+  // node's neighbour position in the row (k_brick_tangent_csr)
+  DVec<int> fz_ptr, fz_ele;
+  DVec<unsigned char> fz_a, fz_tgt;
+  bool fz_ok = false;
+  void fused_tangent_tables();
   DVec<char> scal;
   DVec<double> dscalar;
   int nblk = 0;

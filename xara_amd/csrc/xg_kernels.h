@@ -11,6 +11,13 @@ void launch_brick_update(hipStream_t s, int nele, const int* conn, const int* ma
                          double* tc, int* nfail);
 void launch_brick_resid(hipStream_t s, int nele, const int* conn, const double* crd,
                         const double* bforce, long long npts, const double* sig, double* Re);
+// formTangent of a stdBrick-only model straight into the CSR values (node
+// block rows of 3 equations; per-row element tables, see k_brick_tangent_csr)
+constexpr int kBrickCsrMaxNb = 32;
+void launch_brick_tangent_csr(hipStream_t s, int nbr, const int* rowptr, const int* fz_ptr, const int* fz_ele,
+                              const unsigned char* fz_a, const unsigned char* fz_tgt, const int* conn,
+                              const int* mat, const J2Param* prm, const double* crd, long long npts,
+                              const double* sig, const double* tc, double* vals);
 void launch_brick_tangent(hipStream_t s, int nele, const int* conn, const int* mat, const J2Param* prm,
                           const double* crd, long long npts, const double* sig, const double* tc, double* Ke);
 void launch_j2_commit(hipStream_t s, long long npts, int ncomp, const double* ept, const double* xit,
