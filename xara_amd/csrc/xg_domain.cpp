@@ -804,6 +804,7 @@ void Domain::band_order() {
   band_bw = -1;
   band_pinv.release();
   band_A.release();
+  band2.release();
   band_L.release();
   band_Lr.release();
   const int n = num.neq;
@@ -849,6 +850,8 @@ void Domain::band_order() {
   band_bw = best;
   band_pinv.upload(pinv, stream);
   band_A.resize((size_t)n * (best + 1));
+  band2.resize((size_t)5 * n);
+  scal2.resize(cg_scal_bytes());
   band_L.resize((size_t)n * (best + 1));
   band_Lr.resize((size_t)n * (best + 1));
 }
@@ -903,6 +906,40 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x, const Cg
   XG_HIP(hipGraphLaunch(g.exec, stream));
 }
 
+// Two solves with the current matrix, B1 -> X1 and B2 -> X2
+// (DisplacementControl::update's unbalance and reference-load solves).  With
+// the band preconditioner the two run side by side in one launch (one
+// workgroup each, the same kernel and factor as solve(), so the same
+// results); otherwise one after the other.
+int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x2) {
+  const int n = num.neq;
+  if (n == 0 || !band_active() || band2.n < (size_t)5 * n) {
+    const int r = solve(b1, x1);
+    return r < 0 ? r : solve(b2, x2);
+  }
+  const CgBufs w = cg_bufs();
+  precond_setup(w);
+  const BandRhs R1{b1, x1, w.r0, w.z, w.p, w.q, w.r1, (CGScal*)w.scal};
+  double* v = band2.p;
+  const BandRhs R2{b2, x2, v, v + n, v + 2 * (size_t)n, v + 3 * (size_t)n, v + 4 * (size_t)n, (CGScal*)scal2.p};
+  cg_solve_band(stream, n, rowptr.p, colind.p, vals.p, band_bw, band_pinv.p, w.bandL, w.bandLr, rtol * rtol, maxit,
+                R1, &R2);
+  int bad = 0;
+  XG_HIP(hipMemcpyAsync(&bad, w.pcbad, sizeof(int), hipMemcpyDeviceToHost, stream));
+  int status = 0;
+  for (int k = 0; k < 2; k++) {
+    int conv = 0, it = 0, brk = 0;
+    double rr = 0, bb = 0;
+    cg_read_scal(stream, k == 0 ? w.scal : (void*)scal2.p, &conv, &it, &rr, &bb, &brk);
+    counters.solves++;
+    counters.cg_iters += it;
+    counters.last_cg_relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    if (bb > 0 && !(rr <= rtol * rtol * bb)) status = XG_BAD_LINEAR_SOLVE;
+  }
+  if (bad) return XG_BAD_LINEAR_SOLVE;
+  return status;
+}
+
 int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs) {
   const int n = num.neq;
   if (n == 0) return 0;
@@ -916,8 +953,9 @@ int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs)
   double rr = 0, bb = 0;
   if (band_active()) {
     // one workgroup, one launch, preconditioned by the band's exact factor
-    cg_solve_band(stream, n, rowptr.p, colind.p, vals.p, b, x, w.r0, w.z, w.p, w.q, w.r1, band_bw, band_pinv.p,
-                  w.bandL, w.bandLr, rtol * rtol, maxit, w.scal);
+    const BandRhs R{b, x, w.r0, w.z, w.p, w.q, w.r1, (CGScal*)w.scal};
+    cg_solve_band(stream, n, rowptr.p, colind.p, vals.p, band_bw, band_pinv.p, w.bandL, w.bandLr, rtol * rtol, maxit,
+                  R);
     cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);
   } else if (cg_small_on && cg_small_fits(n, (long long)num.colind.size())) {
     // the whole solve in one workgroup, one launch
@@ -1094,7 +1132,9 @@ int Domain::integ_update(const double* x) {
   // DisplacementControl::update, DisplacementControl.cpp:224-283
   XG_HIP(hipMemcpyAsync(dUbar.p, x, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
   const double dUabar = read_eq(dUbar.p, dc_eq);
-  if (solve(phat.p, dUhat.p) < 0) return XG_BAD_STEP;
+  const bool hat_ready = dc_hat_ready;
+  dc_hat_ready = false;
+  if (!hat_ready && solve(phat.p, dUhat.p) < 0) return XG_BAD_STEP;
   const double dUahat = read_eq(dUhat.p, dc_eq);
   if (dUahat == 0.0) return XG_BAD_STEP;
   const double dL = -dUabar / dUahat;
@@ -1115,7 +1155,14 @@ int Domain::newton() {
   std::vector<double> norms;
   do {
     form_tangent();
-    if (solve(B.p, X.p) < 0) return XG_BAD_LINEAR_SOLVE;
+    if (integ == 2 && band_active()) {
+      // DisplacementControl::update solves for the reference load with the
+      // same matrix: both solves at once
+      if (solve_pair(B.p, X.p, phat.p, dUhat.p) < 0) return XG_BAD_LINEAR_SOLVE;
+      dc_hat_ready = true;
+    } else if (solve(B.p, X.p) < 0) {
+      return XG_BAD_LINEAR_SOLVE;
+    }
     if (integ_update(X.p) < 0) return XG_BAD_STEP;
     form_unbalance();
     result = run_test(iter, norms, numIncr);

@@ -210,6 +210,10 @@ struct Domain {
   int band_bw = -1;
   DVec<int> band_pinv;
   DVec<double> band_A, band_L, band_Lr;
+  DVec<double> band2;  // work vectors of a second band solve (solve_pair)
+  DVec<char> scal2;
+  bool dc_hat_ready = false;  // the Newton iteration has solved for the reference load
+  int solve_pair(const double* b1, double* x1, const double* b2, double* x2);
   bool band_active() const { return cg_small_on && band_bw >= 0; }
   void band_order();
   // the preconditioner's buffers last formed from the current matrix

@@ -105,10 +105,19 @@ void cg_iterations_check(int bw, const SymSpmv* sym);
 // doubles each
 void band_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* colind, const double* vals,
                  const int* pinv, double* A, double* L, double* Lr, int* bad);
-// whole band-preconditioned PCG solve in one workgroup; t: n doubles of work
-void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
-                   double* x, double* r, double* z, double* p, double* q, double* t, int bw, const int* pinv,
-                   const double* L, const double* Lr, double tol2, int maxit, void* scal);
+// whole band-preconditioned PCG solve in one workgroup per right-hand side
+// (one, or two side by side); r, z, p, q, t: n doubles of work each, S the
+// solve's scalars
+struct CGScal;
+struct BandRhs {
+  const double* b;
+  double* x;
+  double *r, *z, *p, *q, *t;
+  CGScal* S;
+};
+void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, int bw,
+                   const int* pinv, const double* L, const double* Lr, double tol2, int maxit, const BandRhs& r0,
+                   const BandRhs* r1 = nullptr);
 // whole solve in one workgroup (k_pcg_small) for systems that fit it
 bool cg_small_fits(int n, long long nnz);
 void cg_solve_small(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,

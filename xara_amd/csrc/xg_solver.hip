@@ -1247,10 +1247,20 @@ __device__ __attribute__((noinline)) void band_sweeps(int n, int bw, const doubl
 // exact factor the first iteration meets the tolerance; the test comes
 // before the next application, so a converged solve applies it once.
 __global__ __launch_bounds__(kSmallPcgThreads) void k_pcg_band(
-    int n, const int* __restrict__ rowptr, const int* __restrict__ colind, const double* __restrict__ vals,
-    const double* __restrict__ b, double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
-    double* __restrict__ p, double* __restrict__ q, double* __restrict__ t, int bw, const int* __restrict__ pinv,
-    const double* __restrict__ L, const double* __restrict__ Lr, double tol2, int maxit, CGScal* __restrict__ S) {
+    int n, const int* __restrict__ rowptr, const int* __restrict__ colind, const double* __restrict__ vals, int bw,
+    const int* __restrict__ pinv, const double* __restrict__ L, const double* __restrict__ Lr, double tol2, int maxit,
+    BandRhs r0, BandRhs r1) {
+  // workgroup k solves right-hand side k (two independent solves with one
+  // factor run side by side on two CUs)
+  const BandRhs& R = blockIdx.x == 0 ? r0 : r1;
+  const double* __restrict__ b = R.b;
+  double* __restrict__ x = R.x;
+  double* __restrict__ r = R.r;
+  double* __restrict__ z = R.z;
+  double* __restrict__ p = R.p;
+  double* __restrict__ q = R.q;
+  double* __restrict__ t = R.t;
+  CGScal* __restrict__ S = R.S;
   __shared__ double sh[kSmallPcgThreads / 64];
   __shared__ double stage[3 * kBandTile];
   const int tid = threadIdx.x;
@@ -1737,11 +1747,11 @@ void band_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* col
   else hipLaunchKernelGGL(k_band_factor<63>, dim3(1), dim3(64), 0, s, n, bw, A, L, Lr, bad);
 }
 
-void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
-                   double* x, double* r, double* z, double* p, double* q, double* t, int bw, const int* pinv,
-                   const double* L, const double* Lr, double tol2, int maxit, void* scal) {
-  hipLaunchKernelGGL(k_pcg_band, dim3(1), dim3(kSmallPcgThreads), 0, s, n, rowptr, colind, vals, b, x, r, z, p, q, t,
-                     bw, pinv, L, Lr, tol2, maxit, (CGScal*)scal);
+void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, int bw,
+                   const int* pinv, const double* L, const double* Lr, double tol2, int maxit, const BandRhs& r0,
+                   const BandRhs* r1) {
+  hipLaunchKernelGGL(k_pcg_band, dim3(r1 ? 2 : 1), dim3(kSmallPcgThreads), 0, s, n, rowptr, colind, vals, bw, pinv, L,
+                     Lr, tol2, maxit, r0, r1 ? *r1 : r0);
 }
 
 bool cg_small_fits(int n, long long nnz) { return n > 0 && n <= 1024 && nnz <= 32768; }
