@@ -146,6 +146,8 @@ def main():
                     help="alpha / beta reductions inside the PCG vector kernels on small systems")
     ap.add_argument("--cg-small", type=int, choices=[0, 1], default=1,
                     help="whole PCG solve in one workgroup for systems that fit it")
+    ap.add_argument("--cg-warm", type=int, choices=[0, 1], default=1,
+                    help="start a step's first solve from the previous step's solution (0 = from zero)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -234,7 +236,7 @@ def gpu_system(m, args, rtol, maxit):
     """The PCG system of a GPU model with the bench's issue pattern."""
     m.system(args.system, rtol, maxit)
     if m._b.prefix == "xg_":
-        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small))
+        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small), bool(args.cg_warm))
 
 
 def newton_pushover(m, n, steps, max_seconds=None, max_steps=None, log=True):
@@ -638,7 +640,7 @@ def ensemble_main(args):
         m = Model(2, 3)
         define(m, k)
         m.system("GpuBlockPCG", 1e-14, 100000)
-        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small))
+        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small), bool(args.cg_warm))
         m.setup()
         for _ in range(args.warmup):
             m.analyze(1, dt)

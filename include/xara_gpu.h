@@ -189,6 +189,11 @@ int  xg_spmv_config(xg_domain* d, int format, int lanes);
 #define XG_CG_GRAPH 1
 #define XG_CG_FUSE  2
 #define XG_CG_SMALL 4
+#define XG_CG_WARM  8  /* under LoadControl the first Newton iteration of a step
+                          starts from the previous step's first solution,
+                          DisplacementControl's reference-load solves from the
+                          previous one (the chunked PCG; the default; the stop
+                          test stays |r| <= rtol |b|)                          */
 int  xg_cg_config(xg_domain* d, int chunk, int flags);
 /* `numberer RCM|Plain`                                                       */
 int  xg_numberer(xg_domain* d, int kind);

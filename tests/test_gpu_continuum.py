@@ -355,6 +355,51 @@ def test_single_workgroup_pcg_matches_launch_per_step(system):
             assert ia <= 3 * sa and ib > 3 * sb
 
 
+@pytest.mark.parametrize("case", ["brick", "frame3d"])
+def test_pcg_warm_start_same_trajectory_fewer_iterations(case):
+    """Warm starts of the chunked PCG (XG_CG_WARM: under LoadControl a step's
+    first solve from the previous step's first solution, DisplacementControl's
+    reference-load solves from the previous one) against starts from zero:
+    the same converged trajectory to 1e-9 on a brick LoadControl pushover
+    past first yield (with far fewer PCG iterations: the elastic steps' first
+    solves start from the answer) and on a 3D RC frame DisplacementControl
+    pushover (where only the reference-load solves start warm, a few percent
+    of the iterations, within the spread that rounding gives the rest)."""
+    print(f"case: warm start {case}", flush=True)
+    res = []
+    for warm in (True, False):
+        if case == "brick":
+            mesh = meshes.brick_cube(6)
+            m = Model(3, 3)
+            meshes.build_brick_cube(m, mesh, pressure=1.0)
+            m.system("GpuBlockPCG", 1e-12, 20000)
+            m.test("NormUnbalance", 1e-8, 30)
+            m.integrator("LoadControl", 20.0)
+            nsteps = 8
+        else:
+            fr = meshes.rc_frame_3d(3, 3, 4)
+            m = Model(3, 6)
+            meshes.build_rc_frame_3d(m, fr)
+            m.system("GpuBlockPCG", 1e-12, 100000)
+            m.test("NormUnbalance", 1e-5, 40)
+            m.integrator("DisplacementControl", fr["roof"], 1, 8.0)
+            nsteps = 6
+        m.cg_config(16, True, True, small=False, warm=warm)
+        m.setup()
+        traj = []
+        for _ in range(nsteps):
+            assert m.analyze(1) == 0
+            traj.append(m.trial_disp().copy())
+        res.append((np.array(traj), m.counters()["cg_iters"]))
+        m.close()
+    (a, ia), (b, ib) = res
+    assert relerr(a, b) <= 1e-9
+    if case == "brick":
+        assert ia < 0.5 * ib, (ia, ib)
+    else:
+        assert ia <= 1.03 * ib, (ia, ib)
+
+
 def test_revert_to_start_and_to_last_commit():
     """Domain::revertToStart after a plastic trial gives back the fresh model
     (its closing update at zero displacement forms the elastic tangent,

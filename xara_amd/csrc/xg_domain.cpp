@@ -608,6 +608,9 @@ void Domain::setup() {
   B.resize(neq);
   B.zero(s);
   X.resize(neq);
+  Xw.resize(neq);
+  xw_valid = false;
+  dc_hat_valid = false;
   X.zero(s);
   Peq.upload(Pe, s);
   nfail.resize(1);
@@ -940,7 +943,7 @@ int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x
   return status;
 }
 
-int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs) {
+int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs, bool warm) {
   const int n = num.neq;
   if (n == 0) return 0;
   const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
@@ -963,7 +966,10 @@ int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs)
                    rtol * rtol, maxit, w.scal);
     cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);
   } else {
-    cg_start(stream, n, b, x, w.r0, w.z, w.p, bw, w.binv, bfirst.p, bsize.p, w.part, rtol * rtol, maxit, w.scal);
+    // a warm start takes the guess in x: q = A x first
+    if (warm) spmv(x, w.q, true);
+    cg_start(stream, n, b, x, w.r0, w.z, w.p, bw, w.binv, bfirst.p, bsize.p, w.part, rtol * rtol, maxit, w.scal,
+             warm ? w.q : nullptr);
     int issued = 0, next = cg_chunk;
     const int L = lanes();
     while (true) {
@@ -1103,7 +1109,8 @@ int Domain::new_step() {
   if (dc_incr < dc_min) dc_incr = dc_min;
   else if (dc_incr > dc_max) dc_incr = dc_max;
   form_tangent();
-  if (solve(phat.p, dUhat.p) < 0) return XG_BAD_STEP;
+  if (solve_from(phat.p, dUhat.p, cg_warm_on && dc_hat_valid) < 0) return XG_BAD_STEP;
+  dc_hat_valid = true;
   const double dUahat = read_eq(dUhat.p, dc_eq);
   if (dUahat == 0.0) return XG_BAD_STEP;
   const double dl = dc_incr / dUahat;
@@ -1134,7 +1141,8 @@ int Domain::integ_update(const double* x) {
   const double dUabar = read_eq(dUbar.p, dc_eq);
   const bool hat_ready = dc_hat_ready;
   dc_hat_ready = false;
-  if (!hat_ready && solve(phat.p, dUhat.p) < 0) return XG_BAD_STEP;
+  if (!hat_ready && solve_from(phat.p, dUhat.p, cg_warm_on && dc_hat_valid) < 0) return XG_BAD_STEP;
+  dc_hat_valid = true;
   const double dUahat = read_eq(dUhat.p, dc_eq);
   if (dUahat == 0.0) return XG_BAD_STEP;
   const double dL = -dUabar / dUahat;
@@ -1153,6 +1161,7 @@ int Domain::newton() {
   form_unbalance();
   int iter = 1, numIncr = 0, result = -1;
   std::vector<double> norms;
+  bool first = true;
   do {
     form_tangent();
     if (integ == 2 && band_active()) {
@@ -1160,9 +1169,22 @@ int Domain::newton() {
       // same matrix: both solves at once
       if (solve_pair(B.p, X.p, phat.p, dUhat.p) < 0) return XG_BAD_LINEAR_SOLVE;
       dc_hat_ready = true;
-    } else if (solve(B.p, X.p) < 0) {
-      return XG_BAD_LINEAR_SOLVE;
+    } else {
+      // under LoadControl the first iteration of a step solves for the same
+      // load increment as the last step's first iteration did, on a tangent
+      // that changed little: start it from that solution.  (The first
+      // residual of a DisplacementControl or Newmark step is the unbalance
+      // left by the predictor, which the last step's solve does not guess.)
+      const bool warm_step = cg_warm_on && first && integ == 1;
+      const bool warm = warm_step && xw_valid;
+      if (warm) launch_copy_d(stream, num.neq, Xw.p, X.p);
+      if (solve_from(B.p, X.p, warm) < 0) return XG_BAD_LINEAR_SOLVE;
+      if (warm_step) {
+        launch_copy_d(stream, num.neq, X.p, Xw.p);
+        xw_valid = true;
+      }
     }
+    first = false;
     if (integ_update(X.p) < 0) return XG_BAD_STEP;
     form_unbalance();
     result = run_test(iter, norms, numIncr);
@@ -1186,6 +1208,8 @@ int Domain::analyze(int nsteps, double dtv) {
     const int n = num.neq;
     phat.resize(n);
     dUhat.resize(n);
+    dUhat.zero(stream);
+    dc_hat_valid = false;
     dUbar.resize(n);
     dU.resize(n);
     lambda += 1.0;

@@ -151,6 +151,13 @@ struct Domain {
   bool cg_graph_on = true;
   bool cg_fuse_on = true;  // alpha / beta reductions fused into their neighbours (small systems)
   bool cg_small_on = true; // whole solve in one workgroup where the system fits (cg_small_fits)
+  // warm starts of the PCG: under LoadControl the first Newton iteration of a
+  // step starts from the previous step's first solution, DisplacementControl's
+  // reference-load solves from the previous reference-load solution (the
+  // chunked PCG only; a solve still stops at |r| <= rtol |b|)
+  bool cg_warm_on = true;
+  DVec<double> Xw;          // the previous step's first-iteration solution
+  bool xw_valid = false, dc_hat_valid = false;
   std::vector<CgGraph> cg_graphs;
   int test_kind = XG_TEST_NORM_UNBALANCE;
   double test_tol = 1e-6;
@@ -222,8 +229,10 @@ struct Domain {
   const void* pc_done = nullptr;
   CgBufs cg_bufs();
   int solve(const double* b, double* x) { return solve_with(b, x, cg_bufs(), cg_graph_on); }  // device pointers
+  // as solve, starting from the guess in x (warm) or from zero
+  int solve_from(const double* b, double* x, bool warm) { return solve_with(b, x, cg_bufs(), cg_graph_on, warm); }
   // returns < 0 on failure; graphs: replay chunks from captured graphs
-  int solve_with(const double* b, double* x, const CgBufs& w, bool graphs);
+  int solve_with(const double* b, double* x, const CgBufs& w, bool graphs, bool warm = false);
   void incr_disp(const double* dU_dev, double scale = 1.0);
   void commit();
   void revert();

@@ -88,9 +88,10 @@ void launch_dot(hipStream_t s, int n, const double* x, const double* y, double* 
 
 size_t cg_scal_bytes();
 size_t cg_part_size(int n);  // doubles of the partial-sum buffer of an n-equation PCG
+// q == nullptr: x = 0; else x holds the initial guess and q = A x
 void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, double* z, double* p, int bw,
               const double* binv, const int* bfirst, const unsigned char* bsize, double* part, double tol2,
-              int maxit, void* scal);
+              int maxit, void* scal, const double* q = nullptr);
 void cg_iterations(hipStream_t s, int iters, int it0, int lanes, int n, const int* rowptr,
                    const int* colind, const double* vals, double* x, double* r0, double* r1, double* z,
                    double* p, double* p2, double* q, int bw, const double* binv, const int* bfirst,

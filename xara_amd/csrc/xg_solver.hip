@@ -638,61 +638,72 @@ __global__ void k_cg_newp(int n, const CGScal* __restrict__ s, const double* __r
 }
 
 // Start: x = 0, r = b, z = M^{-1} b, p = z; partials of r.z, b.b
+// x = 0, r = b (q == nullptr), or, from the initial guess already in x and
+// q = A x, r = b - q; z = M^-1 r, p = z, and per workgroup the partials of
+// r.z, b.b and r.r.
 __global__ __launch_bounds__(256) void k_cg_start(int n, const double* __restrict__ b,
-                                                  double* __restrict__ x, double* __restrict__ r,
-                                                  double* __restrict__ z, double* __restrict__ p,
-                                                  int bw, const double* __restrict__ binv,
+                                                  const double* __restrict__ q, double* __restrict__ x,
+                                                  double* __restrict__ r, double* __restrict__ z,
+                                                  double* __restrict__ p, int bw, const double* __restrict__ binv,
                                                   const int* __restrict__ bfirst,
                                                   const unsigned char* __restrict__ bsize,
                                                   double* __restrict__ part) {
   __shared__ double sh[256];
   const int i = blockIdx.x * 256 + threadIdx.x;
-  double rz = 0.0, bb = 0.0;
+  double rz = 0.0, bb = 0.0, rr = 0.0;
   if (i < n) {
-    x[i] = 0.0;
-    r[i] = b[i];
+    auto res = [&](int k) { return q ? b[k] - q[k] : b[k]; };
+    if (!q) x[i] = 0.0;
+    const double ri = res(i);
+    r[i] = ri;
     double zi;
-    if (bw == 1) zi = binv[i] * b[i];
+    if (bw == 1) zi = binv[i] * ri;
     else {
       zi = 0.0;
       const int f = bfirst[i], m = bsize[i];
-      for (int j = 0; j < m; j++) zi += binv[(long long)i * bw + j] * b[f + j];
+      for (int j = 0; j < m; j++) zi += binv[(long long)i * bw + j] * res(f + j);
     }
     z[i] = zi;
     p[i] = zi;
-    rz = b[i] * zi;
+    rz = ri * zi;
     bb = b[i] * b[i];
+    rr = ri * ri;
   }
   const double s1 = block_sum<256>(rz, sh);
   const double s2 = block_sum<256>(bb, sh);
+  const double s3 = block_sum<256>(rr, sh);
   if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = s1;
-    part[2 * blockIdx.x + 1] = s2;
+    part[3 * blockIdx.x] = s1;
+    part[3 * blockIdx.x + 1] = s2;
+    part[3 * blockIdx.x + 2] = s3;
   }
 }
 
 __global__ __launch_bounds__(256) void k_cg_init_scal(int nb, const double* __restrict__ part,
                                                       double tol2, int maxit, CGScal* __restrict__ s) {
   __shared__ double sh[256];
-  double v1 = 0.0, v2 = 0.0;
+  double v1 = 0.0, v2 = 0.0, v3 = 0.0;
   for (int i = threadIdx.x; i < nb; i += 256) {
-    v1 += part[2 * i];
-    v2 += part[2 * i + 1];
+    v1 += part[3 * i];
+    v2 += part[3 * i + 1];
+    v3 += part[3 * i + 2];
   }
   const double rz = block_sum<256>(v1, sh);
   const double bb = block_sum<256>(v2, sh);
+  const double rr = block_sum<256>(v3, sh);
   if (threadIdx.x == 0) {
     s->rzp[0] = rz;
     s->rzp[1] = 0.0;
     s->bb = bb;
-    s->rr = bb;
+    s->rr = rr;
     s->tol2 = tol2;
     s->maxit = maxit;
     s->iterp[0] = s->iterp[1] = 0;
     s->fresh = 1;
     s->breakdown = 0;
     s->alpha = s->beta = s->pq = 0.0;
-    s->converged = (bb == 0.0) ? 1 : 0;
+    // an initial guess may already meet the tolerance
+    s->converged = (bb == 0.0 || rr <= tol2 * bb) ? 1 : 0;
   }
 }
 
@@ -1538,7 +1549,7 @@ static int vec_blocks(int n) { return std::min(ceil_div(n, 256), kCgMaxBlocks); 
 constexpr int kCgFuseMax = 1024;
 size_t cg_part_size(int n) {
   n = std::max(n, 1);
-  return 2 * (size_t)std::max({ceil_div(n, 256), spmv_blocks(n, 64), kCgFuseMax}) + kCgFuseMax + 16;
+  return 3 * (size_t)std::max({ceil_div(n, 256), spmv_blocks(n, 64), kCgFuseMax}) + kCgFuseMax + 16;
 }
 // SpMV launches; otf != nullptr: the fused PCG's product with the new
 // direction formed on the fly (OtfArgs)
@@ -1661,9 +1672,9 @@ size_t cg_scal_bytes() { return sizeof(CGScal); }
 
 void cg_start(hipStream_t s, int n, const double* b, double* x, double* r, double* z, double* p, int bw,
               const double* binv, const int* bfirst, const unsigned char* bsize, double* part, double tol2,
-              int maxit, void* scal) {
+              int maxit, void* scal, const double* q) {
   const int nb = ceil_div(n, 256);
-  hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(256), 0, s, n, b, x, r, z, p, bw, binv, bfirst, bsize, part);
+  hipLaunchKernelGGL(k_cg_start, dim3(nb), dim3(256), 0, s, n, b, q, x, r, z, p, bw, binv, bfirst, bsize, part);
   hipLaunchKernelGGL(k_cg_init_scal, dim3(1), dim3(256), 0, s, nb, part, tol2, maxit, (CGScal*)scal);
 }
 

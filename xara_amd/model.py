@@ -270,14 +270,17 @@ class Model:
         or block row (0 = from the mean length)."""
         self._check(self._b.spmv_config(self._h, {"auto": 0, "csr": 1, "sym": 2}[fmt], int(lanes)))
 
-    def cg_config(self, chunk=16, graph=True, fuse=True, small=True):
+    def cg_config(self, chunk=16, graph=True, fuse=True, small=True, warm=True):
         """PCG issue pattern: iterations issued between convergence reads
         (even), whether a full chunk is replayed from a captured graph,
         whether small systems run the alpha / beta reductions inside the
-        vector kernels (the iterates do not depend on these three), and
-        whether systems that fit one workgroup are solved by it in one
-        launch (same iteration, sums in another order)."""
-        flags = int(bool(graph)) | 2 * int(bool(fuse)) | 4 * int(bool(small))
+        vector kernels (the iterates do not depend on these three), whether
+        systems that fit one workgroup are solved by it in one launch (same
+        iteration, sums in another order), and whether the first Newton
+        iteration of a step (and DisplacementControl's reference-load solve)
+        starts from the previous step's solution instead of zero (the same
+        tolerance, fewer iterations)."""
+        flags = int(bool(graph)) | 2 * int(bool(fuse)) | 4 * int(bool(small)) | 8 * int(bool(warm))
         self._check(self._b.cg_config(self._h, int(chunk), flags))
 
     def numberer(self, kind):
