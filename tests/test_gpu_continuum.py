@@ -400,6 +400,66 @@ def test_pcg_warm_start_same_trajectory_fewer_iterations(case):
         assert ia <= 1.03 * ib, (ia, ib)
 
 
+def test_morton_sweep_order_matches_equation_order(monkeypatch):
+    """XG_SYM_ORDER=1: the symmetric product's block rows stored and swept in
+    the Morton order of their nodes, the chunked PCG in that order (b and x
+    permuted at the ends of a solve, the block inverses once per matrix).
+    Against the equation order: the product through xg_spmv to 1e-13 (each
+    row sums the same entries in the same order, but of a pair of blocks the
+    order keeps the other one, and the assembled K_IJ and K_JI^T differ in the
+    last bits), a solve to 1e-9, the allocation-free solve bit for bit with the
+    ordinary one, and a warm-started LoadControl pushover to 1e-9."""
+    from xara_amd import _lib
+    print("case: morton sweep order cube 8^3", flush=True)
+    mesh = meshes.brick_cube(8)
+    models = []
+    for order in ("1", "0"):
+        monkeypatch.setenv("XG_SYM_ORDER", order)
+        m = Model(3, 3)
+        meshes.build_brick_cube(m, mesh, pressure=1.0)
+        m.system("GpuBlockPCG", 1e-12, 20000)
+        m.cg_config(16, True, True, small=False)
+        m.setup()
+        models.append(m)
+    monkeypatch.delenv("XG_SYM_ORDER")
+    a, b = models
+    rng = np.random.default_rng(11)
+    U = random_field(mesh, 1e-3, 12)
+    for m in models:
+        m.set_trial_disp(U)
+        m.update()
+        m.form_tangent(values=False)
+    x = rng.normal(size=a.neq)
+    for m in models:
+        m.spmv_config("sym")
+    ya, yb = a.spmv(x), b.spmv(x)
+    assert relerr(ya, yb) <= 1e-13
+    assert np.abs(ya).max() > 0
+    for m in models:
+        m.spmv_config("auto")
+    B = rng.normal(size=a.neq)
+    Xa, Xb = a.solve(B), b.solve(B)
+    assert relerr(Xa, Xb) <= 1e-9
+    lib = _lib.load()
+    X2 = np.zeros(a.neq)
+    assert lib.solve_noalloc(a._h, B.ctypes.data_as(_lib.P_D), X2.ctypes.data_as(_lib.P_D)) == 0, lib.error()
+    assert np.array_equal(Xa, X2)
+    traj = []
+    for m in models:
+        m.revert_to_start()
+        m.set_load_factor(0.0)
+        m.test("NormUnbalance", 1e-8, 30)
+        m.integrator("LoadControl", 25.0)
+        t = []
+        for _ in range(5):
+            assert m.analyze(1) == 0
+            t.append(m.trial_disp().copy())
+        traj.append(np.array(t))
+    assert relerr(traj[0], traj[1]) <= 1e-9
+    for m in models:
+        m.close()
+
+
 def test_revert_to_start_and_to_last_commit():
     """Domain::revertToStart after a plastic trial gives back the fresh model
     (its closing update at zero displacement forms the elastic tangent,

@@ -743,7 +743,7 @@ int xg_time_op(xg_domain* h, int op, int reps, double* ms) {
       } else if (op == XG_OP_UNBALANCE) {
         d.form_unbalance();
       } else if (op == XG_OP_SPMV) {
-        d.spmv(d.p.p, d.q.p, true);  // the PCG's product
+        d.spmv_sweep(d.p.p, d.q.p);  // the PCG's product, on vectors in its own order
       } else if (op == XG_OP_COPY) {
         xg::launch_copy(d.stream, ncopy, cx.p, cy.p);
       } else

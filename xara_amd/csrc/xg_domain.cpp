@@ -358,14 +358,49 @@ const SymSpmv* Domain::sym_prepare(bool solver) {
     sym_ok = false;
     const int M = nb_m, nbr = num.neq / M;
     const std::vector<int> nbc = node_block_columns();
+    if (nb_m != M || nbc.empty()) return nullptr;
+    // XG_SYM_ORDER=1 in the environment sweeps (and stores) the block rows in
+    // the Morton order of their nodes (every node of a block row here has all
+    // its DOFs free), the PCG's vectors in that order too: the partner reads
+    // of the stored blocks then mostly hit L2 (64^3 cube: 1.15x instead of
+    // 1.9x the algorithmic bytes fetched), but four consecutive block rows
+    // share fewer cache lines than in the equation order, and the product is
+    // bound by the L1's line lookups, not by HBM: 2-8% slower, so the
+    // equation order is the default
+    const char* ord_env = std::getenv("XG_SYM_ORDER");
+    const bool use_ord = ord_env && ord_env[0] == '1' && nbr > 1;
+    std::vector<int> ord;  // kept until the uploads have finished
+    if (use_ord) {
+      std::vector<double> xyz((size_t)nbr * ndm, 0.0);
+      for (int nd = 0; nd < nnode; nd++) {
+        const int e0 = num.eq[(size_t)nd * M];
+        if (e0 < 0) continue;
+        for (int d = 0; d < ndm; d++) xyz[(size_t)(e0 / M) * ndm + d] = crd_h[(size_t)nd * ndm + d];
+      }
+      ord = morton_order(nbr, ndm, xyz);
+    }
     SymLayout L;
-    if (nb_m != M || nbc.empty() || !sym_layout(M, nbr, num.rowptr, nbc, L)) return nullptr;
+    if (!sym_layout(M, nbr, num.rowptr, nbc, L, ord)) return nullptr;
+    if (use_ord) {
+      // columns as sweep positions: x is read in the sweep order
+      std::vector<int> pos(nbr);
+      for (int q = 0; q < nbr; q++) pos[ord[q]] = q;
+      for (int& c : L.col) c = pos[c];
+    }
     hipStream_t s = stream;
     sym_ptr.upload(L.ptr, s);
     sym_col.upload(L.col, s);
     sym_blk.upload(L.blk, s);
     sym_src.upload(L.src, s);
     sym_len.upload(L.len, s);
+    sym_sweep = use_ord;
+    if (use_ord) {
+      const int bwj = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
+      sym_ord.upload(ord, s);
+      sw_b.resize(num.neq);
+      sw_x.resize(num.neq);
+      binv_s.resize((size_t)num.neq * bwj);
+    }
     sym_val.resize((size_t)L.nup * M * M);
     XG_HIP(hipStreamSynchronize(s));
     sym_nup = L.nup;
@@ -394,8 +429,20 @@ const SymSpmv* Domain::sym_prepare(bool solver) {
 }
 
 void Domain::spmv(const double* x, double* y, bool solver) {
+  const SymSpmv* sym = sym_prepare(solver);
+  if (sym && sym_sweep) {
+    // y = K x in the equation order through the product's sweep order
+    launch_sweep_perm(stream, sym->nbr, sym->m, sym_ord.p, x, sw_b.p, false);
+    spmv_sweep(sw_b.p, sw_x.p);
+    launch_sweep_perm(stream, sym->nbr, sym->m, sym_ord.p, sw_x.p, y, true);
+    return;
+  }
+  launch_spmv(stream, lanes(), num.neq, rowptr.p, colind.p, vals.p, x, y, nullptr, nullptr, spmv_nbm(), nbcol.p, sym);
+}
+
+void Domain::spmv_sweep(const double* x, double* y) {
   launch_spmv(stream, lanes(), num.neq, rowptr.p, colind.p, vals.p, x, y, nullptr, nullptr, spmv_nbm(), nbcol.p,
-              sym_prepare(solver));
+              sym_prepare(true));
 }
 
 void Domain::setup() {
@@ -595,8 +642,9 @@ void Domain::setup() {
   sym_built = sym_ok = false;
   sym_stale = true;
   sym_view = SymSpmv{};
-  for (auto* v : {&sym_ptr, &sym_col, &sym_blk, &sym_src, &sym_len}) v->release();
-  sym_val.release();
+  sym_sweep = false;
+  for (auto* v : {&sym_ptr, &sym_col, &sym_blk, &sym_src, &sym_len, &sym_ord}) v->release();
+  for (auto* v : {&sym_val, &sw_b, &sw_x, &binv_s}) v->release();
   vals.resize(nnz);
   vals.zero(s);
   Ke.resize(ke_total);
@@ -772,7 +820,8 @@ void Domain::form_tangent() {
 }
 
 Domain::CgBufs Domain::cg_bufs() {
-  return CgBufs{r0.p, r1.p, z.p, p.p, p2.p, q.p, part.p, binv.p, pcbad.p, scal.p, band_A.p, band_L.p, band_Lr.p};
+  return CgBufs{r0.p, r1.p, z.p, p.p, p2.p, q.p, part.p, binv.p, pcbad.p, scal.p, band_A.p, band_L.p, band_Lr.p,
+                sw_b.p, sw_x.p, binv_s.p};
 }
 
 // The preconditioner of the current matrix into w: the band-Cholesky factor
@@ -780,7 +829,8 @@ Domain::CgBufs Domain::cg_bufs() {
 // inverses.  Formed once per matrix and buffer set.
 void Domain::precond_setup(const CgBufs& w) {
   const bool band = band_active();
-  const void* key = band ? (const void*)w.bandL : (const void*)w.binv;
+  const bool sweep = !band && !small_path() && sweep_active(w);
+  const void* key = band ? (const void*)w.bandL : sweep ? (const void*)w.binvs : (const void*)w.binv;
   if (pc_done == key) return;
   XG_HIP(hipMemsetAsync(w.pcbad, 0, sizeof(int), stream));
   if (band) {
@@ -789,6 +839,8 @@ void Domain::precond_setup(const CgBufs& w) {
   } else {
     const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
     launch_block_inverse(stream, nblk, blk_first.p, blk_size.p, rowptr.p, colind.p, vals.p, bw, w.binv, w.pcbad);
+    // a node's M rows of bw entries each, to the sweep order
+    if (sweep) launch_sweep_perm(stream, sym_view.nbr, sym_view.m * bw, sym_ord.p, w.binv, w.binvs, false);
   }
   pc_done = key;
 }
@@ -876,8 +928,10 @@ void Domain::cg_graph_clear() {
 void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x, const CgBufs& w, bool graphs) {
   const int n = num.neq, nbm = spmv_nbm();
   const SymSpmv* sym = sym_prepare(true);
+  // block inverses in the order the vectors are in
+  const double* binvp = sweep_active(w) ? w.binvs : w.binv;
   auto eager = [&](hipStream_t s, int k, int i0) {
-    cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, w.r0, w.r1, w.z, w.p, w.p2, w.q, bw, w.binv,
+    cg_iterations(s, k, i0, L, n, rowptr.p, colind.p, vals.p, x, w.r0, w.r1, w.z, w.p, w.p2, w.q, bw, binvp,
                   bfirst.p, bsize.p, w.part, w.scal, nbm, nbcol.p, cg_fuse_on, sym);
   };
   if (!graphs || (iters & 1) || (it0 & 1)) {
@@ -887,7 +941,7 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x, const Cg
   const std::vector<uintptr_t> key = {
       (uintptr_t)iters,   (uintptr_t)L,       (uintptr_t)n,       (uintptr_t)bw,      (uintptr_t)nbm, (uintptr_t)cg_fuse_on,
       (uintptr_t)x,       (uintptr_t)rowptr.p, (uintptr_t)colind.p, (uintptr_t)vals.p, (uintptr_t)w.r0,
-      (uintptr_t)w.r1,    (uintptr_t)w.z,     (uintptr_t)w.p,     (uintptr_t)w.p2,    (uintptr_t)w.q,     (uintptr_t)w.binv,
+      (uintptr_t)w.r1,    (uintptr_t)w.z,     (uintptr_t)w.p,     (uintptr_t)w.p2,    (uintptr_t)w.q,     (uintptr_t)binvp,
       (uintptr_t)bfirst.p, (uintptr_t)bsize.p, (uintptr_t)w.part, (uintptr_t)w.scal,  (uintptr_t)nbcol.p,
       (uintptr_t)(sym ? sym->val : nullptr), (uintptr_t)(sym ? sym->lanes : 0)};
   for (auto& g : cg_graphs)
@@ -960,16 +1014,27 @@ int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs,
     cg_solve_band(stream, n, rowptr.p, colind.p, vals.p, band_bw, band_pinv.p, w.bandL, w.bandLr, rtol * rtol, maxit,
                   R);
     cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);
-  } else if (cg_small_on && cg_small_fits(n, (long long)num.colind.size())) {
+  } else if (small_path()) {
     // the whole solve in one workgroup, one launch
     cg_solve_small(stream, n, rowptr.p, colind.p, vals.p, b, x, w.r0, w.z, w.p, w.q, bw, w.binv, bfirst.p, bsize.p,
                    rtol * rtol, maxit, w.scal);
     cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);
   } else {
+    // in the sweep order of the symmetric product: b (and a warm start's
+    // guess) gathered into it here, the solution scattered back at the end
+    const bool sweep = sweep_active(w);
+    const double* bs = b;
+    double* xs = x;
+    if (sweep) {
+      launch_sweep_perm(stream, sym_view.nbr, sym_view.m, sym_ord.p, b, w.bs, false);
+      if (warm) launch_sweep_perm(stream, sym_view.nbr, sym_view.m, sym_ord.p, x, w.xs, false);
+      bs = w.bs;
+      xs = w.xs;
+    }
     // a warm start takes the guess in x: q = A x first
-    if (warm) spmv(x, w.q, true);
-    cg_start(stream, n, b, x, w.r0, w.z, w.p, bw, w.binv, bfirst.p, bsize.p, w.part, rtol * rtol, maxit, w.scal,
-             warm ? w.q : nullptr);
+    if (warm) spmv_sweep(xs, w.q);
+    cg_start(stream, n, bs, xs, w.r0, w.z, w.p, bw, sweep ? w.binvs : w.binv, bfirst.p, bsize.p, w.part,
+             rtol * rtol, maxit, w.scal, warm ? w.q : nullptr);
     int issued = 0, next = cg_chunk;
     const int L = lanes();
     while (true) {
@@ -984,10 +1049,11 @@ int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs,
       // host round trips on long solves; iterations issued past
       // convergence return at once
       const int chunk = std::min(next, maxit + 1 - issued);
-      cg_chunk_run(chunk, issued, L, bw, x, w, graphs);
+      cg_chunk_run(chunk, issued, L, bw, xs, w, graphs);
       issued += chunk;
       next = std::min(2 * next, 8 * cg_chunk);
     }
+    if (sweep) launch_sweep_perm(stream, sym_view.nbr, sym_view.m, sym_ord.p, w.xs, x, true);
   }
   counters.solves++;
   counters.cg_iters += it;

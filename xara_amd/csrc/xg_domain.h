@@ -115,6 +115,13 @@ struct Domain {
   int sym_nup = 0;
   bool sym_built = false, sym_ok = false, sym_stale = true;
   SymSpmv sym_view;
+  // the copy's block rows in the Morton order of their nodes (sym_sweep):
+  // the chunked PCG then runs in that order -- b and a warm start's guess
+  // gathered into sw_b / sw_x, the block-Jacobi inverses into binv_s, the
+  // solution scattered back (sym_ord[position] = block row)
+  bool sym_sweep = false;
+  DVec<int> sym_ord;
+  DVec<double> sw_b, sw_x, binv_s;
   DVec<double> vals, Ke, Re, B, X, Peq;
   DVec<int> asm_ptr, asm_idx, rhs_ptr, rhs_idx;
   DVec<int> nfail, pcbad;
@@ -207,7 +214,10 @@ struct Domain {
     int* pcbad;
     void* scal;
     double *bandA, *bandL, *bandLr;  // band matrix and its Cholesky factor (band_active())
+    double *bs = nullptr, *xs = nullptr, *binvs = nullptr;  // b, x and binv in the sweep order (sym_sweep)
   };
+  // the symmetric product in its sweep order runs this solve (chunked PCG)
+  bool sweep_active(const CgBufs& w) { return sym_sweep && w.bs && w.xs && w.binvs && sym_prepare(true); }
   // band-Cholesky preconditioner of small systems (taken by the one-
   // workgroup solve, cg_small): the half bandwidth in the band order
   // (the numbering, or the nodes sorted along the model's longest extent
@@ -262,6 +272,11 @@ struct Domain {
   // y = K x (device pointers) in the configured layout (solver: as the PCG
   // forms it)
   void spmv(const double* x, double* y, bool solver = false);
+  // the PCG's product on vectors in its own order (the sweep order when
+  // sym_sweep, else the equation order)
+  void spmv_sweep(const double* x, double* y);
+  // the whole solve runs in one workgroup (k_pcg_small)
+  bool small_path() const { return cg_small_on && cg_small_fits(num.neq, (long long)num.colind.size()); }
 };
 
 // no-allocation solve (xg_noalloc.cpp): per-device scratch lookup at

@@ -55,18 +55,23 @@ void launch_newmark_predict(hipStream_t s, long long ndofs, const int* eqmap, co
 void launch_newmark_correct(hipStream_t s, long long ndofs, const int* eqmap, const double* dU, double c2, double c3,
                             double* U, double* V, double* A);
 int spmv_blocks(int n, int lanes);
-// Block-symmetric copy of a node-blocked matrix (k_spmv_sym_dot): the upper
-// m x m blocks only; block row I lists its neighbour blocks J ascending with
-// the stored block holding each, (I, J) for J >= I and (J, I) for J < I.
+// Block-symmetric copy of a node-blocked matrix (k_spmv_sym_dot), in the
+// order its block rows are swept (the equation order, or an order of the
+// nodes in space; the product's x and y are then in that order too): block
+// row I lists its neighbour blocks J with the stored block holding each,
+// (I, J) itself, or (J, I) to be read transposed (sign bit of blk set).
 struct SymSpmv {
   int m = 0;      // block size, 2 or 3; 0 = not in use
   int nbr = 0;    // block rows
   int lanes = 32; // lanes per block row
   const int* ptr = nullptr;     // nbr + 1
-  const int* col = nullptr;     // block column of each entry
-  const int* blk = nullptr;     // stored block of each entry
+  const int* col = nullptr;     // block column (sweep position) of each entry
+  const int* blk = nullptr;     // stored block of each entry, | 0x80000000 when transposed
   const double* val = nullptr;  // stored blocks, m*m row-major each
 };
+// Rows of B doubles to the sweep order of a symmetric layout (dst[p*B + t] =
+// src[ord[p]*B + t]) or back (dst[ord[p]*B + t] = src[p*B + t]).
+void launch_sweep_perm(hipStream_t s, int nbr, int B, const int* ord, const double* src, double* dst, bool back);
 // workgroups (= partial sums) of one SpMV launch
 int spmv_grid(int n, int lanes, const SymSpmv* sym);
 // nbm = 2, 3 or 6 with nbcol: node-blocked column indices (k_spmv_nb_dot);

@@ -56,13 +56,15 @@ size_t noalloc_scratch_bytes() { return kScratchBytes; }
 
 // Bytes of scratch a solve of d's system needs: B and X staging, the six
 // PCG vectors, the block-Jacobi inverses, partial sums, scalars, a flag,
-// and the band factor when the band solve takes the system.
+// the band factor when the band solve takes the system, and b, x and the
+// inverses in the sweep order when the symmetric product uses one.
 static size_t needed(const Domain& d, int bw) {
   const size_t n = (size_t)d.num.neq;
   auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
   const size_t band = d.band_active() ? 3 * al(n * (d.band_bw + 1) * sizeof(double)) : 0;
+  const size_t sweep = d.sym_sweep ? 2 * al(n * sizeof(double)) + al(n * bw * sizeof(double)) : 0;
   return 8 * al(n * sizeof(double)) + al(n * bw * sizeof(double)) + al(cg_part_size(d.num.neq) * sizeof(double)) +
-         al(cg_scal_bytes()) + al(sizeof(int)) + band;
+         al(cg_scal_bytes()) + al(sizeof(int)) + band + sweep;
 }
 
 int solve_noalloc(Domain& d, const double* B, double* X) {
@@ -101,6 +103,12 @@ int solve_noalloc(Domain& d, const double* B, double* X) {
   w.scal = c.take<char>(cg_scal_bytes());
   w.pcbad = c.take<int>(1);
   w.bandA = w.bandL = w.bandLr = nullptr;
+  if (d.sym_sweep) {
+    // b, x and the block inverses in the symmetric product's sweep order
+    w.bs = c.take<double>(n);
+    w.xs = c.take<double>(n);
+    w.binvs = c.take<double>((size_t)n * bw);
+  }
   if (d.band_active()) {
     w.bandA = c.take<double>((size_t)n * (d.band_bw + 1));
     w.bandL = c.take<double>((size_t)n * (d.band_bw + 1));

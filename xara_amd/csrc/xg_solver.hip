@@ -385,20 +385,27 @@ __global__ __launch_bounds__(256) void k_spmv_nb_dot(int n, const int* __restric
 // so a node-blocked matrix is kept a second time as its upper M x M blocks
 // only (block-major, row-major inside a block, copied from the CSR values
 // after each assembly): about half the bytes per product.  Block row I lists
-// its neighbours J ascending, each with the stored block that holds it:
-// (I, J) itself for J >= I, (J, I) for J < I, used transposed.  A stored
-// block is read twice per product, by its own row and by its partner's, a
-// few thousand block rows apart in an ascending sweep, so the second read
-// finds it in a cache.  Units of 256/G block rows are dealt out to the XCDs
-// in contiguous eighths (the hardware dispatches workgroup b to XCD b % 8)
-// and each workgroup takes a contiguous run of units, so that a block row's
-// lower partners, read shortly before by the same XCD, are more often found
-// in its L2 (4% faster at 64^3 than a plain grid-stride sweep).  G lanes per
-// block row, one neighbour block per lane and pass (its M*M values, column
-// and M entries of x loaded together), the M row sums reduced across the G
-// lanes.  (Staging a row's blocks in LDS with consecutive lanes reading
-// consecutive doubles measured 1.7x slower: the barrier between the copy and
-// the products and the LDS footprint cost more than the scattered loads.)
+// its neighbours J ascending, each with the stored block that holds it: its
+// own (I, J), or its partner's (J, I) used transposed (sign bit of blk).  The
+// block rows are swept in the layout's order (the equation order, or the
+// Morton order of the nodes, the vectors then in that order too).  A stored
+// block is read twice per product, by its own row and by its partner's, so
+// the second read may find it in a cache.  Units of 256/G block rows are
+// dealt out to the XCDs in contiguous eighths (the hardware dispatches
+// workgroup b to XCD b % 8), one unit per workgroup (up to kSymMaxBlocks
+// units, then contiguous runs): the workgroups resident on an XCD then sweep
+// one front of its rows, and a row's partners, read shortly before by the
+// same XCD, are more often found in its L2 (about 8% faster at 64^3 than
+// contiguous runs of units per workgroup, 12% than a grid-stride sweep).  G
+// lanes per block row, one neighbour block per lane and pass (its M*M values,
+// column and M entries of x loaded together), the M row sums reduced across
+// the G lanes.  The kernel is bound by the L1's address processing (TA and
+// TD 80-84% busy at 64^3, about 33 cache lines per load instruction: each
+// lane's 72-byte block arrives in five 16-byte pieces from lines of its own),
+// not by HBM.  (Tried and slower: staging a row's blocks in LDS with
+// consecutive lanes reading consecutive doubles, 1.7x; staging a wavefront's
+// own blocks in LDS by coalesced loads behind a barrier, 10%; two
+// neighbours per lane in flight at once, 5%.)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void xcd_units(int units, int& ua, int& ub) {
   const int nx = min(8, (int)gridDim.x);
@@ -444,13 +451,14 @@ __global__ __launch_bounds__(256) void k_spmv_sym_dot(int nbr, const int* __rest
       const int b = ptr[I], e = ptr[I + 1];
       for (int k = b + lane; k < e; k += G) {
         const int J = col[k];
-        const double* a = sv + (long long)blk[k] * MM;
+        const int bk = blk[k];
+        const double* a = sv + (long long)(bk & 0x7fffffff) * MM;
         double v[MM], xj[M];
 #pragma unroll
         for (int q = 0; q < MM; q++) v[q] = a[q];
 #pragma unroll
         for (int c = 0; c < M; c++) xj[c] = xr(M * J + c);
-        const bool lower = J < I;
+        const bool lower = bk < 0;  // the partner's stored block, transposed
 #pragma unroll
         for (int r = 0; r < M; r++)
 #pragma unroll
@@ -493,6 +501,21 @@ __global__ __launch_bounds__(256) void k_sym_convert(int nup, const int* __restr
       for (int c = 0; c < M; c++) v[r * M + c] = vals[s0 + (long long)r * L + c];
 #pragma unroll
     for (int q = 0; q < M * M; q++) sv[(long long)k * (M * M) + q] = v[q];
+  }
+}
+
+// Rows of B doubles between the equation order and the sweep order of the
+// symmetric product: dst[p*B + t] = src[ord[p]*B + t] (to the sweep order),
+// or dst[ord[p]*B + t] = src[p*B + t] (back).
+__global__ __launch_bounds__(256) void k_sweep_perm(int nbr, int B, const int* __restrict__ ord,
+                                                    const double* __restrict__ src, double* __restrict__ dst,
+                                                    int back) {
+  const long long total = (long long)nbr * B;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const long long p = i / B, t = i - p * B;
+    const long long o = (long long)ord[p] * B + t;
+    if (back) dst[o] = src[i];
+    else dst[i] = src[o];
   }
 }
 
@@ -1542,6 +1565,10 @@ void launch_add_diag(hipStream_t s, int n, const long long* pos, const double* m
 // SpMV, whose rows vary in length, balances better over more workgroups.
 constexpr int kCgMaxBlocks = 2048;
 constexpr int kSpmvMaxBlocks = 8192;
+// The symmetric product runs one unit of block rows per workgroup up to this
+// many units, so that the workgroups resident on an XCD sweep one front of
+// its rows (beyond it a workgroup takes a contiguous run of units).
+constexpr int kSymMaxBlocks = 65536;
 int spmv_blocks(int n, int lanes) { return std::min(ceil_div(n, 256 / lanes), kSpmvMaxBlocks); }
 static int vec_blocks(int n) { return std::min(ceil_div(n, 256), kCgMaxBlocks); }
 // the fused alpha / beta kernels: at most this many partials per reduction;
@@ -1549,7 +1576,10 @@ static int vec_blocks(int n) { return std::min(ceil_div(n, 256), kCgMaxBlocks); 
 constexpr int kCgFuseMax = 1024;
 size_t cg_part_size(int n) {
   n = std::max(n, 1);
-  return 3 * (size_t)std::max({ceil_div(n, 256), spmv_blocks(n, 64), kCgFuseMax}) + kCgFuseMax + 16;
+  // the symmetric product's workgroups: at most one per 8 equations (block
+  // rows of >= 2 equations, >= 4 block rows per workgroup)
+  const int sym = std::min(ceil_div(n, 8), kSymMaxBlocks);
+  return 3 * (size_t)std::max({ceil_div(n, 256), spmv_blocks(n, 64), sym, kCgFuseMax}) + kCgFuseMax + 16;
 }
 // SpMV launches; otf != nullptr: the fused PCG's product with the new
 // direction formed on the fly (OtfArgs)
@@ -1616,7 +1646,7 @@ static void launch_spmv_t(hipStream_t s, int lanes, int n, const int* rowptr, co
 }
 
 int spmv_grid(int n, int lanes, const SymSpmv* sym) {
-  if (sym && sym->m) return std::min(ceil_div(std::max(sym->nbr, 1), 256 / lanes), kSpmvMaxBlocks);
+  if (sym && sym->m) return std::min(ceil_div(std::max(sym->nbr, 1), 256 / lanes), kSymMaxBlocks);
   return spmv_blocks(n, lanes);
 }
 
@@ -1627,6 +1657,13 @@ void launch_sym_convert(hipStream_t s, const SymSpmv& S, int nup, const int* src
   if (S.m == 2) hipLaunchKernelGGL(k_sym_convert<2>, g, b, 0, s, nup, src, len, vals, sval);
   else if (S.m == 3) hipLaunchKernelGGL(k_sym_convert<3>, g, b, 0, s, nup, src, len, vals, sval);
   else throw Error(XG_ERR_ARG, "symmetric SpMV block size must be 2 or 3");
+}
+
+void launch_sweep_perm(hipStream_t s, int nbr, int B, const int* ord, const double* src, double* dst, bool back) {
+  if (nbr <= 0 || B <= 0) return;
+  const long long total = (long long)nbr * B;
+  const dim3 g((unsigned)std::min<long long>((total + 255) / 256, 8192)), b(256);
+  hipLaunchKernelGGL(k_sweep_perm, g, b, 0, s, nbr, B, ord, src, dst, back ? 1 : 0);
 }
 
 void launch_spmv(hipStream_t s, int lanes, int n, const int* rowptr, const int* colind, const double* vals,
