@@ -148,6 +148,8 @@ def main():
                     help="whole PCG solve in one workgroup for systems that fit it")
     ap.add_argument("--cg-warm", type=int, choices=[0, 1], default=1,
                     help="start a step's first solve from the previous step's solution (0 = from zero)")
+    ap.add_argument("--cg-wide", type=int, choices=[0, 1], default=0,
+                    help="exact tiled band-Cholesky preconditioner for banded systems above 1,024 equations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -236,7 +238,8 @@ def gpu_system(m, args, rtol, maxit):
     """The PCG system of a GPU model with the bench's issue pattern."""
     m.system(args.system, rtol, maxit)
     if m._b.prefix == "xg_":
-        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small), bool(args.cg_warm))
+        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small), bool(args.cg_warm),
+                    bool(args.cg_wide))
 
 
 def newton_pushover(m, n, steps, max_seconds=None, max_steps=None, log=True):
@@ -640,7 +643,8 @@ def ensemble_main(args):
         m = Model(2, 3)
         define(m, k)
         m.system("GpuBlockPCG", 1e-14, 100000)
-        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small), bool(args.cg_warm))
+        m.cg_config(16, bool(args.cg_graph), bool(args.cg_fuse), bool(args.cg_small), bool(args.cg_warm),
+                    bool(args.cg_wide))
         m.setup()
         for _ in range(args.warmup):
             m.analyze(1, dt)

@@ -194,6 +194,11 @@ int  xg_spmv_config(xg_domain* d, int format, int lanes);
                           DisplacementControl's reference-load solves from the
                           previous one (the chunked PCG; the default; the stop
                           test stays |r| <= rtol |b|)                          */
+#define XG_CG_WIDE  16 /* with XG_CG_SMALL: systems of more than 1,024
+                          equations whose half bandwidth in the band order is
+                          at most 1,024 are preconditioned by their exact band
+                          Cholesky factor, formed in 64 x 64 tiles (one or two
+                          PCG iterations per solve); off by default          */
 int  xg_cg_config(xg_domain* d, int chunk, int flags);
 /* `numberer RCM|Plain`                                                       */
 int  xg_numberer(xg_domain* d, int kind);

@@ -400,14 +400,15 @@ int xg_cg_config(xg_domain* h, int chunk, int flags) {
   return guard([&] {
     auto& d = D(h);
     if (chunk < 2 || chunk > 1024 || (chunk & 1)) throw xg::Error(XG_ERR_ARG, "cg chunk must be even, 2..1024");
-    if (flags & ~(XG_CG_GRAPH | XG_CG_FUSE | XG_CG_SMALL | XG_CG_WARM))
-      throw xg::Error(XG_ERR_ARG, "cg flags: XG_CG_GRAPH | XG_CG_FUSE | XG_CG_SMALL | XG_CG_WARM");
+    if (flags & ~(XG_CG_GRAPH | XG_CG_FUSE | XG_CG_SMALL | XG_CG_WARM | XG_CG_WIDE))
+      throw xg::Error(XG_ERR_ARG, "cg flags: XG_CG_GRAPH | XG_CG_FUSE | XG_CG_SMALL | XG_CG_WARM | XG_CG_WIDE");
     d.cg_graph_clear();
     d.cg_chunk = chunk;
     d.cg_graph_on = (flags & XG_CG_GRAPH) != 0;
     d.cg_fuse_on = (flags & XG_CG_FUSE) != 0;
     d.cg_small_on = (flags & XG_CG_SMALL) != 0;
     d.cg_warm_on = (flags & XG_CG_WARM) != 0;
+    d.cg_wide_on = (flags & XG_CG_WIDE) != 0;
     return 0;
   });
 }
@@ -714,7 +715,7 @@ int xg_get_counters(xg_domain* h, xg_counters* c) {
   return guard([&] {
     auto& d = D(h);
     *c = d.counters;
-    c->band_bw = d.band_active() ? d.band_bw : -1;
+    c->band_bw = d.band_active() ? d.band_bw : (d.wide_active() && !d.small_path()) ? d.wb_bw : -1;
     return 0;
   });
 }

@@ -821,7 +821,7 @@ void Domain::form_tangent() {
 
 Domain::CgBufs Domain::cg_bufs() {
   return CgBufs{r0.p, r1.p, z.p, p.p, p2.p, q.p, part.p, binv.p, pcbad.p, scal.p, band_A.p, band_L.p, band_Lr.p,
-                sw_b.p, sw_x.p, binv_s.p};
+                sw_b.p, sw_x.p, binv_s.p, true};
 }
 
 // The preconditioner of the current matrix into w: the band-Cholesky factor
@@ -829,11 +829,17 @@ Domain::CgBufs Domain::cg_bufs() {
 // inverses.  Formed once per matrix and buffer set.
 void Domain::precond_setup(const CgBufs& w) {
   const bool band = band_active();
-  const bool sweep = !band && !small_path() && sweep_active(w);
-  const void* key = band ? (const void*)w.bandL : sweep ? (const void*)w.binvs : (const void*)w.binv;
+  const bool wide = !band && !small_path() && w.wide && wide_active();
+  const bool sweep = !band && !wide && !small_path() && sweep_active(w);
+  const void* key = band   ? (const void*)w.bandL
+                    : wide ? (const void*)wb_L.p
+                    : sweep ? (const void*)w.binvs
+                            : (const void*)w.binv;
   if (pc_done == key) return;
   XG_HIP(hipMemsetAsync(w.pcbad, 0, sizeof(int), stream));
-  if (band) {
+  if (wide) {
+    wb_factor(w.pcbad);
+  } else if (band) {
     band_factor(stream, num.neq, band_bw, rowptr.p, colind.p, vals.p, band_pinv.p, w.bandA, w.bandL, w.bandLr,
                 w.pcbad);
   } else {
@@ -843,6 +849,121 @@ void Domain::precond_setup(const CgBufs& w) {
     if (sweep) launch_sweep_perm(stream, sym_view.nbr, sym_view.m * bw, sym_ord.p, w.binv, w.binvs, false);
   }
   pc_done = key;
+}
+
+// The wide-band factor of the current matrix; the launch sequence (one
+// per tile column) replayed from a captured graph when cg_graph_on.
+void Domain::wb_factor(int* bad) {
+  const int n = num.neq;
+  auto run = [&] {
+    wband_factor(stream, n, wb_bw, rowptr.p, colind.p, vals.p, band_pinv.p, wb_L.p, wb_D.p, bad);
+  };
+  if (!cg_graph_on) {
+    run();
+    return;
+  }
+  if (!wb_fe || bad != pcbad.p) {
+    destroy_graph(stream, wb_fg, wb_fe);
+    if (bad != pcbad.p) {  // not the domain's own flag: no graph kept
+      run();
+      return;
+    }
+    capture_graph(stream, run, &wb_fg, &wb_fe);
+  }
+  XG_HIP(hipGraphLaunch(wb_fe, stream));
+}
+
+// z = M^-1 r for k = 1 or 2 right-hand sides, M the wide-band factor.
+void Domain::wb_apply(int k, const double* const* r, double* const* z) {
+  const int n = num.neq;
+  const size_t np = (size_t)wband_padded(n);
+  double* Yb = wb_Y.p;
+  double* Yo = Yb + 2 * np;
+  double* Xo = Yo + 2 * np;
+  for (int s = 0; s < k; s++) wband_perm_in(stream, n, wb_bw, band_pinv.p, r[s], Yb + s * np);
+  auto run = [&] { wband_sweeps(stream, n, wb_bw, k, wb_L.p, wb_D.p, Yb, Yo, Xo); };
+  if (cg_graph_on) {
+    if (!wb_se[k - 1]) capture_graph(stream, run, &wb_sg[k - 1], &wb_se[k - 1]);
+    XG_HIP(hipGraphLaunch(wb_se[k - 1], stream));
+  } else {
+    run();
+  }
+  for (int s = 0; s < k; s++) wband_perm_out(stream, n, band_pinv.p, Xo + s * np, z[s]);
+}
+
+// PCG preconditioned by the exact wide-band factor, for R = 1 or 2
+// right-hand sides side by side (the preconditioner applied to both in one
+// set of sweeps): from x = 0, stopping at |r| <= rtol |b| like the other
+// paths.  The factor makes the first iterate the direct solution up to
+// rounding, so a solve takes one or two iterations.
+int Domain::solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w) {
+  const int n = num.neq;
+  const double tol2 = rtol * rtol;
+  double* vr[2] = {w.r0, wb_v.p};
+  double* vz[2] = {w.z, wb_v.p + n};
+  double* vp[2] = {w.p, wb_v.p + 2 * (size_t)n};
+  double* vq[2] = {w.q, wb_v.p + 3 * (size_t)n};
+  double bb[2] = {0, 0}, rz[2] = {0, 0}, rr[2] = {0, 0};
+  bool conv[2] = {true, true}, fail[2] = {false, false};
+  int its[2] = {0, 0};
+  for (int s = 0; s < R; s++) {
+    wband_zero(stream, n, x[s]);
+    launch_copy(stream, n, b[s], vr[s]);
+    bb[s] = dot_dev(b[s], b[s], n);
+    conv[s] = !(bb[s] > 0);
+  }
+  auto apply_open = [&] {
+    const double* rs[2];
+    double* zs[2];
+    int k = 0;
+    for (int s = 0; s < R; s++)
+      if (!conv[s]) {
+        rs[k] = vr[s];
+        zs[k] = vz[s];
+        k++;
+      }
+    if (k) wb_apply(k, rs, zs);
+  };
+  apply_open();
+  for (int s = 0; s < R; s++)
+    if (!conv[s]) {
+      rz[s] = dot_dev(vr[s], vz[s], n);
+      launch_copy(stream, n, vz[s], vp[s]);
+    }
+  for (int it = 0; it < maxit && !(conv[0] && conv[1]); it++) {
+    for (int s = 0; s < R; s++) {
+      if (conv[s]) continue;
+      its[s]++;
+      spmv(vp[s], vq[s], true);
+      const double pq = dot_dev(vp[s], vq[s], n);
+      if (pq == 0.0 || !std::isfinite(pq)) {
+        conv[s] = fail[s] = true;
+        continue;
+      }
+      const double alpha = rz[s] / pq;
+      launch_axpby(stream, n, alpha, vp[s], 1.0, x[s]);
+      launch_axpby(stream, n, -alpha, vq[s], 1.0, vr[s]);
+      rr[s] = dot_dev(vr[s], vr[s], n);
+      if (rr[s] <= tol2 * bb[s]) conv[s] = true;
+    }
+    if (conv[0] && conv[1]) break;
+    apply_open();
+    for (int s = 0; s < R; s++) {
+      if (conv[s]) continue;
+      const double rzn = dot_dev(vr[s], vz[s], n);
+      const double beta = rzn / rz[s];
+      rz[s] = rzn;
+      launch_axpby(stream, n, 1.0, vz[s], beta, vp[s]);
+    }
+  }
+  int status = 0;
+  for (int s = 0; s < R; s++) {
+    counters.solves++;
+    counters.cg_iters += its[s];
+    counters.last_cg_relres = bb[s] > 0 ? std::sqrt(rr[s] / bb[s]) : 0.0;
+    if (fail[s] || (bb[s] > 0 && !(rr[s] <= tol2 * bb[s]))) status = XG_BAD_LINEAR_SOLVE;
+  }
+  return status;
 }
 
 // Half bandwidth of the pattern under an equation order (pinv: position of
@@ -857,13 +978,15 @@ static int band_width(const Numbering& num, const std::vector<int>& pinv) {
 
 void Domain::band_order() {
   band_bw = -1;
+  wb_bw = -1;
   band_pinv.release();
   band_A.release();
   band2.release();
   band_L.release();
   band_Lr.release();
+  for (auto* v : {&wb_L, &wb_D, &wb_Y, &wb_v}) v->release();
   const int n = num.neq;
-  if (n == 0 || n > kBandMaxEqn) return;
+  if (n == 0 || n > kWbMaxEqn) return;
   std::vector<int> ident(n);
   for (int i = 0; i < n; i++) ident[i] = i;
   int best = band_width(num, ident);
@@ -901,7 +1024,21 @@ void Domain::band_order() {
       pinv = geo;
     }
   }
-  if (best > 63) return;
+  if (best > 63 || n > kBandMaxEqn) {
+    // the tiled factor of xg_wband.hip, for systems the one-workgroup
+    // solve does not take
+    if (n <= 1024 || best > kWbMaxBw) return;
+    const size_t np = (size_t)wband_padded(n);
+    const size_t need = (wband_factor_doubles(n, best) + wband_diag_doubles(n) + 6 * np + 4 * (size_t)n) * 8;
+    if (need > kWbMaxBytes) return;
+    wb_bw = best;
+    band_pinv.upload(pinv, stream);
+    wb_L.resize(wband_factor_doubles(n, best));
+    wb_D.resize(wband_diag_doubles(n));
+    wb_Y.resize(6 * np);
+    wb_v.resize(4 * (size_t)n);
+    return;
+  }
   band_bw = best;
   band_pinv.upload(pinv, stream);
   band_A.resize((size_t)n * (best + 1));
@@ -916,6 +1053,8 @@ void Domain::band_order() {
 void Domain::cg_graph_clear() {
   for (auto& g : cg_graphs) destroy_graph(stream, g.graph, g.exec);
   cg_graphs.clear();
+  destroy_graph(stream, wb_fg, wb_fe);
+  for (int k = 0; k < 2; k++) destroy_graph(stream, wb_sg[k], wb_se[k]);
 }
 
 // One chunk of PCG iterations.  A chunk of an even length starting on an
@@ -970,6 +1109,17 @@ void Domain::cg_chunk_run(int iters, int it0, int L, int bw, double* x, const Cg
 // results); otherwise one after the other.
 int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x2) {
   const int n = num.neq;
+  if (n > 0 && !small_path() && wide_active()) {
+    const CgBufs w = cg_bufs();
+    precond_setup(w);
+    int bad = 0;
+    XG_HIP(hipMemcpyAsync(&bad, w.pcbad, sizeof(int), hipMemcpyDeviceToHost, stream));
+    XG_HIP(hipStreamSynchronize(stream));
+    if (bad) return XG_BAD_LINEAR_SOLVE;
+    const double* bs[2] = {b1, b2};
+    double* xs[2] = {x1, x2};
+    return solve_wide(2, bs, xs, w);
+  }
   if (n == 0 || !band_active() || band2.n < (size_t)5 * n) {
     const int r = solve(b1, x1);
     return r < 0 ? r : solve(b2, x2);
@@ -1014,6 +1164,10 @@ int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs,
     cg_solve_band(stream, n, rowptr.p, colind.p, vals.p, band_bw, band_pinv.p, w.bandL, w.bandLr, rtol * rtol, maxit,
                   R);
     cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);
+  } else if (!small_path() && w.wide && wide_active()) {
+    const double* bs[1] = {b};
+    double* xs[1] = {x};
+    return solve_wide(1, bs, xs, w);
   } else if (small_path()) {
     // the whole solve in one workgroup, one launch
     cg_solve_small(stream, n, rowptr.p, colind.p, vals.p, b, x, w.r0, w.z, w.p, w.q, bw, w.binv, bfirst.p, bsize.p,
@@ -1230,7 +1384,7 @@ int Domain::newton() {
   bool first = true;
   do {
     form_tangent();
-    if (integ == 2 && band_active()) {
+    if (integ == 2 && (band_active() || (wide_active() && !small_path()))) {
       // DisplacementControl::update solves for the reference load with the
       // same matrix: both solves at once
       if (solve_pair(B.p, X.p, phat.p, dUhat.p) < 0) return XG_BAD_LINEAR_SOLVE;

@@ -129,6 +129,24 @@ bool cg_small_fits(int n, long long nnz);
 void cg_solve_small(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, const double* b,
                     double* x, double* r, double* z, double* p, double* q, int bw, const double* binv,
                     const int* bfirst, const unsigned char* bsize, double tol2, int maxit, void* scal);
+// Band Cholesky of systems whose half bandwidth (in the band order pinv) is
+// 1 .. kWbMaxBw, in 64 x 64 tiles (xg_wband.hip): L holds
+// wband_factor_doubles, D wband_diag_doubles; the sweeps work on
+// right-hand sides in the band order, padded to wband_padded(n) each (Yb
+// the input, overwritten; Yo and Xo two more such arrays), one or two at a
+// time.
+constexpr int kWbMaxBw = 1024;
+int wband_tiles(int bw);
+int wband_padded(int n);
+size_t wband_factor_doubles(int n, int bw);
+size_t wband_diag_doubles(int n);
+void wband_zero(hipStream_t s, long long n, double* v);
+void wband_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* colind, const double* vals,
+                  const int* pinv, double* L, double* D, int* bad);
+void wband_perm_in(hipStream_t s, int n, int bw, const int* pinv, const double* v, double* y);
+void wband_perm_out(hipStream_t s, int n, const int* pinv, const double* y, double* v);
+void wband_sweeps(hipStream_t s, int n, int bw, int nrhs, const double* L, const double* D, double* Yb, double* Yo,
+                  double* Xo);
 void cg_read_scal(hipStream_t s, const void* scal, int* converged, int* iter, double* rr, double* bb,
                   int* breakdown);
 

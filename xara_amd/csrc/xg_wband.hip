@@ -1,0 +1,425 @@
+// Band Cholesky of wide-banded systems (half bandwidth 64 .. kWbMaxBw), the
+// exact preconditioner of the PCG for models such as RC moment frames,
+// where block Jacobi needs thousands of iterations per solve.  The matrix,
+// in a band order of the equations (pinv), is held as 64 x 64 tiles: tile
+// column k (positions 64k .. 64k+63) keeps T + 1 tiles, tile (k, i) covering
+// rows 64(k+i) .. and the columns of k, row-major; T = ceil(bw / 64), so the
+// band (and the factor, which does not fill outside it) fits.  Positions
+// past n pad the last tile column with identity rows.
+//
+// The factorisation runs one launch per tile column (k_wb_step): the
+// column's workgroups apply the previous column's update to their tile,
+// factor the diagonal tile (each redundantly, in LDS) and solve their tile
+// against it; the remaining workgroups apply the previous column's update to
+// the next T - 1 tile columns, which no workgroup of this launch reads.  The
+// forward and backward sweeps run one launch per tile column as well
+// (k_wb_fwd, k_wb_bwd), for one or two right-hand sides.
+#include <algorithm>
+#include <utility>
+
+#include "../../include/xara_gpu.h"
+#include "xg_common.h"
+#include "xg_kernels.h"
+
+namespace xg {
+
+namespace {
+
+constexpr int TB = 64;           // tile size
+constexpr int TP = TB + 1;       // padded LDS row stride
+constexpr int TS = TB * TB;      // doubles per tile
+
+__global__ __launch_bounds__(256) void k_wb_fill(long long total, double* __restrict__ a) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < total; i += (long long)gridDim.x * 256) a[i] = 0.0;
+}
+
+// CSR lower triangle (in band positions) into the tiles; identity on the
+// padded positions.  An entry outside the tiles' reach sets bad.
+__global__ __launch_bounds__(256) void k_wb_gather(int n, int npad, int T, const int* __restrict__ rowptr,
+                                                   const int* __restrict__ colind, const double* __restrict__ vals,
+                                                   const int* __restrict__ pinv, double* __restrict__ L,
+                                                   int* __restrict__ bad) {
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= npad) return;
+  if (r >= n) {
+    const int k = r / TB;
+    L[((long long)k * (T + 1)) * TS + (r % TB) * TB + (r % TB)] = 1.0;
+    return;
+  }
+  const int pr = pinv[r];
+  for (int q = rowptr[r]; q < rowptr[r + 1]; q++) {
+    const int pc = pinv[colind[q]];
+    if (pc > pr) continue;
+    const int k = pc / TB, i = pr / TB - k;
+    if (i > T) {
+      atomicExch(bad, 2);
+      continue;
+    }
+    L[((long long)k * (T + 1) + i) * TS + (pr % TB) * TB + (pc % TB)] = vals[q];
+  }
+}
+
+// The factor and sweep kernels below run one wavefront per workgroup,
+// lane r holding row r (or column r) of a tile in registers (arrays indexed
+// by compile-time constants only, loops over them fully unrolled); values a
+// lane needs from the other rows come from LDS (broadcast reads) or from
+// readlane.  A tile row is loaded by each lane from its own 512 bytes, all
+// loads issued before their first use.
+__device__ __forceinline__ void row_load(double (&v)[TB], const double* __restrict__ g) {
+  const double2* g2 = reinterpret_cast<const double2*>(g);
+#pragma unroll
+  for (int j = 0; j < TB / 2; j++) {
+    const double2 d = g2[j];
+    v[2 * j] = d.x;
+    v[2 * j + 1] = d.y;
+  }
+}
+__device__ __forceinline__ void row_store(double* __restrict__ g, const double (&v)[TB]) {
+  double2* g2 = reinterpret_cast<double2*>(g);
+#pragma unroll
+  for (int j = 0; j < TB / 2; j++) g2[j] = make_double2(v[2 * j], v[2 * j + 1]);
+}
+__device__ __forceinline__ double lane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+// v -= X Y^T for the lane's row r, X and Y staged in LDS transposed
+// (xt[m*TB + j] = X[j][m], likewise yt): outer products over m, the lane's
+// X[r][m] times row m of yt read as broadcast pairs (a rolled loop: the
+// kernels keep their unrolled code for the factor and the solve).
+__device__ __forceinline__ void row_gemm_nt_sub(double (&v)[TB], const double* xt, const double* yt) {
+  const int r = threadIdx.x;
+#pragma unroll 1
+  for (int m = 0; m < TB; m++) {
+    const double xm = xt[m * TB + r];
+    const double2* y2 = reinterpret_cast<const double2*>(yt + m * TB);
+#pragma unroll
+    for (int j = 0; j < TB / 2; j++) {
+      const double2 y = y2[j];
+      v[2 * j] -= xm * y.x;
+      v[2 * j + 1] -= xm * y.y;
+    }
+  }
+}
+
+// stage tile Y (row-major, global) in LDS transposed: yt[m*TB + j] = Y[j][m]
+// (lane j loads its row, writes it down column j)
+__device__ __forceinline__ void stage_t(double* yt, const double* __restrict__ Y) {
+  const int r = threadIdx.x;
+  double y[TB];
+  row_load(y, Y + r * TB);
+#pragma unroll
+  for (int m = 0; m < TB; m++) yt[m * TB + r] = y[m];
+  __syncthreads();
+}
+
+// Cholesky of the diagonal tile, lane r holding row r in a (lower part
+// used); on return a[c] = L(r, c) for c <= r, rd = 1 / L(r, r) on lane r.
+// Column c is broadcast through col (LDS) once scaled.  One template
+// instance per pivot, so every register index is a compile-time constant
+// (a loop the compiler declined to unroll would move the row to scratch).
+template <int C>
+__device__ __forceinline__ void potrf_pivot(double (&a)[TB], double& rd, double* col, bool& nonpos) {
+  const int r = threadIdx.x;
+  double d = lane_d(a[C], C);
+  const bool ok = (d > 0.0) && isfinite(d);
+  nonpos |= !ok;
+  d = ok ? d : 1.0;
+  const double s = sqrt(d), inv = 1.0 / s;
+  const double l = (r == C) ? s : a[C] * inv;
+  a[C] = l;
+  if (r == C) rd = inv;
+  col[r] = l;
+  __syncthreads();
+  // pairs of the broadcast column, a few in flight at a time
+#pragma unroll
+  for (int j0 = (C + 1) & ~1; j0 < TB; j0 += 8) {
+#pragma unroll
+    for (int j = j0; j < j0 + 8 && j < TB; j += 2) {
+      const double2 cp = *reinterpret_cast<const double2*>(col + j);
+      if (j > C) a[j] -= l * cp.x;
+      a[j + 1] -= l * cp.y;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  __syncthreads();
+}
+template <int... Cs>
+__device__ __forceinline__ void potrf_all(std::integer_sequence<int, Cs...>, double (&a)[TB], double& rd, double* col,
+                                          bool& nonpos) {
+  (potrf_pivot<Cs>(a, rd, col, nonpos), ...);
+}
+// (a pivot that is not positive is taken as 1 and reported once, after the
+// last pivot: a store to global memory inside the chain would keep the row
+// in scratch)
+__device__ __forceinline__ void row_potrf(double (&a)[TB], double& rd, double* col, int* __restrict__ bad) {
+  rd = 1.0;
+  bool nonpos = false;
+  potrf_all(std::make_integer_sequence<int, TB>{}, a, rd, col, nonpos);
+  if (nonpos && threadIdx.x == 0) atomicExch(bad, 1);
+}
+
+// Column C of W := W L^-T for the lane's row: w[C] = (w[C] - sum_{j<C}
+// w[j] L(C, j)) / L(C, C), L's rows in lk (LDS, broadcast reads).
+template <int C>
+__device__ __forceinline__ void trsm_col(double (&w)[TB], const double* lk, const double* rds) {
+  double s = 0.0;
+#pragma unroll
+  for (int j0 = 0; j0 < C; j0 += 8) {
+#pragma unroll
+    for (int j = j0; j < j0 + 8 && j < C; j += 2) {
+      const double2 lp = *reinterpret_cast<const double2*>(lk + C * TB + j);
+      s += w[j] * lp.x;
+      if (j + 1 < C) s += w[j + 1] * lp.y;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  w[C] = (w[C] - s) * rds[C];
+}
+template <int... Cs>
+__device__ __forceinline__ void trsm_all(std::integer_sequence<int, Cs...>, double (&w)[TB], const double* lk,
+                                         const double* rds) {
+  (trsm_col<Cs>(w, lk, rds), ...);
+}
+
+// One tile column k of the factorisation (see the file comment).  Grid:
+// T + 1 column workgroups, then T (T - 1) / 2 update workgroups; one
+// wavefront each.  The diagonal tiles of L go to D column-major
+// (D[k][c*TB + r] = L(r, c)) for the sweeps.
+__global__ __launch_bounds__(64) void k_wb_step(int k, int nt, int T, double* __restrict__ L,
+                                                double* __restrict__ D, int* __restrict__ bad) {
+  __shared__ double yt[TS];   // transposed update operands
+  __shared__ double xt[TS];
+  __shared__ double lk[TS];   // rows of L_kk (row-major) for the solve
+  __shared__ double col[TB];
+  __shared__ double rds[TB];
+  const int b = blockIdx.x, r = threadIdx.x;
+  auto tile = [&](int cl, int off) { return L + ((long long)cl * (T + 1) + off) * TS; };
+  if (b <= T) {
+    const int i = b;
+    if (k + i >= nt) return;
+    const bool upd = k >= 1 && T >= 1;
+    double rd;
+    {
+      double a[TB];
+      row_load(a, tile(k, 0) + r * TB);
+      if (upd) {
+        // A_kk -= L(k, k-1) L(k, k-1)^T
+        stage_t(yt, tile(k - 1, 1));
+        row_gemm_nt_sub(a, yt, yt);
+      }
+      row_potrf(a, rd, col, bad);
+      if (i == 0) {
+        double* Dk = D + (long long)k * TS;
+#pragma unroll
+        for (int c = 0; c < TB; c++) Dk[c * TB + r] = (c <= r) ? a[c] : 0.0;
+        return;
+      }
+#pragma unroll
+      for (int c = 0; c < TB; c++) lk[r * TB + c] = (c <= r) ? a[c] : 0.0;
+      rds[r] = rd;
+    }
+    double w[TB];
+    row_load(w, tile(k, i) + r * TB);
+    if (upd && i + 1 <= T) {
+      // A(k+i, k) -= L(k+i, k-1) L(k, k-1)^T
+      stage_t(xt, tile(k - 1, i + 1));
+      row_gemm_nt_sub(w, xt, yt);
+    }
+    __syncthreads();
+    // W := W L_kk^-T
+    trsm_all(std::make_integer_sequence<int, TB>{}, w, lk, rds);
+    row_store(tile(k, i) + r * TB, w);
+    return;
+  }
+  // the previous column's update of tile columns k+1 .. k+T-1:
+  // tile (k-1+a, k-1+bb) -= L(k-1+a, k-1) L(k-1+bb, k-1)^T, 2 <= bb <= a <= T
+  if (k < 1) return;
+  int o = b - (T + 1), bb = 2;
+  while (bb <= T && o >= T - bb + 1) {
+    o -= T - bb + 1;
+    bb++;
+  }
+  if (bb > T) return;
+  const int a = bb + o;
+  if (k - 1 + a >= nt) return;
+  double* dst = tile(k - 1 + bb, a - bb);
+  double v[TB];
+  row_load(v, dst + r * TB);
+  stage_t(xt, tile(k - 1, a));
+  stage_t(yt, tile(k - 1, bb));
+  row_gemm_nt_sub(v, xt, yt);
+  row_store(dst + r * TB, v);
+}
+
+// Forward sweep, tile column k: y_k = L_kk^-1 b_k on every workgroup (lane r
+// holding row r of L_kk, a readlane chain over the columns), written to Yo
+// by workgroup 0; workgroup i >= 1: b_{k+i} -= L(k+i, k) y_k.  R right-hand
+// sides, each npad long.
+template <int R>
+__global__ __launch_bounds__(64) void k_wb_fwd(int k, int nt, int T, int npad, const double* __restrict__ L,
+                                               const double* __restrict__ D, double* __restrict__ Yb,
+                                               double* __restrict__ Yo) {
+  const int i = blockIdx.x, r = threadIdx.x;
+  if (k + i >= nt) return;
+  const double* Dk = D + (long long)k * TS;
+  double l[TB], t[TB];
+#pragma unroll
+  for (int c = 0; c < TB; c++) l[c] = Dk[c * TB + r];
+  if (i >= 1) row_load(t, L + ((long long)k * (T + 1) + i) * TS + r * TB);
+  double bv[R], y[R];
+#pragma unroll
+  for (int s = 0; s < R; s++) {
+    bv[s] = Yb[(long long)s * npad + k * TB + r];
+    y[s] = 0.0;
+  }
+  double dg = 1.0;
+#pragma unroll
+  for (int c = 0; c < TB; c++)
+    if (r == c) dg = l[c];
+  const double rdl = 1.0 / dg;
+#pragma unroll
+  for (int c = 0; c < TB; c++) {
+    const double rc = lane_d(rdl, c);
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const double yc = lane_d(bv[s], c) * rc;
+      if (r == c) y[s] = yc;
+      bv[s] -= l[c] * yc;
+    }
+  }
+  if (i == 0) {
+#pragma unroll
+    for (int s = 0; s < R; s++) Yo[(long long)s * npad + k * TB + r] = y[s];
+    return;
+  }
+#pragma unroll
+  for (int s = 0; s < R; s++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < TB; j++) acc += t[j] * lane_d(y[s], j);
+    Yb[(long long)s * npad + (k + i) * TB + r] -= acc;
+  }
+}
+
+// Backward sweep, tile column k (k from the last down): x_k = L_kk^-T z_k
+// on every workgroup (lane r holding column r of L_kk), written to Xo by
+// workgroup 0; workgroup j >= 1: z_{k-j} -= L(k, k-j)^T x_k, z kept in Yo.
+template <int R>
+__global__ __launch_bounds__(64) void k_wb_bwd(int k, int nt, int T, int npad, const double* __restrict__ L,
+                                               const double* __restrict__ D, double* __restrict__ Yo,
+                                               double* __restrict__ Xo) {
+  const int j = blockIdx.x, r = threadIdx.x;
+  if (k - j < 0) return;
+  const double* Dk = D + (long long)k * TS;
+  double l[TB], t[TB];
+  row_load(l, Dk + r * TB);  // column r of L_kk: l[c] = L(c, r)
+  if (j >= 1) {
+    // tile (k-j, j) holds L(k, k-j), row-major: lane r takes its column r
+    const double* Lt = L + ((long long)(k - j) * (T + 1) + j) * TS;
+#pragma unroll
+    for (int q = 0; q < TB; q++) t[q] = Lt[q * TB + r];
+  }
+  double zv[R], x[R];
+#pragma unroll
+  for (int s = 0; s < R; s++) {
+    zv[s] = Yo[(long long)s * npad + k * TB + r];
+    x[s] = 0.0;
+  }
+  double dg = 1.0;
+#pragma unroll
+  for (int c = 0; c < TB; c++)
+    if (r == c) dg = l[c];
+  const double rdl = 1.0 / dg;
+#pragma unroll
+  for (int c = TB - 1; c >= 0; c--) {
+    const double rc = lane_d(rdl, c);
+#pragma unroll
+    for (int s = 0; s < R; s++) {
+      const double xc = lane_d(zv[s], c) * rc;
+      if (r == c) x[s] = xc;
+      zv[s] -= l[c] * xc;
+    }
+  }
+  if (j == 0) {
+#pragma unroll
+    for (int s = 0; s < R; s++) Xo[(long long)s * npad + k * TB + r] = x[s];
+    return;
+  }
+#pragma unroll
+  for (int s = 0; s < R; s++) {
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < TB; q++) acc += t[q] * lane_d(x[s], q);
+    Yo[(long long)s * npad + (k - j) * TB + r] -= acc;
+  }
+}
+
+// to the band order (padding zeroed) and back
+__global__ __launch_bounds__(256) void k_wb_perm_in(int n, int npad, const int* __restrict__ pinv,
+                                                    const double* __restrict__ v, double* __restrict__ y) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p < n) y[pinv[p]] = v[p];
+  else if (p < npad) y[p] = 0.0;
+}
+__global__ __launch_bounds__(256) void k_wb_perm_out(int n, const int* __restrict__ pinv,
+                                                     const double* __restrict__ y, double* __restrict__ v) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p < n) v[p] = y[pinv[p]];
+}
+
+}  // namespace
+
+int wband_tiles(int bw) { return (bw + TB - 1) / TB; }
+int wband_padded(int n) { return ((n + TB - 1) / TB) * TB; }
+void wband_zero(hipStream_t s, long long n, double* v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_wb_fill, dim3((unsigned)std::min<long long>((n + 255) / 256, 8192)), dim3(256), 0, s, n, v);
+}
+size_t wband_factor_doubles(int n, int bw) {
+  const long long nt = (n + TB - 1) / TB;
+  return (size_t)(nt * (wband_tiles(bw) + 1) * TS);
+}
+size_t wband_diag_doubles(int n) { return (size_t)((n + TB - 1) / TB) * TS; }
+
+void wband_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* colind, const double* vals,
+                  const int* pinv, double* L, double* D, int* bad) {
+  if (n <= 0) return;
+  const int nt = (n + TB - 1) / TB, T = wband_tiles(bw), npad = nt * TB;
+  const long long total = (long long)nt * (T + 1) * TS;
+  hipLaunchKernelGGL(k_wb_fill, dim3((unsigned)std::min<long long>((total + 255) / 256, 8192)), dim3(256), 0, s,
+                     total, L);
+  hipLaunchKernelGGL(k_wb_gather, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, T, rowptr, colind, vals, pinv,
+                     L, bad);
+  const int grid = (T + 1) + T * (T - 1) / 2;
+  for (int k = 0; k < nt; k++)
+    hipLaunchKernelGGL(k_wb_step, dim3(grid), dim3(64), 0, s, k, nt, T, L, D, bad);
+}
+
+void wband_perm_in(hipStream_t s, int n, int bw, const int* pinv, const double* v, double* y) {
+  const int npad = ((n + TB - 1) / TB) * TB;
+  hipLaunchKernelGGL(k_wb_perm_in, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pinv, v, y);
+  (void)bw;
+}
+void wband_perm_out(hipStream_t s, int n, const int* pinv, const double* y, double* v) {
+  hipLaunchKernelGGL(k_wb_perm_out, dim3((n + 255) / 256), dim3(256), 0, s, n, pinv, y, v);
+}
+
+void wband_sweeps(hipStream_t s, int n, int bw, int nrhs, const double* L, const double* D, double* Yb, double* Yo,
+                  double* Xo) {
+  if (n <= 0) return;
+  const int nt = (n + TB - 1) / TB, T = wband_tiles(bw), npad = nt * TB;
+  if (nrhs != 1 && nrhs != 2) throw Error(XG_ERR_ARG, "band sweeps: one or two right-hand sides");
+  for (int k = 0; k < nt; k++) {
+    if (nrhs == 1) hipLaunchKernelGGL(k_wb_fwd<1>, dim3(T + 1), dim3(64), 0, s, k, nt, T, npad, L, D, Yb, Yo);
+    else hipLaunchKernelGGL(k_wb_fwd<2>, dim3(T + 1), dim3(64), 0, s, k, nt, T, npad, L, D, Yb, Yo);
+  }
+  for (int k = nt - 1; k >= 0; k--) {
+    if (nrhs == 1) hipLaunchKernelGGL(k_wb_bwd<1>, dim3(T + 1), dim3(64), 0, s, k, nt, T, npad, L, D, Yo, Xo);
+    else hipLaunchKernelGGL(k_wb_bwd<2>, dim3(T + 1), dim3(64), 0, s, k, nt, T, npad, L, D, Yo, Xo);
+  }
+}
+
+}  // namespace xg

@@ -270,7 +270,7 @@ class Model:
         or block row (0 = from the mean length)."""
         self._check(self._b.spmv_config(self._h, {"auto": 0, "csr": 1, "sym": 2}[fmt], int(lanes)))
 
-    def cg_config(self, chunk=16, graph=True, fuse=True, small=True, warm=True):
+    def cg_config(self, chunk=16, graph=True, fuse=True, small=True, warm=True, wide=False):
         """PCG issue pattern: iterations issued between convergence reads
         (even), whether a full chunk is replayed from a captured graph,
         whether small systems run the alpha / beta reductions inside the
@@ -279,8 +279,11 @@ class Model:
         iteration, sums in another order), and whether the first Newton
         iteration of a step (and DisplacementControl's reference-load solve)
         starts from the previous step's solution instead of zero (the same
-        tolerance, fewer iterations)."""
-        flags = int(bool(graph)) | 2 * int(bool(fuse)) | 4 * int(bool(small)) | 8 * int(bool(warm))
+        tolerance, fewer iterations), and whether systems of more than 1,024
+        equations with a band of at most 1,024 are preconditioned by their
+        exact tiled band factor (with small)."""
+        flags = (int(bool(graph)) | 2 * int(bool(fuse)) | 4 * int(bool(small)) | 8 * int(bool(warm))
+                 | 16 * int(bool(wide)))
         self._check(self._b.cg_config(self._h, int(chunk), flags))
 
     def numberer(self, kind):
