@@ -51,7 +51,7 @@ def test_wide_band_solve_matches_oracle(case):
     g, o = Model(ndm, ndf), _oracle.oracle_model(ndm, ndf)
     for m in (g, o):
         mesh = build(case, m)
-        m.system("GpuBlockPCG", 1e-12, 10000)
+        m.system("GpuBlockPCG", 1e-13, 10000)
         if m is g:
             m.cg_config(16, True, True, wide=True)
         m.setup()
@@ -70,7 +70,12 @@ def test_wide_band_solve_matches_oracle(case):
     Xg = g.solve(B)
     c = g.counters()
     Xo = o.solve(B)
-    assert relerr(Xg, Xo) <= RTOL
+    # backward error through the oracle's own product (the frame's stiffness
+    # range puts its rounding floor near 1e-12), and the difference from its
+    # direct solve (two backward-stable solves of the slender strip differ by
+    # its condition number times the rounding: about 1e-9)
+    assert np.linalg.norm(o.spmv(Xg) - B) <= 1e-10 * np.linalg.norm(B)
+    assert relerr(Xg, Xo) <= (RTOL if case == "frame3d" else 1e-8)
     bw = c["band_bw"]
     assert bw > (63 if case == "frame3d" else 0), bw
     assert g.neq > 16384 or bw > 63

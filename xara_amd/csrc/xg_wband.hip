@@ -85,153 +85,214 @@ __device__ __forceinline__ double lane_d(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
-// v -= X Y^T for the lane's row r, X and Y staged in LDS transposed
-// (xt[m*TB + j] = X[j][m], likewise yt): outer products over m, the lane's
-// X[r][m] times row m of yt read as broadcast pairs (a rolled loop: the
-// kernels keep their unrolled code for the factor and the solve).
-__device__ __forceinline__ void row_gemm_nt_sub(double (&v)[TB], const double* xt, const double* yt) {
-  const int r = threadIdx.x;
-#pragma unroll 1
+// The factor step runs four wavefronts per workgroup.  Thread (w, r) of
+// wavefront w holds row r, columns 16w .. 16w+15 of the tile being updated
+// (v16); operands are staged in LDS transposed (xt[m*TB + j] = X[j][m]) so
+// that the lane's X[r][m] and row m of Y^T are plain and broadcast reads.
+constexpr int WQ = TB / 4;  // columns per wavefront
+
+// v16 -= (X Y^T)[r][16w ..]: outer products over m (rolled loop)
+__device__ __forceinline__ void quarter_gemm_nt_sub(double (&v)[WQ], const double* xt, const double* yt) {
+  const int r = threadIdx.x % TB, w = threadIdx.x / TB;
+#pragma unroll 2
   for (int m = 0; m < TB; m++) {
     const double xm = xt[m * TB + r];
-    const double2* y2 = reinterpret_cast<const double2*>(yt + m * TB);
+    const double2* y2 = reinterpret_cast<const double2*>(yt + m * TB + w * WQ);
 #pragma unroll
-    for (int j = 0; j < TB / 2; j++) {
+    for (int j = 0; j < WQ / 2; j++) {
       const double2 y = y2[j];
       v[2 * j] -= xm * y.x;
       v[2 * j + 1] -= xm * y.y;
     }
   }
 }
-
-// stage tile Y (row-major, global) in LDS transposed: yt[m*TB + j] = Y[j][m]
-// (lane j loads its row, writes it down column j)
-__device__ __forceinline__ void stage_t(double* yt, const double* __restrict__ Y) {
-  const int r = threadIdx.x;
-  double y[TB];
-  row_load(y, Y + r * TB);
+__device__ __forceinline__ void quarter_load(double (&v)[WQ], const double* __restrict__ tile) {
+  const int r = threadIdx.x % TB, w = threadIdx.x / TB;
+  const double2* g2 = reinterpret_cast<const double2*>(tile + r * TB + w * WQ);
 #pragma unroll
-  for (int m = 0; m < TB; m++) yt[m * TB + r] = y[m];
-  __syncthreads();
+  for (int j = 0; j < WQ / 2; j++) {
+    const double2 d = g2[j];
+    v[2 * j] = d.x;
+    v[2 * j + 1] = d.y;
+  }
+}
+__device__ __forceinline__ void quarter_store(double* __restrict__ tile, const double (&v)[WQ]) {
+  const int r = threadIdx.x % TB, w = threadIdx.x / TB;
+  double2* g2 = reinterpret_cast<double2*>(tile + r * TB + w * WQ);
+#pragma unroll
+  for (int j = 0; j < WQ / 2; j++) g2[j] = make_double2(v[2 * j], v[2 * j + 1]);
+}
+// stage tile Y transposed: thread (w, r) loads Y[r][16w ..] and writes them
+// down column r of yt (no barrier here)
+__device__ __forceinline__ void stage_t(double* yt, const double* __restrict__ Y) {
+  const int r = threadIdx.x % TB, w = threadIdx.x / TB;
+  double y[WQ];
+  quarter_load(y, Y);
+#pragma unroll
+  for (int m = 0; m < WQ; m++) yt[(w * WQ + m) * TB + r] = y[m];
 }
 
-// Cholesky of the diagonal tile, lane r holding row r in a (lower part
-// used); on return a[c] = L(r, c) for c <= r, rd = 1 / L(r, r) on lane r.
-// Column c is broadcast through col (LDS) once scaled.  One template
-// instance per pivot, so every register index is a compile-time constant
-// (a loop the compiler declined to unroll would move the row to scratch).
+// 1/sqrt(a): the hardware estimate and two Newton steps (double precision
+// to a few ulp; the factor only preconditions the PCG)
+__device__ __forceinline__ double rsqrt_nr(double a) {
+  double y = __builtin_amdgcn_rsq(a);
+  const double h = 0.5 * a;
+  y = y * fma(-h, y * y, 1.5);
+  y = y * fma(-h, y * y, 1.5);
+  return y;
+}
+
+// Cholesky of the diagonal tile, every wavefront redundantly (lane r holding
+// row r in a, lower part used); on return a[c] = L(r, c) for c <= r and rd =
+// 1 / L(r, r) on lane r.  Scaled column c is broadcast through the
+// wavefront's own col buffer (two, alternating): LDS accesses of one
+// wavefront complete in order, so no barrier is needed, and the rank-1
+// update of the later columns overlaps the next pivots.  The next pivot's
+// diagonal is updated first, from the lane's own value.  One template
+// instance per pivot: every register index is a compile-time constant.
 template <int C>
 __device__ __forceinline__ void potrf_pivot(double (&a)[TB], double& rd, double* col, bool& nonpos) {
-  const int r = threadIdx.x;
+  const int r = threadIdx.x % TB;
   double d = lane_d(a[C], C);
   const bool ok = (d > 0.0) && isfinite(d);
   nonpos |= !ok;
   d = ok ? d : 1.0;
-  const double s = sqrt(d), inv = 1.0 / s;
+  const double inv = rsqrt_nr(d), s = d * inv;
   const double l = (r == C) ? s : a[C] * inv;
   a[C] = l;
   if (r == C) rd = inv;
-  col[r] = l;
-  __syncthreads();
-  // pairs of the broadcast column, a few in flight at a time
-#pragma unroll
-  for (int j0 = (C + 1) & ~1; j0 < TB; j0 += 8) {
-#pragma unroll
-    for (int j = j0; j < j0 + 8 && j < TB; j += 2) {
-      const double2 cp = *reinterpret_cast<const double2*>(col + j);
-      if (j > C) a[j] -= l * cp.x;
-      a[j + 1] -= l * cp.y;
-    }
-    __builtin_amdgcn_sched_barrier(0);
+  if constexpr (C + 1 < TB) {
+    if (r == C + 1) a[C + 1] -= l * l;
   }
-  __syncthreads();
+  double* cb = col + (C & 1) * TB;
+  cb[r] = l;
+  // orders the compiler only: the wavefront's LDS accesses run in order
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int j = (C + 1) & ~1; j < TB; j += 2) {
+    const double2 cp = *reinterpret_cast<const double2*>(cb + j);
+    // lane C+1 has updated its own a[C+1] already
+    if (j == C + 1) a[j] = (r == C + 1) ? a[j] : a[j] - l * cp.x;
+    else if (j > C) a[j] -= l * cp.x;
+    if (j + 1 == C + 1) a[j + 1] = (r == C + 1) ? a[j + 1] : a[j + 1] - l * cp.y;
+    else a[j + 1] -= l * cp.y;
+  }
+  // keep the scheduler from hoisting later pivots' reads over this one
+  // (their values would not fit in registers next to the row)
+  __builtin_amdgcn_sched_barrier(0);
 }
 template <int... Cs>
 __device__ __forceinline__ void potrf_all(std::integer_sequence<int, Cs...>, double (&a)[TB], double& rd, double* col,
                                           bool& nonpos) {
   (potrf_pivot<Cs>(a, rd, col, nonpos), ...);
 }
-// (a pivot that is not positive is taken as 1 and reported once, after the
-// last pivot: a store to global memory inside the chain would keep the row
-// in scratch)
-__device__ __forceinline__ void row_potrf(double (&a)[TB], double& rd, double* col, int* __restrict__ bad) {
-  rd = 1.0;
-  bool nonpos = false;
-  potrf_all(std::make_integer_sequence<int, TB>{}, a, rd, col, nonpos);
-  if (nonpos && threadIdx.x == 0) atomicExch(bad, 1);
-}
 
-// Column C of W := W L^-T for the lane's row: w[C] = (w[C] - sum_{j<C}
-// w[j] L(C, j)) / L(C, C), L's rows in lk (LDS, broadcast reads).
+// Column C of the right-looking solve W := W L^-T, the row's 64 entries over
+// a quad of lanes (lane q holding columns 4jj + q): the owner finishes
+// x_C = w_C / L(C, C), a quad broadcast hands it to the other three, and
+// every lane takes x_C L(j, C) off its later columns (lkt[C*TB + j] =
+// L(j, C)).
 template <int C>
-__device__ __forceinline__ void trsm_col(double (&w)[TB], const double* lk, const double* rds) {
-  double s = 0.0;
+__device__ __forceinline__ void trsm_col(double (&w)[WQ], const double* lkt, const double* rds, int q) {
+  constexpr int JJ = C / 4, QC = C % 4;
+  double x = w[JJ] * rds[C];
+  if (q == QC) w[JJ] = x;
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), QC * 0x55, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), QC * 0x55, 0xf, 0xf, false);
+  x = __hiloint2double(hi, lo);
 #pragma unroll
-  for (int j0 = 0; j0 < C; j0 += 8) {
-#pragma unroll
-    for (int j = j0; j < j0 + 8 && j < C; j += 2) {
-      const double2 lp = *reinterpret_cast<const double2*>(lk + C * TB + j);
-      s += w[j] * lp.x;
-      if (j + 1 < C) s += w[j + 1] * lp.y;
+  for (int jj = JJ; jj < WQ; jj++)
+    if (4 * jj + 3 > C) {
+      const double l = lkt[C * TB + 4 * jj + q];
+      if (4 * jj + q > C) w[jj] -= x * l;
     }
-    __builtin_amdgcn_sched_barrier(0);
-  }
-  w[C] = (w[C] - s) * rds[C];
 }
 template <int... Cs>
-__device__ __forceinline__ void trsm_all(std::integer_sequence<int, Cs...>, double (&w)[TB], const double* lk,
-                                         const double* rds) {
-  (trsm_col<Cs>(w, lk, rds), ...);
+__device__ __forceinline__ void trsm_all(std::integer_sequence<int, Cs...>, double (&w)[WQ], const double* lkt,
+                                         const double* rds, int q) {
+  (trsm_col<Cs>(w, lkt, rds, q), ...);
 }
 
 // One tile column k of the factorisation (see the file comment).  Grid:
-// T + 1 column workgroups, then T (T - 1) / 2 update workgroups; one
-// wavefront each.  The diagonal tiles of L go to D column-major
+// T + 1 column workgroups, then T (T - 1) / 2 update workgroups; 256
+// threads each.  The diagonal tiles of L go to D column-major
 // (D[k][c*TB + r] = L(r, c)) for the sweeps.
-__global__ __launch_bounds__(64) void k_wb_step(int k, int nt, int T, double* __restrict__ L,
-                                                double* __restrict__ D, int* __restrict__ bad) {
-  __shared__ double yt[TS];   // transposed update operands
+__global__ __launch_bounds__(256) void k_wb_step(int k, int nt, int T, double* __restrict__ L,
+                                                 double* __restrict__ D, int* __restrict__ bad) {
+  __shared__ double yt[TS];        // transposed update operands
   __shared__ double xt[TS];
-  __shared__ double lk[TS];   // rows of L_kk (row-major) for the solve
-  __shared__ double col[TB];
+  __shared__ double wt[TB * TP];   // the tile in work, row-major, padded
+  __shared__ double lkt[TS];       // L_kk transposed: lkt[c*TB + j] = L(j, c)
+  __shared__ double col[4][2 * TB];
   __shared__ double rds[TB];
-  const int b = blockIdx.x, r = threadIdx.x;
+  const int b = blockIdx.x, t = threadIdx.x, r = t % TB, w = t / TB;
   auto tile = [&](int cl, int off) { return L + ((long long)cl * (T + 1) + off) * TS; };
   if (b <= T) {
     const int i = b;
     if (k + i >= nt) return;
     const bool upd = k >= 1 && T >= 1;
-    double rd;
     {
-      double a[TB];
-      row_load(a, tile(k, 0) + r * TB);
+      double v[WQ];
+      quarter_load(v, tile(k, 0));
       if (upd) {
         // A_kk -= L(k, k-1) L(k, k-1)^T
         stage_t(yt, tile(k - 1, 1));
-        row_gemm_nt_sub(a, yt, yt);
+        __syncthreads();
+        quarter_gemm_nt_sub(v, yt, yt);
       }
-      row_potrf(a, rd, col, bad);
+#pragma unroll
+      for (int j = 0; j < WQ; j++) wt[r * TP + w * WQ + j] = v[j];
+    }
+    __syncthreads();
+    // wavefront 0 factors the diagonal tile (the others would only compete
+    // for the LDS broadcasts) while the others load their own tile; its rows
+    // leave wt before the others reuse it
+    double a[TB];
+    if (w == 0) {
+#pragma unroll
+      for (int c = 0; c < TB; c++) a[c] = wt[r * TP + c];
+    }
+    __syncthreads();
+    if (w == 0) {
+      double rd = 1.0;
+      bool nonpos = false;
+      potrf_all(std::make_integer_sequence<int, TB>{}, a, rd, col[0], nonpos);
+      if (nonpos && r == 0) atomicExch(bad, 1);
       if (i == 0) {
         double* Dk = D + (long long)k * TS;
 #pragma unroll
         for (int c = 0; c < TB; c++) Dk[c * TB + r] = (c <= r) ? a[c] : 0.0;
-        return;
+      } else {
+#pragma unroll
+        for (int c = 0; c < TB; c++) lkt[c * TB + r] = (c <= r) ? a[c] : 0.0;
+        rds[r] = rd;
+      }
+    }
+    if (i == 0) return;
+    {
+      double v[WQ];
+      quarter_load(v, tile(k, i));
+      if (upd && i + 1 <= T) {
+        // A(k+i, k) -= L(k+i, k-1) L(k, k-1)^T
+        stage_t(xt, tile(k - 1, i + 1));
+        __syncthreads();
+        quarter_gemm_nt_sub(v, xt, yt);
       }
 #pragma unroll
-      for (int c = 0; c < TB; c++) lk[r * TB + c] = (c <= r) ? a[c] : 0.0;
-      rds[r] = rd;
-    }
-    double w[TB];
-    row_load(w, tile(k, i) + r * TB);
-    if (upd && i + 1 <= T) {
-      // A(k+i, k) -= L(k+i, k-1) L(k, k-1)^T
-      stage_t(xt, tile(k - 1, i + 1));
-      row_gemm_nt_sub(w, xt, yt);
+      for (int j = 0; j < WQ; j++) wt[r * TP + w * WQ + j] = v[j];
     }
     __syncthreads();
-    // W := W L_kk^-T
-    trsm_all(std::make_integer_sequence<int, TB>{}, w, lk, rds);
-    row_store(tile(k, i) + r * TB, w);
+    // W := W L_kk^-T, row rr over the quad of lanes 4rr .. 4rr+3
+    const int rr = t / 4, q = t % 4;
+    double wv[WQ];
+#pragma unroll
+    for (int jj = 0; jj < WQ; jj++) wv[jj] = wt[rr * TP + 4 * jj + q];
+    trsm_all(std::make_integer_sequence<int, TB>{}, wv, lkt, rds, q);
+    double* dst = tile(k, i) + rr * TB;
+#pragma unroll
+    for (int jj = 0; jj < WQ; jj++) dst[4 * jj + q] = wv[jj];
     return;
   }
   // the previous column's update of tile columns k+1 .. k+T-1:
@@ -246,12 +307,13 @@ __global__ __launch_bounds__(64) void k_wb_step(int k, int nt, int T, double* __
   const int a = bb + o;
   if (k - 1 + a >= nt) return;
   double* dst = tile(k - 1 + bb, a - bb);
-  double v[TB];
-  row_load(v, dst + r * TB);
+  double v[WQ];
+  quarter_load(v, dst);
   stage_t(xt, tile(k - 1, a));
   stage_t(yt, tile(k - 1, bb));
-  row_gemm_nt_sub(v, xt, yt);
-  row_store(dst + r * TB, v);
+  __syncthreads();
+  quarter_gemm_nt_sub(v, xt, yt);
+  quarter_store(dst, v);
 }
 
 // Forward sweep, tile column k: y_k = L_kk^-1 b_k on every workgroup (lane r
@@ -395,7 +457,7 @@ void wband_factor(hipStream_t s, int n, int bw, const int* rowptr, const int* co
                      L, bad);
   const int grid = (T + 1) + T * (T - 1) / 2;
   for (int k = 0; k < nt; k++)
-    hipLaunchKernelGGL(k_wb_step, dim3(grid), dim3(64), 0, s, k, nt, T, L, D, bad);
+    hipLaunchKernelGGL(k_wb_step, dim3(grid), dim3(256), 0, s, k, nt, T, L, D, bad);
 }
 
 void wband_perm_in(hipStream_t s, int n, int bw, const int* pinv, const double* v, double* y) {
