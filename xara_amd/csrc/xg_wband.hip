@@ -74,11 +74,6 @@ __device__ __forceinline__ void row_load(double (&v)[TB], const double* __restri
     v[2 * j + 1] = d.y;
   }
 }
-__device__ __forceinline__ void row_store(double* __restrict__ g, const double (&v)[TB]) {
-  double2* g2 = reinterpret_cast<double2*>(g);
-#pragma unroll
-  for (int j = 0; j < TB / 2; j++) g2[j] = make_double2(v[2 * j], v[2 * j + 1]);
-}
 __device__ __forceinline__ double lane_d(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
@@ -91,20 +86,52 @@ __device__ __forceinline__ double lane_d(double v, int l) {
 // that the lane's X[r][m] and row m of Y^T are plain and broadcast reads.
 constexpr int WQ = TB / 4;  // columns per wavefront
 
-// v16 -= (X Y^T)[r][16w ..]: outer products over m (rolled loop)
-__device__ __forceinline__ void quarter_gemm_nt_sub(double (&v)[WQ], const double* xt, const double* yt) {
-  const int r = threadIdx.x % TB, w = threadIdx.x / TB;
+// The tile GEMM W -= X Y^T: thread t forms the 4 x 4 block of rows
+// 4(t/16) .., columns 4(t%16) .. (v[a][b]), from X and Y staged in LDS
+// transposed: per m, four rows of column m of X and four of Y, two 16-byte
+// reads each, then 16 products (a rolled loop).
+__device__ __forceinline__ void block_gemm_nt_sub(double (&v)[4][4], const double* xt, const double* yt) {
+  const int r0 = 4 * (threadIdx.x / 16), c0 = 4 * (threadIdx.x % 16);
 #pragma unroll 2
   for (int m = 0; m < TB; m++) {
-    const double xm = xt[m * TB + r];
-    const double2* y2 = reinterpret_cast<const double2*>(yt + m * TB + w * WQ);
+    const double2 x01 = *reinterpret_cast<const double2*>(xt + m * TB + r0);
+    const double2 x23 = *reinterpret_cast<const double2*>(xt + m * TB + r0 + 2);
+    const double2 y01 = *reinterpret_cast<const double2*>(yt + m * TB + c0);
+    const double2 y23 = *reinterpret_cast<const double2*>(yt + m * TB + c0 + 2);
+    const double xv[4] = {x01.x, x01.y, x23.x, x23.y}, yv[4] = {y01.x, y01.y, y23.x, y23.y};
 #pragma unroll
-    for (int j = 0; j < WQ / 2; j++) {
-      const double2 y = y2[j];
-      v[2 * j] -= xm * y.x;
-      v[2 * j + 1] -= xm * y.y;
-    }
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) v[a][b] -= xv[a] * yv[b];
   }
+}
+__device__ __forceinline__ void block_load(double (&v)[4][4], const double* __restrict__ tile) {
+  const int r0 = 4 * (threadIdx.x / 16), c0 = 4 * (threadIdx.x % 16);
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const double2* g2 = reinterpret_cast<const double2*>(tile + (r0 + a) * TB + c0);
+    const double2 p0 = g2[0], p1 = g2[1];
+    v[a][0] = p0.x;
+    v[a][1] = p0.y;
+    v[a][2] = p1.x;
+    v[a][3] = p1.y;
+  }
+}
+__device__ __forceinline__ void block_store(double* __restrict__ tile, const double (&v)[4][4]) {
+  const int r0 = 4 * (threadIdx.x / 16), c0 = 4 * (threadIdx.x % 16);
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    double2* g2 = reinterpret_cast<double2*>(tile + (r0 + a) * TB + c0);
+    g2[0] = make_double2(v[a][0], v[a][1]);
+    g2[1] = make_double2(v[a][2], v[a][3]);
+  }
+}
+__device__ __forceinline__ void block_to_lds(double* wt, const double (&v)[4][4]) {
+  const int r0 = 4 * (threadIdx.x / 16), c0 = 4 * (threadIdx.x % 16);
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) wt[(r0 + a) * TP + c0 + b] = v[a][b];
 }
 __device__ __forceinline__ void quarter_load(double (&v)[WQ], const double* __restrict__ tile) {
   const int r = threadIdx.x % TB, w = threadIdx.x / TB;
@@ -115,12 +142,6 @@ __device__ __forceinline__ void quarter_load(double (&v)[WQ], const double* __re
     v[2 * j] = d.x;
     v[2 * j + 1] = d.y;
   }
-}
-__device__ __forceinline__ void quarter_store(double* __restrict__ tile, const double (&v)[WQ]) {
-  const int r = threadIdx.x % TB, w = threadIdx.x / TB;
-  double2* g2 = reinterpret_cast<double2*>(tile + r * TB + w * WQ);
-#pragma unroll
-  for (int j = 0; j < WQ / 2; j++) g2[j] = make_double2(v[2 * j], v[2 * j + 1]);
 }
 // stage tile Y transposed: thread (w, r) loads Y[r][16w ..] and writes them
 // down column r of yt (no barrier here)
@@ -234,16 +255,15 @@ __global__ __launch_bounds__(256) void k_wb_step(int k, int nt, int T, double* _
     if (k + i >= nt) return;
     const bool upd = k >= 1 && T >= 1;
     {
-      double v[WQ];
-      quarter_load(v, tile(k, 0));
+      double v[4][4];
+      block_load(v, tile(k, 0));
       if (upd) {
         // A_kk -= L(k, k-1) L(k, k-1)^T
         stage_t(yt, tile(k - 1, 1));
         __syncthreads();
-        quarter_gemm_nt_sub(v, yt, yt);
+        block_gemm_nt_sub(v, yt, yt);
       }
-#pragma unroll
-      for (int j = 0; j < WQ; j++) wt[r * TP + w * WQ + j] = v[j];
+      block_to_lds(wt, v);
     }
     __syncthreads();
     // wavefront 0 factors the diagonal tile (the others would only compete
@@ -272,16 +292,15 @@ __global__ __launch_bounds__(256) void k_wb_step(int k, int nt, int T, double* _
     }
     if (i == 0) return;
     {
-      double v[WQ];
-      quarter_load(v, tile(k, i));
+      double v[4][4];
+      block_load(v, tile(k, i));
       if (upd && i + 1 <= T) {
         // A(k+i, k) -= L(k+i, k-1) L(k, k-1)^T
         stage_t(xt, tile(k - 1, i + 1));
         __syncthreads();
-        quarter_gemm_nt_sub(v, xt, yt);
+        block_gemm_nt_sub(v, xt, yt);
       }
-#pragma unroll
-      for (int j = 0; j < WQ; j++) wt[r * TP + w * WQ + j] = v[j];
+      block_to_lds(wt, v);
     }
     __syncthreads();
     // W := W L_kk^-T, row rr over the quad of lanes 4rr .. 4rr+3
@@ -307,13 +326,13 @@ __global__ __launch_bounds__(256) void k_wb_step(int k, int nt, int T, double* _
   const int a = bb + o;
   if (k - 1 + a >= nt) return;
   double* dst = tile(k - 1 + bb, a - bb);
-  double v[WQ];
-  quarter_load(v, dst);
+  double v[4][4];
+  block_load(v, dst);
   stage_t(xt, tile(k - 1, a));
   stage_t(yt, tile(k - 1, bb));
   __syncthreads();
-  quarter_gemm_nt_sub(v, xt, yt);
-  quarter_store(dst, v);
+  block_gemm_nt_sub(v, xt, yt);
+  block_store(dst, v);
 }
 
 // Forward sweep, tile column k: y_k = L_kk^-1 b_k on every workgroup (lane r
