@@ -148,7 +148,7 @@ def main():
                     help="whole PCG solve in one workgroup for systems that fit it")
     ap.add_argument("--cg-warm", type=int, choices=[0, 1], default=1,
                     help="start a step's first solve from the previous step's solution (0 = from zero)")
-    ap.add_argument("--cg-wide", type=int, choices=[0, 1], default=0,
+    ap.add_argument("--cg-wide", type=int, choices=[0, 1], default=1,
                     help="exact tiled band-Cholesky preconditioner for banded systems above 1,024 equations")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -185,7 +185,7 @@ int  xg_spmv_config(xg_domain* d, int format, int lanes);
    the same bits with and without either.
    XG_CG_SMALL: systems of at most 1,024 equations and 32,768 nonzeros are
    solved whole by one workgroup in one launch (same iteration, its sums in
-   another order: the same result to rounding).  Default all three.         */
+   another order: the same result to rounding).  Default all of them.     */
 #define XG_CG_GRAPH 1
 #define XG_CG_FUSE  2
 #define XG_CG_SMALL 4
@@ -194,11 +194,12 @@ int  xg_spmv_config(xg_domain* d, int format, int lanes);
                           DisplacementControl's reference-load solves from the
                           previous one (the chunked PCG; the default; the stop
                           test stays |r| <= rtol |b|)                          */
-#define XG_CG_WIDE  16 /* with XG_CG_SMALL: systems of more than 1,024
+#define XG_CG_WIDE  16 /* with XG_CG_SMALL, under LoadControl and
+                          DisplacementControl: systems of more than 1,024
                           equations whose half bandwidth in the band order is
                           at most 1,024 are preconditioned by their exact band
                           Cholesky factor, formed in 64 x 64 tiles (one or two
-                          PCG iterations per solve); off by default          */
+                          PCG iterations per solve); on by default           */
 int  xg_cg_config(xg_domain* d, int chunk, int flags);
 /* `numberer RCM|Plain`                                                       */
 int  xg_numberer(xg_domain* d, int kind);
@@ -261,11 +262,13 @@ int  xg_solve(xg_domain* d, const double* B, double* X);
    that cannot allocate device memory (code around HIP graph capture, tight
    serving loops, harnesses that forbid hipMalloc).  Same result as
    xg_solve; every PCG work buffer (staging of B and X, the six iteration
-   vectors, the block-Jacobi inverses or the band factor, partial sums,
-   scalars) comes from a fixed 128 MiB device array per GPU, looked up at
-   xg_create, and the kernels are launched one by one (no graph).  A system whose buffers do
-   not fit (about 1.1 M equations with nodal 3x3 blocks) is XG_ERR_ARG.
-   Calls that overlap on one GPU share that array and race.                 */
+   vectors, the block-Jacobi inverses or the one-workgroup band factor,
+   partial sums, scalars) comes from a fixed 128 MiB device array per GPU,
+   looked up at xg_create; the tiled band factor (XG_CG_WIDE) uses the
+   domain's own buffers, allocated at xg_setup; the kernels are launched
+   one by one (no graph).  A system whose buffers do not fit (about 1.1 M
+   equations with nodal 3x3 blocks) is XG_ERR_ARG.  Calls that overlap on
+   one GPU share that array and race.                                       */
 int  xg_solve_noalloc(xg_domain* d, const double* B, double* X);
 /* y = A x with the assembled CSR values (for tests)                          */
 int  xg_spmv(xg_domain* d, const double* x, double* y);

@@ -858,23 +858,18 @@ void Domain::wb_factor(int* bad) {
   auto run = [&] {
     wband_factor(stream, n, wb_bw, rowptr.p, colind.p, vals.p, band_pinv.p, wb_L.p, wb_D.p, bad);
   };
-  if (!cg_graph_on) {
+  // plain launches when graphs are off or the flag is not the domain's own
+  // (the allocation-free solve: the captured graph is left as it is)
+  if (!cg_graph_on || bad != pcbad.p) {
     run();
     return;
   }
-  if (!wb_fe || bad != pcbad.p) {
-    destroy_graph(stream, wb_fg, wb_fe);
-    if (bad != pcbad.p) {  // not the domain's own flag: no graph kept
-      run();
-      return;
-    }
-    capture_graph(stream, run, &wb_fg, &wb_fe);
-  }
+  if (!wb_fe) capture_graph(stream, run, &wb_fg, &wb_fe);
   XG_HIP(hipGraphLaunch(wb_fe, stream));
 }
 
 // z = M^-1 r for k = 1 or 2 right-hand sides, M the wide-band factor.
-void Domain::wb_apply(int k, const double* const* r, double* const* z) {
+void Domain::wb_apply(int k, const double* const* r, double* const* z, bool graphs) {
   const int n = num.neq;
   const size_t np = (size_t)wband_padded(n);
   double* Yb = wb_Y.p;
@@ -882,7 +877,7 @@ void Domain::wb_apply(int k, const double* const* r, double* const* z) {
   double* Xo = Yo + 2 * np;
   for (int s = 0; s < k; s++) wband_perm_in(stream, n, wb_bw, band_pinv.p, r[s], Yb + s * np);
   auto run = [&] { wband_sweeps(stream, n, wb_bw, k, wb_L.p, wb_D.p, Yb, Yo, Xo); };
-  if (cg_graph_on) {
+  if (graphs) {
     if (!wb_se[k - 1]) capture_graph(stream, run, &wb_sg[k - 1], &wb_se[k - 1]);
     XG_HIP(hipGraphLaunch(wb_se[k - 1], stream));
   } else {
@@ -896,7 +891,7 @@ void Domain::wb_apply(int k, const double* const* r, double* const* z) {
 // set of sweeps): from x = 0, stopping at |r| <= rtol |b| like the other
 // paths.  The factor makes the first iterate the direct solution up to
 // rounding, so a solve takes one or two iterations.
-int Domain::solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w) {
+int Domain::solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w, bool graphs) {
   const int n = num.neq;
   const double tol2 = rtol * rtol;
   double* vr[2] = {w.r0, wb_v.p};
@@ -922,7 +917,7 @@ int Domain::solve_wide(int R, const double* const* b, double* const* x, const Cg
         zs[k] = vz[s];
         k++;
       }
-    if (k) wb_apply(k, rs, zs);
+    if (k) wb_apply(k, rs, zs, graphs);
   };
   apply_open();
   for (int s = 0; s < R; s++)
@@ -1118,7 +1113,7 @@ int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x
     if (bad) return XG_BAD_LINEAR_SOLVE;
     const double* bs[2] = {b1, b2};
     double* xs[2] = {x1, x2};
-    return solve_wide(2, bs, xs, w);
+    return solve_wide(2, bs, xs, w, cg_graph_on);
   }
   if (n == 0 || !band_active() || band2.n < (size_t)5 * n) {
     const int r = solve(b1, x1);
@@ -1167,7 +1162,7 @@ int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs,
   } else if (!small_path() && w.wide && wide_active()) {
     const double* bs[1] = {b};
     double* xs[1] = {x};
-    return solve_wide(1, bs, xs, w);
+    return solve_wide(1, bs, xs, w, graphs);
   } else if (small_path()) {
     // the whole solve in one workgroup, one launch
     cg_solve_small(stream, n, rowptr.p, colind.p, vals.p, b, x, w.r0, w.z, w.p, w.q, bw, w.binv, bfirst.p, bsize.p,

@@ -238,20 +238,23 @@ struct Domain {
   // more than 1,024 equations (the one-workgroup solve takes smaller ones)
   // whose half bandwidth in the band order is at most kWbMaxBw, or at most
   // 63 with more than kBandMaxEqn equations, and whose factor fits in
-  // kWbMaxBytes.  The PCG preconditioned by the exact factor converges in
-  // one or two iterations (solve_wide); factor and sweeps are replayed from
-  // captured graphs when cg_graph_on.
+  // kWbMaxBytes, under the static integrators (a transient's K + c3 M is
+  // well conditioned by the mass term: block Jacobi converges in tens of
+  // iterations there, faster than a factorisation per Newton iteration).
+  // The PCG preconditioned by the exact factor converges in one or two
+  // iterations (solve_wide); factor and sweeps are replayed from captured
+  // graphs when cg_graph_on.
   static constexpr int kWbMaxEqn = 1 << 18;
   static constexpr size_t kWbMaxBytes = size_t(1) << 30;
   int wb_bw = -1;
   DVec<double> wb_L, wb_D, wb_Y, wb_v;  // factor tiles, diagonal tiles, 3 x 2 sweep vectors, 4 PCG vectors
   hipGraph_t wb_fg = nullptr, wb_sg[2] = {nullptr, nullptr};
   hipGraphExec_t wb_fe = nullptr, wb_se[2] = {nullptr, nullptr};
-  bool cg_wide_on = false;
-  bool wide_active() const { return cg_small_on && cg_wide_on && wb_bw >= 0 && band_bw < 0; }
+  bool cg_wide_on = true;
+  bool wide_active() const { return cg_small_on && cg_wide_on && wb_bw >= 0 && band_bw < 0 && integ != 3; }
   void wb_factor(int* bad);
-  void wb_apply(int k, const double* const* r, double* const* z);
-  int solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w);
+  void wb_apply(int k, const double* const* r, double* const* z, bool graphs);
+  int solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w, bool graphs);
   // the preconditioner's buffers last formed from the current matrix
   // (nullptr: the matrix changed since), so repeated solves of one matrix
   // form it once

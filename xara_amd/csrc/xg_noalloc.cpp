@@ -103,6 +103,10 @@ int solve_noalloc(Domain& d, const double* B, double* X) {
   w.scal = c.take<char>(cg_scal_bytes());
   w.pcbad = c.take<int>(1);
   w.bandA = w.bandL = w.bandLr = nullptr;
+  // the tiled band factor, where the solve takes it, lives in the domain's
+  // own buffers (formed at set-up); here it is formed and applied with
+  // plain launches (instantiating a graph would allocate)
+  w.wide = true;
   if (d.sym_sweep) {
     // b, x and the block inverses in the symmetric product's sweep order
     w.bs = c.take<double>(n);

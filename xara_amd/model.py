@@ -270,7 +270,7 @@ class Model:
         or block row (0 = from the mean length)."""
         self._check(self._b.spmv_config(self._h, {"auto": 0, "csr": 1, "sym": 2}[fmt], int(lanes)))
 
-    def cg_config(self, chunk=16, graph=True, fuse=True, small=True, warm=True, wide=False):
+    def cg_config(self, chunk=16, graph=True, fuse=True, small=True, warm=True, wide=True):
         """PCG issue pattern: iterations issued between convergence reads
         (even), whether a full chunk is replayed from a captured graph,
         whether small systems run the alpha / beta reductions inside the
