@@ -77,6 +77,9 @@ typedef struct xg_counters {
   double    time_solve_ms;
   long long band_bw;          /* half bandwidth of the band-Cholesky
                                  preconditioner, -1 when it is not used    */
+  long long band_fallbacks;   /* matrices whose tiled band factor met a pivot
+                                 that is not positive and were solved with
+                                 block Jacobi instead                      */
 } xg_counters;
 
 /* ---- lifetime -------------------------------------------------------------- */

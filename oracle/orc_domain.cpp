@@ -1562,6 +1562,7 @@ int orc_element_resisting_force(xg_domain* d, int ele_tag, double* R, int* ndof)
 int orc_get_counters(xg_domain* d, xg_counters* c) {
   *c = d->counters;
   c->band_bw = -1;  // direct solve, no preconditioner
+  c->band_fallbacks = 0;
   return 0;
 }
 // Wall time of `reps` serial updates (the reference execution model,

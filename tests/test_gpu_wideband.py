@@ -4,8 +4,9 @@ the PCG preconditioned by the exact factor) against the CPU oracle's direct
 solve: single solves on a 3D RC frame (half bandwidth above 63, several
 tiles per column) and on a quad strip of more than 16,384 equations (one
 tile below the diagonal), a DisplacementControl pushover whose two solves
-per Newton iteration share the sweeps, and graph replay of the factor and
-sweep launches against the launches issued one by one."""
+per Newton iteration share the sweeps, graph replay of the factor and
+sweep launches against the launches issued one by one, and the fall-back
+to block Jacobi where a softened tangent is no longer positive definite."""
 import numpy as np
 import pytest
 
@@ -113,5 +114,37 @@ def test_wide_band_pushover_matches_oracle_and_graph_replay():
     c = g.counters()
     assert c["band_bw"] > 63
     assert c["cg_iters"] <= 3 * c["solves"]
+    for m in models:
+        m.close()
+
+
+def test_wide_band_falls_back_past_the_peak():
+    """A DisplacementControl pushover of the 5x5x8 frame through its peak
+    (0.25% roof drift per step): where the softened tangent is no longer
+    positive definite the band factor meets a pivot that is not positive and
+    that matrix is solved with block Jacobi instead (counted in
+    band_fallbacks); the trajectory stays with the oracle's direct solve to
+    1e-9 through and past the peak."""
+    print("case: wide band fallback frame3d 5x5x8", flush=True)
+    fr = meshes.rc_frame_3d(5, 5, 8)
+    models = []
+    for kind in ("gpu", "oracle"):
+        m = _oracle.oracle_model(3, 6) if kind == "oracle" else Model(3, 6)
+        meshes.build_rc_frame_3d(m, fr)
+        m.system("GpuBlockPCG", 1e-12, 20000)
+        m.test("NormUnbalance", 1e-3, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 0.25 * fr["height"] / 100.0)
+        m.setup()
+        models.append(m)
+    g, o = models
+    lam = []
+    for step in range(20):
+        for m in models:
+            assert m.analyze(1) == 0, step
+        assert relerr(g.trial_disp(), o.trial_disp()) <= RTOL, step
+        assert abs(g.load_factor - o.load_factor) <= RTOL * abs(o.load_factor)
+        lam.append(o.load_factor)
+    assert max(lam) > lam[-1]  # past the peak
+    assert g.counters()["band_fallbacks"] > 0
     for m in models:
         m.close()

@@ -30,6 +30,7 @@ class Counters(C.Structure):
         ("time_unbalance_ms", C.c_double),
         ("time_solve_ms", C.c_double),
         ("band_bw", C.c_longlong),
+        ("band_fallbacks", C.c_longlong),
     ]
 
 

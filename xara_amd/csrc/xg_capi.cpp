@@ -601,6 +601,7 @@ int xg_add_a(xg_domain* h, int nblk, int bs, const double* m, const int* id) {
     xg::launch_add_diag(d.stream, (int)pos.size(), dpos.p, dval.p, 1.0, d.vals.p);
     d.sym_stale = true;
     d.pc_done = nullptr;
+    d.wb_failed = false;
     XG_HIP(hipStreamSynchronize(d.stream));
     return 0;
   });
@@ -716,6 +717,7 @@ int xg_get_counters(xg_domain* h, xg_counters* c) {
     auto& d = D(h);
     *c = d.counters;
     c->band_bw = d.band_active() ? d.band_bw : (d.wide_active() && !d.small_path()) ? d.wb_bw : -1;
+    c->band_fallbacks = d.wb_fallbacks;
     return 0;
   });
 }

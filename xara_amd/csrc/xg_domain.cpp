@@ -687,6 +687,7 @@ void Domain::setup() {
   if (spmv_format != XG_SPMV_CSR) sym_prepare(true);
   band_order();
   pc_done = nullptr;
+  wb_failed = false;
   XG_HIP(hipStreamSynchronize(s));
   lambda = 0;
   time = 0;
@@ -815,6 +816,7 @@ void Domain::form_tangent() {
   }
   sym_stale = true;
   pc_done = nullptr;
+  wb_failed = false;
   // Newmark::formNodTangent: c2 C + c3 M of the nodal masses (C = alphaM M)
   if (integ == 3) launch_add_diag(stream, nmass_eq, mass_pos.p, mass_val.p, nm_c3 + nm_c2 * alphaM, vals.p);
 }
@@ -1106,14 +1108,15 @@ int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x
   const int n = num.neq;
   if (n > 0 && !small_path() && wide_active()) {
     const CgBufs w = cg_bufs();
-    precond_setup(w);
-    int bad = 0;
-    XG_HIP(hipMemcpyAsync(&bad, w.pcbad, sizeof(int), hipMemcpyDeviceToHost, stream));
-    XG_HIP(hipStreamSynchronize(stream));
-    if (bad) return XG_BAD_LINEAR_SOLVE;
-    const double* bs[2] = {b1, b2};
-    double* xs[2] = {x1, x2};
-    return solve_wide(2, bs, xs, w, cg_graph_on);
+    if (!precond_ready(w)) return XG_BAD_LINEAR_SOLVE;
+    if (wide_active()) {
+      const double* bs[2] = {b1, b2};
+      double* xs[2] = {x1, x2};
+      return solve_wide(2, bs, xs, w, cg_graph_on);
+    }
+    // the factor gave way to block Jacobi: one solve after the other
+    const int r = solve(b1, x1);
+    return r < 0 ? r : solve(b2, x2);
   }
   if (n == 0 || !band_active() || band2.n < (size_t)5 * n) {
     const int r = solve(b1, x1);
@@ -1142,15 +1145,28 @@ int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x
   return status;
 }
 
+// The preconditioner of the current matrix into w, checked: a band factor
+// that met a pivot that is not positive gives way to block Jacobi.
+bool Domain::precond_ready(const CgBufs& w) {
+  for (;;) {
+    const bool wide = w.wide && wide_active() && !small_path();
+    precond_setup(w);
+    int bad = 0;
+    XG_HIP(hipMemcpyAsync(&bad, w.pcbad, sizeof(int), hipMemcpyDeviceToHost, stream));
+    XG_HIP(hipStreamSynchronize(stream));
+    if (!bad) return true;
+    if (!wide) return false;
+    wb_failed = true;
+    wb_fallbacks++;
+    pc_done = nullptr;
+  }
+}
+
 int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs, bool warm) {
   const int n = num.neq;
   if (n == 0) return 0;
   const int bw = (system == XG_SYSTEM_PCG_BLOCKJACOBI) ? ndf : 1;
-  precond_setup(w);
-  int bad = 0;
-  XG_HIP(hipMemcpyAsync(&bad, w.pcbad, sizeof(int), hipMemcpyDeviceToHost, stream));
-  XG_HIP(hipStreamSynchronize(stream));
-  if (bad) return XG_BAD_LINEAR_SOLVE;
+  if (!precond_ready(w)) return XG_BAD_LINEAR_SOLVE;
   int conv = 0, it = 0, brk = 0;
   double rr = 0, bb = 0;
   if (band_active()) {

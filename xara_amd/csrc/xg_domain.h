@@ -251,7 +251,14 @@ struct Domain {
   hipGraph_t wb_fg = nullptr, wb_sg[2] = {nullptr, nullptr};
   hipGraphExec_t wb_fe = nullptr, wb_se[2] = {nullptr, nullptr};
   bool cg_wide_on = true;
-  bool wide_active() const { return cg_small_on && cg_wide_on && wb_bw >= 0 && band_bw < 0 && integ != 3; }
+  // the current matrix's band factor met a pivot that is not positive (an
+  // indefinite tangent past a peak): its solves fall back to block Jacobi
+  // until the next assembly
+  bool wb_failed = false;
+  long long wb_fallbacks = 0;
+  bool wide_active() const {
+    return cg_small_on && cg_wide_on && !wb_failed && wb_bw >= 0 && band_bw < 0 && integ != 3;
+  }
   void wb_factor(int* bad);
   void wb_apply(int k, const double* const* r, double* const* z, bool graphs);
   int solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w, bool graphs);
@@ -265,6 +272,7 @@ struct Domain {
   int solve_from(const double* b, double* x, bool warm) { return solve_with(b, x, cg_bufs(), cg_graph_on, warm); }
   // returns < 0 on failure; graphs: replay chunks from captured graphs
   int solve_with(const double* b, double* x, const CgBufs& w, bool graphs, bool warm = false);
+  bool precond_ready(const CgBufs& w);
   void incr_disp(const double* dU_dev, double scale = 1.0);
   void commit();
   void revert();
