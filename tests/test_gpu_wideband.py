@@ -148,3 +148,34 @@ def test_wide_band_falls_back_past_the_peak():
     assert g.counters()["band_fallbacks"] > 0
     for m in models:
         m.close()
+
+
+def test_narrow_band_falls_back_past_the_peak():
+    """The same fall-back for the one-workgroup band factor (half bandwidth
+    11): a 4-story, 2-bay 2D RC frame pushed by DisplacementControl through
+    its peak; the trajectory stays with the oracle to 1e-9."""
+    print("case: narrow band fallback frame2d 4x2", flush=True)
+    fr = meshes.rc_frame_2d(4, 2)
+    models = []
+    for kind in ("gpu", "oracle"):
+        m = _oracle.oracle_model(2, 3) if kind == "oracle" else Model(2, 3)
+        meshes.build_rc_frame_2d(m, fr)
+        P = np.zeros((len(fr["floor_nodes"]), 3))
+        P[:, 0] = fr["floor_level"] / fr["floor_level"].max()
+        m.loads(fr["floor_nodes"], P)
+        m.system("GpuBlockPCG", 1e-12, 100000)
+        m.test("NormUnbalance", 1e-5, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 25.0)
+        m.setup()
+        models.append(m)
+    g, o = models
+    lam = []
+    for step in range(30):
+        for m in models:
+            assert m.analyze(1) == 0, step
+        assert relerr(g.trial_disp(), o.trial_disp()) <= RTOL, step
+        lam.append(o.load_factor)
+    c = g.counters()
+    assert max(lam) > lam[-1] and c["band_fallbacks"] > 0, (max(lam), lam[-1], c["band_fallbacks"])
+    for m in models:
+        m.close()

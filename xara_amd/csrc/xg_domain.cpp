@@ -1123,7 +1123,11 @@ int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x
     return r < 0 ? r : solve(b2, x2);
   }
   const CgBufs w = cg_bufs();
-  precond_setup(w);
+  if (!precond_ready(w)) return XG_BAD_LINEAR_SOLVE;
+  if (!band_active()) {  // the factor gave way to the PCG without it
+    const int r = solve(b1, x1);
+    return r < 0 ? r : solve(b2, x2);
+  }
   const BandRhs R1{b1, x1, w.r0, w.z, w.p, w.q, w.r1, (CGScal*)w.scal};
   double* v = band2.p;
   const BandRhs R2{b2, x2, v, v + n, v + 2 * (size_t)n, v + 3 * (size_t)n, v + 4 * (size_t)n, (CGScal*)scal2.p};
@@ -1146,10 +1150,10 @@ int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x
 }
 
 // The preconditioner of the current matrix into w, checked: a band factor
-// that met a pivot that is not positive gives way to block Jacobi.
+// that met a pivot that is not positive gives way to the PCG without it.
 bool Domain::precond_ready(const CgBufs& w) {
   for (;;) {
-    const bool wide = w.wide && wide_active() && !small_path();
+    const bool wide = band_active() || (w.wide && wide_active() && !small_path());
     precond_setup(w);
     int bad = 0;
     XG_HIP(hipMemcpyAsync(&bad, w.pcbad, sizeof(int), hipMemcpyDeviceToHost, stream));

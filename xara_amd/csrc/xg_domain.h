@@ -232,7 +232,7 @@ struct Domain {
   DVec<char> scal2;
   bool dc_hat_ready = false;  // the Newton iteration has solved for the reference load
   int solve_pair(const double* b1, double* x1, const double* b2, double* x2);
-  bool band_active() const { return cg_small_on && band_bw >= 0; }
+  bool band_active() const { return cg_small_on && band_bw >= 0 && !wb_failed; }
   void band_order();
   // band-Cholesky preconditioner of wider bands (xg_wband.hip): systems of
   // more than 1,024 equations (the one-workgroup solve takes smaller ones)
@@ -251,9 +251,9 @@ struct Domain {
   hipGraph_t wb_fg = nullptr, wb_sg[2] = {nullptr, nullptr};
   hipGraphExec_t wb_fe = nullptr, wb_se[2] = {nullptr, nullptr};
   bool cg_wide_on = true;
-  // the current matrix's band factor met a pivot that is not positive (an
-  // indefinite tangent past a peak): its solves fall back to block Jacobi
-  // until the next assembly
+  // the current matrix's band factor (either kind) met a pivot that is not
+  // positive (an indefinite tangent past a peak): its solves fall back to
+  // the PCG without it until the next assembly
   bool wb_failed = false;
   long long wb_fallbacks = 0;
   bool wide_active() const {
