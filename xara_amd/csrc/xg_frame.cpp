@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "xg_frame_dev.h"
 
@@ -90,6 +91,7 @@ struct FrameGroup : Group {
   DVec<int> dconn, dsecdef, ddef_off, dfkind, dfmat, dflocal, drun_off, drun_beg, drun_kind, drun_mat, ddef_last;
   DVec<double> dgeo, dfy, dfz, dfA, dGJ, de, ds, dks, delasE;
   DVec<long long> dsb, dcb, deb;
+  DVec<int> drsec;
   DVec<Steel02Param> dsteelp;
   DVec<Concrete02Param> dconcp;
   DVec<double> st_c, st_t, cs_c, cs_t, el_c, el_t;
@@ -349,6 +351,20 @@ struct FrameGroup : Group {
     f.cs_t = cs_t.p;
     f.el_c = el_c.p;
     f.el_t = el_t.p;
+    // the displacement-based update's sections, grouped by definition, four
+    // to a wavefront (XG_FRAME_ROWS=0: one wavefront per section instead)
+    const char* rows_env = std::getenv("XG_FRAME_ROWS");
+    if (!force && nsec > 0 && !(rows_env && rows_env[0] == '0')) {
+      std::vector<std::vector<int>> by_def(GJ.size());
+      for (long long sec = 0; sec < nsec; sec++) by_def[secdef[sec / nip]].push_back((int)sec);
+      std::vector<int> rsec;
+      for (const auto& v : by_def)
+        for (size_t i = 0; i < v.size(); i += 4)
+          for (size_t k = 0; k < 4; k++) rsec.push_back(i + k < v.size() ? v[i + k] : -1);
+      drsec.upload(rsec, s);
+      f.rsec = drsec.p;
+      f.nrw = (long long)rsec.size() / 4;
+    }
     if (force) setup_force(d, def_off, fkind, fmat, fy, fz, fA, GJ, steelp, concp, elasE);
     stc_init = std::move(stc);
     csc_init = std::move(csc);

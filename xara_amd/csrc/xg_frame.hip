@@ -13,6 +13,15 @@ __global__ __launch_bounds__(256) void k_frame_update(FrameDev f, const double* 
   section_update<D3>(f, sec, lane, U);
 }
 
+// Four sections of one definition per wavefront (section_rows).
+template <bool D3>
+__global__ __launch_bounds__(256) void k_frame_update_rows(FrameDev f, const double* __restrict__ U) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long w = (long long)blockIdx.x * 4 + wave;
+  if (w >= f.nrw) return;
+  section_rows<D3>(f, w, lane, U);
+}
+
 template <bool D3>
 __global__ __launch_bounds__(256) void k_frame_revert(FrameDev f) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -47,6 +56,12 @@ __global__ void k_copy_i(long long n, const int* __restrict__ a, int* __restrict
 void launch_frame_update(hipStream_t s, const FrameDev& f, bool d3, const double* U, int* err) {
   (void)err;
   if (f.nsec <= 0) return;
+  if (f.rsec) {
+    const dim3 g(ceil_div(f.nrw, 4)), b(256);
+    if (d3) hipLaunchKernelGGL(k_frame_update_rows<true>, g, b, 0, s, f, U);
+    else hipLaunchKernelGGL(k_frame_update_rows<false>, g, b, 0, s, f, U);
+    return;
+  }
   const dim3 g(ceil_div(f.nsec, 4)), b(256);
   if (d3) hipLaunchKernelGGL(k_frame_update<true>, g, b, 0, s, f, U);
   else hipLaunchKernelGGL(k_frame_update<false>, g, b, 0, s, f, U);

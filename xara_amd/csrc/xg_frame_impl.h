@@ -359,6 +359,180 @@ __device__ __forceinline__ void section_update(const FrameDev& f, long long sec,
   section_fibers<D3, PrevCommitted>(f, sec, lane, d, sr, kt);
 }
 
+// Row totals of K per-lane values (K = 8 or 16) over each 16-lane row of the
+// wavefront, by the transposed butterfly of wave_sums confined to the row:
+// lane p of a row returns the row's total of value p / (16 / K).
+template <int K>
+__device__ __forceinline__ double row_sums(const double (&x)[K], int lane) {
+  constexpr int G = 16 / K;
+  typedef double vec_t __attribute__((ext_vector_type(K)));
+  vec_t v;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = x[k];
+#pragma unroll
+  for (int c = K; c > 1; c >>= 1) {
+    const int o = G * c / 2;  // 8, 4, ... inside the row
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < c / 2; i++) {
+      const double lo = v[i], hi = v[i + c / 2];
+      const double keep = up ? hi : lo;
+      const double send = up ? lo : hi;
+      v[i] = keep + __shfl_xor(send, o, 64);
+    }
+  }
+  double t = v[0];
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  return t;
+}
+
+// The displacement-based section update with four sections of one
+// definition per wavefront, a 16-lane row each (row = lane / 16 takes
+// section rsec[4w + row]): every row evaluates its own section deformation,
+// its lanes walk the section's fibers f = p, p + 16, ... of each run (Steel02
+// and elastic one fiber per lane and pass, Concrete02 two, the odd one
+// alone), and the row totals come from one butterfly over the row.  Against
+// one wavefront per section, the 20 Steel02 fibers of an RC section take
+// half a pass instead of a whole one (20 of 64 lanes busy) and the section
+// sums a quarter of the exchanges.  Same fiber arithmetic as section_fibers
+// (FiberSection2d/3d::setTrialSectionDeformation), other summation order.
+template <bool D3>
+__device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int lane, const double* __restrict__ U) {
+  const int row = lane >> 4, pos = lane & 15;
+  const int sec0 = uniform_i(f.rsec[4 * w]);
+  const int secr = f.rsec[4 * w + row];
+  const bool live = secr >= 0;
+  const long long sec = live ? secr : sec0;
+  const int def = uniform_i(f.secdef[sec0 / f.nip]);
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+  section_deformation<D3>(f, sec, U, d);
+  const long long sb = f.steel_base[sec], cb = f.conc_base[sec], eb = f.elas_base[sec];
+  SecSums S;
+  const int r0 = uniform_i(f.run_off[def]), r1 = uniform_i(f.run_off[def + 1]);
+  for (int r = r0; r < r1; r++) {
+    const int kind = uniform_i(f.run_kind[r]), mat = uniform_i(f.run_mat[r]);
+    const int b0 = uniform_i(f.run_beg[r]), b1 = uniform_i(f.run_beg[r + 1]);
+    const int e1 = live ? b1 : b0;  // an empty row runs no fibers
+    if (kind == 1) {
+      const Steel02Param* q = f.steelp + mat;
+      const Steel02Param p{uniform_d(q->Fy), uniform_d(q->E0),  uniform_d(q->b),  uniform_d(q->R0),
+                           uniform_d(q->cR1), uniform_d(q->cR2), uniform_d(q->a1), uniform_d(q->a2),
+                           uniform_d(q->a3), uniform_d(q->a4),  uniform_d(q->sigini)};
+      for (int fi = b0 + pos; fi < e1; fi += 16) {
+        const double y = f.fy[fi], z = D3 ? f.fz[fi] : 0.0;
+        const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
+        double stress, tangent;
+        steel_fiber(f, p, sb + f.flocal[fi], strain, stress, tangent);
+        S.add<D3>(y, z, f.fA[fi], stress, tangent);
+      }
+    } else if (kind == 2) {
+      const Concrete02Param* q = f.concp + mat;
+      const Concrete02Param p{uniform_d(q->fc),    uniform_d(q->epsc0), uniform_d(q->fcu), uniform_d(q->epscu),
+                              uniform_d(q->rat),   uniform_d(q->ft),    uniform_d(q->Ets)};
+      const long long n = f.nconc;
+      int fi = b0 + pos;
+      for (; fi + 16 < e1; fi += 32) {
+        const int fj = fi + 16;
+        const long long ia = cb + f.flocal[fi], ib = cb + f.flocal[fj];
+        const double ya = f.fy[fi], za = D3 ? f.fz[fi] : 0.0;
+        const double yb = f.fy[fj], zb = D3 ? f.fz[fj] : 0.0;
+        double ca[4], cb2[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          ca[k] = f.cs_c[k * n + ia];
+          cb2[k] = f.cs_c[k * n + ib];
+        }
+        double sa = f.cs_t[3 * n + ia], ea = f.cs_t[4 * n + ia];
+        double sb2 = f.cs_t[3 * n + ib], eb2 = f.cs_t[4 * n + ib];
+        const double Aa = f.fA[fi], Ab = f.fA[fj];
+        const double ea_ = D3 ? d[0] - ya * d[1] + za * d[2] : d[0] - ya * d[1];
+        const double eb_ = D3 ? d[0] - yb * d[1] + zb * d[2] : d[0] - yb * d[1];
+        double m1, d1, e1c, m2, d2, e2c;
+        concrete02_trial(p, ca[0], ca[1], ca[2], ca[3], ea_, m1, d1, e1c, sa, ea);
+        concrete02_trial(p, cb2[0], cb2[1], cb2[2], cb2[3], eb_, m2, d2, e2c, sb2, eb2);
+        __builtin_nontemporal_store(m1, &f.cs_t[0 * n + ia]);
+        __builtin_nontemporal_store(d1, &f.cs_t[1 * n + ia]);
+        __builtin_nontemporal_store(e1c, &f.cs_t[2 * n + ia]);
+        __builtin_nontemporal_store(sa, &f.cs_t[3 * n + ia]);
+        __builtin_nontemporal_store(ea, &f.cs_t[4 * n + ia]);
+        S.add<D3>(ya, za, Aa, sa, ea);
+        __builtin_nontemporal_store(m2, &f.cs_t[0 * n + ib]);
+        __builtin_nontemporal_store(d2, &f.cs_t[1 * n + ib]);
+        __builtin_nontemporal_store(e2c, &f.cs_t[2 * n + ib]);
+        __builtin_nontemporal_store(sb2, &f.cs_t[3 * n + ib]);
+        __builtin_nontemporal_store(eb2, &f.cs_t[4 * n + ib]);
+        S.add<D3>(yb, zb, Ab, sb2, eb2);
+      }
+      if (fi < e1) {
+        const long long ia = cb + f.flocal[fi];
+        const double ya = f.fy[fi], za = D3 ? f.fz[fi] : 0.0;
+        double ca[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) ca[k] = f.cs_c[k * n + ia];
+        double sa = f.cs_t[3 * n + ia], ea = f.cs_t[4 * n + ia];
+        const double Aa = f.fA[fi];
+        const double ea_ = D3 ? d[0] - ya * d[1] + za * d[2] : d[0] - ya * d[1];
+        double m1, d1, e1c;
+        concrete02_trial(p, ca[0], ca[1], ca[2], ca[3], ea_, m1, d1, e1c, sa, ea);
+        __builtin_nontemporal_store(m1, &f.cs_t[0 * n + ia]);
+        __builtin_nontemporal_store(d1, &f.cs_t[1 * n + ia]);
+        __builtin_nontemporal_store(e1c, &f.cs_t[2 * n + ia]);
+        __builtin_nontemporal_store(sa, &f.cs_t[3 * n + ia]);
+        __builtin_nontemporal_store(ea, &f.cs_t[4 * n + ia]);
+        S.add<D3>(ya, za, Aa, sa, ea);
+      }
+    } else {
+      const double E = uniform_d(f.elasE[mat]);
+      for (int fi = b0 + pos; fi < e1; fi += 16) {
+        const double y = f.fy[fi], z = D3 ? f.fz[fi] : 0.0;
+        const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
+        f.el_t[eb + f.flocal[fi]] = strain;
+        S.add<D3>(y, z, f.fA[fi], E * strain, E);
+      }
+    }
+  }
+  const long long n = f.nsec;
+  const int base = lane & 48;
+  if (D3) {
+    // totals: lane p of the row holds value p of x; kData[p] (row-major 4x4)
+    // is value kSrc[p] (15: a zero), the resultant s0..s2 values 6..8
+    const double x[16] = {S.k0, S.k1, S.k2, S.k5, S.k6, S.k10, S.s0, S.s1, S.s2, 0, 0, 0, 0, 0, 0, 0};
+    const double t = row_sums<16>(x, lane);
+    constexpr unsigned long long kSrc = 0x0ull | (1ull << 4) | (2ull << 8) | (15ull << 12) | (1ull << 16) |
+                                        (3ull << 20) | (4ull << 24) | (15ull << 28) | (2ull << 32) | (4ull << 36) |
+                                        (5ull << 40) | (15ull << 44) | (15ull << 48) | (15ull << 52) | (15ull << 56) |
+                                        (15ull << 60);
+    const int src = (int)((kSrc >> (4 * pos)) & 15);
+    const double kv = __shfl(t, base | src, 64);
+    const double sv = __shfl(t, base | (6 + (pos & 3)), 64);
+    const double GJ = uniform_d(f.GJ[def]);
+    if (live) {
+      f.ks[pos * n + sec] = pos == 15 ? (GJ >= 0.0 ? GJ : 0.0) : kv;
+      if (pos < 4) {
+        const double dv = pos == 0 ? d[0] : pos == 1 ? d[1] : pos == 2 ? d[2] : d[3];
+        f.s[pos * n + sec] = pos < 3 ? sv : (GJ >= 0.0 ? GJ * d[3] : 0.0);
+        f.e[pos * n + sec] = dv;
+      }
+    }
+  } else {
+    // totals: lanes 2v, 2v+1 of the row hold value v of x; kData = {k0, k1,
+    // k1, k5}, the resultant s0, s1 values 3, 4
+    const double x[8] = {S.k0, S.k1, S.k5, S.s0, S.s1, 0, 0, 0};
+    const double t = row_sums<8>(x, lane);
+    const int src = pos == 0 ? 0 : (pos == 3 ? 4 : 2);
+    const double kv = __shfl(t, base | src, 64);
+    const double sv = __shfl(t, base | (6 + 2 * (pos & 1)), 64);
+    if (live) {
+      if (pos < 4) f.ks[pos * n + sec] = kv;
+      if (pos < 2) {
+        f.s[pos * n + sec] = sv;
+        f.e[pos * n + sec] = pos == 0 ? d[0] : d[1];
+      }
+    }
+  }
+}
+
 // revertToLastCommit of every fiber followed by the section recomputation of
 // FiberSection2d::revertToLastCommit (FiberSection2d.cpp:376-409: the
 // resultant keeps only the last fiber's contribution, `sData[0] = fs0`) /
