@@ -70,6 +70,64 @@ def test_frame3d_update_sections_and_assembly():
     assert relerr(g.element_resisting_force(tag), o.element_resisting_force(tag)) <= RTOL
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_frame_update_row_widths(dim, monkeypatch):
+    """The displacement-based update in its three layouts: one wavefront per
+    section, four sections per wavefront (16-lane rows, section totals by
+    butterfly) and three (20-lane rows, totals through LDS), on frames whose
+    section counts (130 and 50) leave rows empty in both: section strains,
+    resultants and tangents against the oracle to 1e-9 over three loading
+    steps, and the committed Steel02 / Concrete02 fiber states across the
+    layouts to 1e-12 (the same fiber formulas; the compiler may fuse their
+    multiply-adds differently in each kernel, and the section sums' order
+    differs)."""
+    print(f"case: frame{dim}d row widths", flush=True)
+    ndm, ndf = (3, 6) if dim == 3 else (2, 3)
+    fr = meshes.rc_frame_3d(2, 1, 2) if dim == 3 else meshes.rc_frame_2d(2, 2)
+    build = meshes.build_rc_frame_3d if dim == 3 else meshes.build_rc_frame_2d
+    amps = (20.0, 60.0, -35.0)
+
+    def field(amp, seed):
+        if dim == 3:
+            return lateral_field(fr, amp, seed)
+        rng = np.random.default_rng(seed)
+        y = fr["crds"][:, 1]
+        U = np.zeros((len(y), 3))
+        U[:, 0] = amp * y / y.max()
+        U[:, 1] = -1e-3 * amp * y / y.max()
+        U += rng.normal(0, abs(amp) * 0.05, U.shape) * (y[:, None] > 0)
+        U[:, 2] *= 1e-4
+        return U
+
+    nsec = 5 * len(meshes.frame_elements(fr)[0])
+    assert nsec % 4 and nsec % 3
+    states = []
+    for width in ("0", "16", "20"):
+        monkeypatch.setenv("XG_FRAME_ROWS", width)
+        g, o = Model(ndm, ndf), _oracle.oracle_model(ndm, ndf)
+        for m in (g, o):
+            build(m, fr)
+            m.system("GpuBlockPCG", 1e-14, 100000)
+            m.setup()
+        for step, amp in enumerate(amps):
+            for m in (g, o):
+                m.set_trial_disp(field(amp, step))
+                m.update()
+            for what in ("strain", "stress", "tangent"):
+                pg, po = g.point_data(what, nsec, 16), o.point_data(what, nsec, 16)
+                assert relerr(pg, po) <= RTOL, (width, what, step)
+            for m in (g, o):
+                m.commit()
+        states.append((g.get_state("Steel02"), g.get_state("Concrete02")))
+        for m in (g, o):
+            m.close()
+    for width, s in zip(("16", "20"), states[1:]):
+        for k in range(2):
+            e = relerr(s[k], states[0][k])
+            print(f"width {width} state {k}: max rel. difference {e:.3g}", flush=True)
+            assert e <= 1e-12, (width, k, e)
+
+
 def test_frame3d_revert_matches_reference_semantics():
     print("case: frame3d revert", flush=True)
     fr, (g, o) = frame3d_pair(1, 1, 2)

@@ -351,19 +351,40 @@ struct FrameGroup : Group {
     f.cs_t = cs_t.p;
     f.el_c = el_c.p;
     f.el_t = el_t.p;
-    // the displacement-based update's sections, grouped by definition, four
-    // to a wavefront (XG_FRAME_ROWS=0: one wavefront per section instead)
+    // the displacement-based update's sections, grouped by definition, 64 /
+    // rw to a wavefront.  The row width is the one of 16 and 20 lanes with
+    // fewer wavefront passes per section: a run of n fibers takes ceil(n /
+    // rw) passes over the row (a Steel02 pass counted twice, for its two
+    // powers; an elastic one a quarter), the section deformation and totals
+    // one, all shared by the 64 / rw rows of a wavefront.  XG_FRAME_ROWS=16
+    // or 20 fixes the width, =0 keeps one wavefront per section.
     const char* rows_env = std::getenv("XG_FRAME_ROWS");
     if (!force && nsec > 0 && !(rows_env && rows_env[0] == '0')) {
+      auto passes = [&](int w) {
+        double c = 0.0;
+        for (int e = 0; e < ne; e++) {
+          const int df = secdef[e];
+          double p = 1.0;
+          for (int r = run_off[df]; r < run_off[df + 1]; r++) {
+            const int nf = run_beg[r + 1] - run_beg[r];
+            p += (run_kind[r] == 1 ? 2.0 : run_kind[r] == 2 ? 1.0 : 0.25) * ((nf + w - 1) / w);
+          }
+          c += nip * p / (64 / w);
+        }
+        return c;
+      };
+      const int fixed = rows_env ? std::atoi(rows_env) : 0;
+      const int rw = (fixed == 16 || fixed == 20) ? fixed : (passes(20) < passes(16) ? 20 : 16), nr = 64 / rw;
       std::vector<std::vector<int>> by_def(GJ.size());
       for (long long sec = 0; sec < nsec; sec++) by_def[secdef[sec / nip]].push_back((int)sec);
       std::vector<int> rsec;
       for (const auto& v : by_def)
-        for (size_t i = 0; i < v.size(); i += 4)
-          for (size_t k = 0; k < 4; k++) rsec.push_back(i + k < v.size() ? v[i + k] : -1);
+        for (size_t i = 0; i < v.size(); i += nr)
+          for (size_t k = 0; k < (size_t)nr; k++) rsec.push_back(i + k < v.size() ? v[i + k] : -1);
       drsec.upload(rsec, s);
       f.rsec = drsec.p;
-      f.nrw = (long long)rsec.size() / 4;
+      f.nrw = (long long)rsec.size() / nr;
+      f.rw = rw;
     }
     if (force) setup_force(d, def_off, fkind, fmat, fy, fz, fA, GJ, steelp, concp, elasE);
     stc_init = std::move(stc);

@@ -13,13 +13,25 @@ __global__ __launch_bounds__(256) void k_frame_update(FrameDev f, const double* 
   section_update<D3>(f, sec, lane, U);
 }
 
-// Four sections of one definition per wavefront (section_rows).
+// 64 / RW sections of one definition per wavefront (section_rows): rows of
+// 16 lanes (four sections, totals by butterfly) or 20 (three sections,
+// totals through LDS; held to three waves per SIMD)
 template <bool D3>
 __global__ __launch_bounds__(256) void k_frame_update_rows(FrameDev f, const double* __restrict__ U) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const long long w = (long long)blockIdx.x * 4 + wave;
   if (w >= f.nrw) return;
-  section_rows<D3>(f, w, lane, U);
+  section_rows<D3, 16>(f, w, lane, U, nullptr);
+}
+template <bool D3>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_frame_update_rows20(
+    FrameDev f, const double* __restrict__ U) {
+  constexpr int NV = D3 ? 9 : 5;
+  __shared__ double red[4 * NV * 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long w = (long long)blockIdx.x * 4 + wave;
+  if (w >= f.nrw) return;
+  section_rows<D3, 20>(f, w, lane, U, red + wave * NV * 64);
 }
 
 template <bool D3>
@@ -58,8 +70,13 @@ void launch_frame_update(hipStream_t s, const FrameDev& f, bool d3, const double
   if (f.nsec <= 0) return;
   if (f.rsec) {
     const dim3 g(ceil_div(f.nrw, 4)), b(256);
-    if (d3) hipLaunchKernelGGL(k_frame_update_rows<true>, g, b, 0, s, f, U);
-    else hipLaunchKernelGGL(k_frame_update_rows<false>, g, b, 0, s, f, U);
+    if (f.rw == 20) {
+      if (d3) hipLaunchKernelGGL(k_frame_update_rows20<true>, g, b, 0, s, f, U);
+      else hipLaunchKernelGGL(k_frame_update_rows20<false>, g, b, 0, s, f, U);
+    } else {
+      if (d3) hipLaunchKernelGGL(k_frame_update_rows<true>, g, b, 0, s, f, U);
+      else hipLaunchKernelGGL(k_frame_update_rows<false>, g, b, 0, s, f, U);
+    }
     return;
   }
   const dim3 g(ceil_div(f.nsec, 4)), b(256);

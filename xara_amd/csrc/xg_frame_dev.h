@@ -49,11 +49,12 @@ struct FrameDev {
   double* cs_t = nullptr;
   double* el_c = nullptr;  // elastic fiber strain
   double* el_t = nullptr;
-  // displacement-based update with four sections of one definition per
-  // wavefront (16 lanes each): wavefront w updates sections rsec[4w ..
-  // 4w+3] (-1: an empty row); nullptr: one wavefront per section
+  // displacement-based update with NR = 64 / rw sections of one definition
+  // per wavefront (rw lanes each): wavefront w updates sections rsec[NR w ..
+  // NR w + NR - 1] (-1: an empty row); nullptr: one wavefront per section
   const int* rsec = nullptr;
   long long nrw = 0;
+  int rw = 16;  // lanes per section: 16 (four sections) or 20 (three)
   // ---- force-based elements (ForceBeamColumn2d/3d), NB = 3 or 6 basic
   // forces, NS = ord section forces; element arrays [nele][..], section
   // arrays [nsec][..] ----

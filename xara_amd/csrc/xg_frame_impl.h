@@ -387,21 +387,37 @@ __device__ __forceinline__ double row_sums(const double (&x)[K], int lane) {
   return t;
 }
 
-// The displacement-based section update with four sections of one
-// definition per wavefront, a 16-lane row each (row = lane / 16 takes
-// section rsec[4w + row]): every row evaluates its own section deformation,
-// its lanes walk the section's fibers f = p, p + 16, ... of each run (Steel02
-// and elastic one fiber per lane and pass, Concrete02 two, the odd one
-// alone), and the row totals come from one butterfly over the row.  Against
-// one wavefront per section, the 20 Steel02 fibers of an RC section take
-// half a pass instead of a whole one (20 of 64 lanes busy) and the section
-// sums a quarter of the exchanges.  Same fiber arithmetic as section_fibers
+// Which section sum gives entry p of the section tangent: 3D kData[p]
+// (row-major 4x4) of the sums {k0, k1, k2, k5, k6, k10, ...}, 15 for a zero
+// entry (a sum that is always zero); 2D {k0, k1, k1, k5} of {k0, k1, k5}.
+__device__ __forceinline__ int kv_source(int p) {
+  constexpr unsigned long long kSrc = 0x0ull | (1ull << 4) | (2ull << 8) | (15ull << 12) | (1ull << 16) |
+                                      (3ull << 20) | (4ull << 24) | (15ull << 28) | (2ull << 32) | (4ull << 36) |
+                                      (5ull << 40) | (15ull << 44) | (15ull << 48) | (15ull << 52) | (15ull << 56) |
+                                      (15ull << 60);
+  return (int)((kSrc >> (4 * p)) & 15);
+}
+__device__ __forceinline__ int kv_source2(int p) { return p == 0 ? 0 : (p == 3 ? 2 : 1); }
+
+// The displacement-based section update with NR = 64 / RW sections of one
+// definition per wavefront, a row of RW lanes each (row = lane / RW takes
+// section rsec[NR w + row]; lanes from NR RW on idle): every row evaluates
+// its own section deformation, its lanes walk the section's fibers f = p,
+// p + RW, ... of each run (Steel02 and elastic one fiber per lane and pass,
+// Concrete02 two, the odd one alone), and the row totals come from one
+// butterfly over the row (RW = 16) or from the wavefront's LDS slice red
+// (64 doubles per sum).  Against one wavefront per section, the 20 Steel02
+// fibers of an RC section take half a pass (RW = 16) or a third (RW = 20)
+// instead of a whole one (20 of 64 lanes busy), and the section sums a
+// fraction of the exchanges.  Same fiber arithmetic as section_fibers
 // (FiberSection2d/3d::setTrialSectionDeformation), other summation order.
-template <bool D3>
-__device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int lane, const double* __restrict__ U) {
-  const int row = lane >> 4, pos = lane & 15;
-  const int sec0 = uniform_i(f.rsec[4 * w]);
-  const int secr = f.rsec[4 * w + row];
+template <bool D3, int RW>
+__device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int lane, const double* __restrict__ U,
+                                             double* red) {
+  constexpr int NR = 64 / RW;  // rows per wavefront; lanes from NR * RW on idle
+  const int row = lane / RW, pos = lane - row * RW;
+  const int sec0 = uniform_i(f.rsec[NR * w]);
+  const int secr = row < NR ? f.rsec[NR * w + row] : -1;
   const bool live = secr >= 0;
   const long long sec = live ? secr : sec0;
   const int def = uniform_i(f.secdef[sec0 / f.nip]);
@@ -419,7 +435,7 @@ __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int
       const Steel02Param p{uniform_d(q->Fy), uniform_d(q->E0),  uniform_d(q->b),  uniform_d(q->R0),
                            uniform_d(q->cR1), uniform_d(q->cR2), uniform_d(q->a1), uniform_d(q->a2),
                            uniform_d(q->a3), uniform_d(q->a4),  uniform_d(q->sigini)};
-      for (int fi = b0 + pos; fi < e1; fi += 16) {
+      for (int fi = b0 + pos; fi < e1; fi += RW) {
         const double y = f.fy[fi], z = D3 ? f.fz[fi] : 0.0;
         const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
         double stress, tangent;
@@ -432,8 +448,8 @@ __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int
                               uniform_d(q->rat),   uniform_d(q->ft),    uniform_d(q->Ets)};
       const long long n = f.nconc;
       int fi = b0 + pos;
-      for (; fi + 16 < e1; fi += 32) {
-        const int fj = fi + 16;
+      for (; fi + RW < e1; fi += 2 * RW) {
+        const int fj = fi + RW;
         const long long ia = cb + f.flocal[fi], ib = cb + f.flocal[fj];
         const double ya = f.fy[fi], za = D3 ? f.fz[fi] : 0.0;
         const double yb = f.fy[fj], zb = D3 ? f.fz[fj] : 0.0;
@@ -484,7 +500,7 @@ __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int
       }
     } else {
       const double E = uniform_d(f.elasE[mat]);
-      for (int fi = b0 + pos; fi < e1; fi += 16) {
+      for (int fi = b0 + pos; fi < e1; fi += RW) {
         const double y = f.fy[fi], z = D3 ? f.fz[fi] : 0.0;
         const double strain = D3 ? d[0] - y * d[1] + z * d[2] : d[0] - y * d[1];
         f.el_t[eb + f.flocal[fi]] = strain;
@@ -493,21 +509,52 @@ __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int
     }
   }
   const long long n = f.nsec;
-  const int base = lane & 48;
+  // the sums: 3D {k0, k1, k2, k5, k6, k10, s0, s1, s2}, 2D {k0, k1, k5, s0, s1}
+  constexpr int NV = D3 ? 9 : 5, K = D3 ? 16 : 8;
+  double x[K];
+#pragma unroll
+  for (int v = 0; v < K; v++) x[v] = 0.0;
+  if constexpr (D3) {
+    x[0] = S.k0, x[1] = S.k1, x[2] = S.k2, x[3] = S.k5, x[4] = S.k6, x[5] = S.k10;
+    x[6] = S.s0, x[7] = S.s1, x[8] = S.s2;
+  } else {
+    x[0] = S.k0, x[1] = S.k1, x[2] = S.k5, x[3] = S.s0, x[4] = S.s1;
+  }
+  // row_total(v): the row's total of value v of x, in every lane that asks
+  double kv, sv;
+  if constexpr (RW == 16) {
+    // butterfly over the row: lane p holds value p / (16 / K)
+    const double t = row_sums<K>(x, lane);
+    const int base = lane & 48;
+    if (D3) {
+      kv = __shfl(t, base | kv_source(pos), 64);
+      sv = __shfl(t, base | (6 + (pos & 3)), 64);
+    } else {
+      kv = __shfl(t, base | (2 * kv_source2(pos)), 64);
+      sv = __shfl(t, base | (6 + 2 * (pos & 1)), 64);
+    }
+  } else {
+    // other widths: the partial sums through this wavefront's LDS slice
+    // red[v * 64 + lane], each total summed by one lane over the row in lane
+    // order (a wavefront's LDS accesses complete in order: no barrier)
+#pragma unroll
+    for (int v = 0; v < NV; v++) red[v * 64 + lane] = x[v];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int r0l = row < NR ? row * RW : 0;
+    const int kvs = D3 ? kv_source(pos & 15) : kv_source2(pos & 3);
+    const int svs = D3 ? 6 + (pos & 3) : 3 + (pos & 1);
+    kv = 0.0;
+    sv = 0.0;
+    if (kvs < NV)
+      for (int j = 0; j < RW; j++) kv += red[kvs * 64 + r0l + j];
+    if (svs < NV)
+      for (int j = 0; j < RW; j++) sv += red[svs * 64 + r0l + j];
+  }
   if (D3) {
-    // totals: lane p of the row holds value p of x; kData[p] (row-major 4x4)
-    // is value kSrc[p] (15: a zero), the resultant s0..s2 values 6..8
-    const double x[16] = {S.k0, S.k1, S.k2, S.k5, S.k6, S.k10, S.s0, S.s1, S.s2, 0, 0, 0, 0, 0, 0, 0};
-    const double t = row_sums<16>(x, lane);
-    constexpr unsigned long long kSrc = 0x0ull | (1ull << 4) | (2ull << 8) | (15ull << 12) | (1ull << 16) |
-                                        (3ull << 20) | (4ull << 24) | (15ull << 28) | (2ull << 32) | (4ull << 36) |
-                                        (5ull << 40) | (15ull << 44) | (15ull << 48) | (15ull << 52) | (15ull << 56) |
-                                        (15ull << 60);
-    const int src = (int)((kSrc >> (4 * pos)) & 15);
-    const double kv = __shfl(t, base | src, 64);
-    const double sv = __shfl(t, base | (6 + (pos & 3)), 64);
     const double GJ = uniform_d(f.GJ[def]);
-    if (live) {
+    if (live && pos < 16) {
       f.ks[pos * n + sec] = pos == 15 ? (GJ >= 0.0 ? GJ : 0.0) : kv;
       if (pos < 4) {
         const double dv = pos == 0 ? d[0] : pos == 1 ? d[1] : pos == 2 ? d[2] : d[3];
@@ -516,13 +563,7 @@ __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int
       }
     }
   } else {
-    // totals: lanes 2v, 2v+1 of the row hold value v of x; kData = {k0, k1,
-    // k1, k5}, the resultant s0, s1 values 3, 4
-    const double x[8] = {S.k0, S.k1, S.k5, S.s0, S.s1, 0, 0, 0};
-    const double t = row_sums<8>(x, lane);
-    const int src = pos == 0 ? 0 : (pos == 3 ? 4 : 2);
-    const double kv = __shfl(t, base | src, 64);
-    const double sv = __shfl(t, base | (6 + 2 * (pos & 1)), 64);
+    // kData = {k0, k1, k1, k5}, the resultant s0, s1
     if (live) {
       if (pos < 4) f.ks[pos * n + sec] = kv;
       if (pos < 2) {
