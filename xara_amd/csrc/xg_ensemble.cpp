@@ -254,6 +254,9 @@ void Ensemble::setup() {
   M.nfail.resize(1);
   M.nfail.zero(s);
   G.setup(M);  // geometry, section definitions, fiber pools, zero update
+  // the fiber update's wavefront items grouped by model (a model's items
+  // never span workgroups)
+  const std::vector<int> row_off = frame_rows_by_parts(G, ele_off, s);
   nnode = M.nnode;
   neq = eq_off.back();
   nele = ele_off.back();
@@ -316,6 +319,10 @@ void Ensemble::setup() {
   E.ndf = 3;
   E.node_off = d_node_off.p;
   E.ele_off = d_ele_off.p;
+  if (!row_off.empty()) {
+    d_row_off.upload(row_off, s);
+    E.row_off = d_row_off.p;
+  }
   E.eq_off = d_eq_off.p;
   E.band_off = d_band_off.p;
   E.bw = d_bw.p;

@@ -13,6 +13,7 @@ struct EnsDev {
   // per model (size nmodel or nmodel+1)
   const int* node_off = nullptr;
   const int* ele_off = nullptr;
+  const int* row_off = nullptr;   // first wavefront item of the frame rows (f.rsec), nullptr: none
   const int* eq_off = nullptr;
   const long long* band_off = nullptr;
   const int* bw = nullptr;

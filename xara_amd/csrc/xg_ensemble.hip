@@ -62,7 +62,14 @@ template <int NT, bool First>
 __device__ void model_update_and_unbalance(const EnsDev& E, int m, double ag) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const long long s0 = (long long)E.ele_off[m] * E.f.nip, s1 = (long long)E.ele_off[m + 1] * E.f.nip;
-  for (long long sec = s0 + wave; sec < s1; sec += NT / 64) section_update<false, First>(E.f, sec, lane, E.U);
+  if (E.row_off && E.f.rw == 20) {
+    // three sections of one element per wavefront item, a 20-lane row each
+    __shared__ double red[NT / 64 * 5 * 64];
+    for (long long w = E.row_off[m] + wave; w < E.row_off[m + 1]; w += NT / 64)
+      section_rows<false, 20, First>(E.f, w, lane, E.U, red + wave * 5 * 64);
+  } else {
+    for (long long sec = s0 + wave; sec < s1; sec += NT / 64) section_update<false, First>(E.f, sec, lane, E.U);
+  }
   __syncthreads();
   for (int e = E.ele_off[m] + tid; e < E.ele_off[m + 1]; e += NT) element_resid<false>(E.f, e, E.Re);
   __syncthreads();

@@ -41,7 +41,7 @@ struct Ensemble {
   int block = 256, grid = 0, order_mode = 0;
   long long steps_done = 0;
   std::vector<int> order_h, size_h, model_ele, node_off_h;
-  DVec<int> d_node_off, d_ele_off, d_eq_off, d_bw, d_bent_off, d_gm_off, d_gm_n, d_eqmap, d_eq2dof, d_order;
+  DVec<int> d_node_off, d_ele_off, d_row_off, d_eq_off, d_bw, d_bent_off, d_gm_off, d_gm_n, d_eqmap, d_eq2dof, d_order;
   DVec<long long> d_band_off, d_bent_pos;
   DVec<int> d_bent_ptr, d_bent_idx, d_rhs_ptr, d_rhs_idx;
   DVec<double> d_gm_dt, d_gm_fac, d_alphaM, d_M, d_R, d_gm;

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <map>
 
 #include "xg_frame_dev.h"
 
@@ -351,40 +352,28 @@ struct FrameGroup : Group {
     f.cs_t = cs_t.p;
     f.el_c = el_c.p;
     f.el_t = el_t.p;
-    // the displacement-based update's sections, grouped by definition, 64 /
-    // rw to a wavefront.  The row width is the one of 16 and 20 lanes with
-    // fewer wavefront passes per section: a run of n fibers takes ceil(n /
-    // rw) passes over the row (a Steel02 pass counted twice, for its two
-    // powers; an elastic one a quarter), the section deformation and totals
-    // one, all shared by the 64 / rw rows of a wavefront.  XG_FRAME_ROWS=16
-    // or 20 fixes the width, =0 keeps one wavefront per section.
+    // the displacement-based update in rows (section_rows): passes per
+    // wavefront of each definition for rows of 16 and 20 lanes -- a run of
+    // n fibers takes ceil(n / rw) passes over the row (a Steel02 pass counted
+    // twice, for its two powers; an elastic one a quarter), the section
+    // deformation and totals one.  XG_FRAME_ROWS=16 or 20 fixes the width,
+    // =0 keeps one wavefront per section.
     const char* rows_env = std::getenv("XG_FRAME_ROWS");
-    if (!force && nsec > 0 && !(rows_env && rows_env[0] == '0')) {
-      auto passes = [&](int w) {
-        double c = 0.0;
-        for (int e = 0; e < ne; e++) {
-          const int df = secdef[e];
-          double p = 1.0;
-          for (int r = run_off[df]; r < run_off[df + 1]; r++) {
-            const int nf = run_beg[r + 1] - run_beg[r];
-            p += (run_kind[r] == 1 ? 2.0 : run_kind[r] == 2 ? 1.0 : 0.25) * ((nf + w - 1) / w);
-          }
-          c += nip * p / (64 / w);
-        }
-        return c;
-      };
+    rows_on = !force && nsec > 0 && !(rows_env && rows_env[0] == '0');
+    if (rows_on) {
       const int fixed = rows_env ? std::atoi(rows_env) : 0;
-      const int rw = (fixed == 16 || fixed == 20) ? fixed : (passes(20) < passes(16) ? 20 : 16), nr = 64 / rw;
-      std::vector<std::vector<int>> by_def(GJ.size());
-      for (long long sec = 0; sec < nsec; sec++) by_def[secdef[sec / nip]].push_back((int)sec);
-      std::vector<int> rsec;
-      for (const auto& v : by_def)
-        for (size_t i = 0; i < v.size(); i += nr)
-          for (size_t k = 0; k < (size_t)nr; k++) rsec.push_back(i + k < v.size() ? v[i + k] : -1);
-      drsec.upload(rsec, s);
-      f.rsec = drsec.p;
-      f.nrw = (long long)rsec.size() / nr;
-      f.rw = rw;
+      rows_fixed = (fixed == 16 || fixed == 20) ? fixed : 0;
+      rows_secdef = secdef;
+      rows_pass16.assign(GJ.size(), 1.0);
+      rows_pass20.assign(GJ.size(), 1.0);
+      for (size_t df = 0; df < GJ.size(); df++)
+        for (int r = run_off[df]; r < run_off[df + 1]; r++) {
+          const int nf = run_beg[r + 1] - run_beg[r];
+          const double wk = run_kind[r] == 1 ? 2.0 : run_kind[r] == 2 ? 1.0 : 0.25;
+          rows_pass16[df] += wk * ((nf + 15) / 16);
+          rows_pass20[df] += wk * ((nf + 19) / 20);
+        }
+      build_rows({0, ne}, s);
     }
     if (force) setup_force(d, def_off, fkind, fmat, fy, fz, fA, GJ, steelp, concp, elasE);
     stc_init = std::move(stc);
@@ -494,6 +483,62 @@ struct FrameGroup : Group {
   }
   std::vector<double> stc_init, csc_init;
   std::vector<int> konc_init;
+
+  // rows of the displacement-based update (section_rows)
+  bool rows_on = false;
+  int rows_fixed = 0;                            // XG_FRAME_ROWS=16 / 20, else 0
+  std::vector<int> rows_secdef;                  // section definition of each element
+  std::vector<double> rows_pass16, rows_pass20;  // wavefront passes per definition
+
+  // Sections grouped by (part, definition), the parts being the element
+  // ranges [parts[p], parts[p+1]), 64 / rw of a group to a wavefront, the
+  // row width rw the one of 16 and 20 with fewer passes in all (rows left
+  // empty at the end of a group counted); returns the first wavefront of
+  // each part and the end (empty if rows are off).
+  std::vector<int> build_rows(const std::vector<int>& parts, hipStream_t s) {
+    std::vector<int> off;
+    if (!rows_on) return off;
+    std::vector<std::pair<int, std::vector<int>>> groups;  // (definition, sections)
+    std::vector<size_t> part_first;
+    for (size_t p = 0; p + 1 < parts.size(); p++) {
+      part_first.push_back(groups.size());
+      std::map<int, size_t> at;
+      for (int e = parts[p]; e < parts[p + 1]; e++) {
+        const int df = rows_secdef[e];
+        auto it = at.find(df);
+        if (it == at.end()) {
+          it = at.emplace(df, groups.size()).first;
+          groups.push_back({df, {}});
+        }
+        for (int i = 0; i < nip; i++) groups[it->second].second.push_back(e * nip + i);
+      }
+    }
+    part_first.push_back(groups.size());
+    auto cost = [&](int w) {
+      const size_t nr = 64 / w;
+      double c = 0.0;
+      for (const auto& g : groups) c += (double)((g.second.size() + nr - 1) / nr) * (w == 20 ? rows_pass20 : rows_pass16)[g.first];
+      return c;
+    };
+    const int rw = rows_fixed ? rows_fixed : (cost(20) < cost(16) ? 20 : 16);
+    const size_t nr = 64 / rw;
+    std::vector<int> rsec;
+    for (size_t p = 0; p + 1 < part_first.size(); p++) {
+      off.push_back((int)(rsec.size() / nr));
+      for (size_t g = part_first[p]; g < part_first[p + 1]; g++) {
+        const std::vector<int>& v = groups[g].second;
+        for (size_t i = 0; i < v.size(); i += nr)
+          for (size_t k = 0; k < nr; k++) rsec.push_back(i + k < v.size() ? v[i + k] : -1);
+      }
+    }
+    off.push_back((int)(rsec.size() / nr));
+    if (rsec.empty()) rsec.push_back(-1);
+    drsec.upload(rsec, s);
+    f.rsec = drsec.p;
+    f.nrw = off.back();
+    f.rw = rw;
+    return off;
+  }
 
   void zero_update(Domain& d) {
     DVec<double> zeroU;
@@ -643,6 +688,12 @@ FrameDev frame_dev(Group& g) {
   auto* fg = dynamic_cast<FrameGroup*>(&g);
   if (!fg) throw Error(XG_ERR_ARG, "not a frame-element group");
   return fg->f;
+}
+
+std::vector<int> frame_rows_by_parts(Group& g, const std::vector<int>& ele_off, hipStream_t s) {
+  auto* fg = dynamic_cast<FrameGroup*>(&g);
+  if (!fg) throw Error(XG_ERR_ARG, "not a frame-element group");
+  return fg->build_rows(ele_off, s);
 }
 
 std::unique_ptr<Group> make_frame_group(Domain& d, bool force) {

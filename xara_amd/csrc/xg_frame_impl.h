@@ -411,7 +411,7 @@ __device__ __forceinline__ int kv_source2(int p) { return p == 0 ? 0 : (p == 3 ?
 // instead of a whole one (20 of 64 lanes busy), and the section sums a
 // fraction of the exchanges.  Same fiber arithmetic as section_fibers
 // (FiberSection2d/3d::setTrialSectionDeformation), other summation order.
-template <bool D3, int RW>
+template <bool D3, int RW, bool PrevCommitted = false>
 __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int lane, const double* __restrict__ U,
                                              double* red) {
   constexpr int NR = 64 / RW;  // rows per wavefront; lanes from NR * RW on idle
@@ -447,6 +447,9 @@ __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int
       const Concrete02Param p{uniform_d(q->fc),    uniform_d(q->epsc0), uniform_d(q->fcu), uniform_d(q->epscu),
                               uniform_d(q->rat),   uniform_d(q->ft),    uniform_d(q->Ets)};
       const long long n = f.nconc;
+      // Concrete02's previous trial stress and tangent (kept on its early
+      // return): the committed ones on the first update after a commit
+      const double* prev = PrevCommitted ? f.cs_c : f.cs_t;
       int fi = b0 + pos;
       for (; fi + RW < e1; fi += 2 * RW) {
         const int fj = fi + RW;
@@ -459,8 +462,8 @@ __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int
           ca[k] = f.cs_c[k * n + ia];
           cb2[k] = f.cs_c[k * n + ib];
         }
-        double sa = f.cs_t[3 * n + ia], ea = f.cs_t[4 * n + ia];
-        double sb2 = f.cs_t[3 * n + ib], eb2 = f.cs_t[4 * n + ib];
+        double sa = prev[3 * n + ia], ea = prev[4 * n + ia];
+        double sb2 = prev[3 * n + ib], eb2 = prev[4 * n + ib];
         const double Aa = f.fA[fi], Ab = f.fA[fj];
         const double ea_ = D3 ? d[0] - ya * d[1] + za * d[2] : d[0] - ya * d[1];
         const double eb_ = D3 ? d[0] - yb * d[1] + zb * d[2] : d[0] - yb * d[1];
@@ -486,7 +489,7 @@ __device__ __forceinline__ void section_rows(const FrameDev& f, long long w, int
         double ca[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) ca[k] = f.cs_c[k * n + ia];
-        double sa = f.cs_t[3 * n + ia], ea = f.cs_t[4 * n + ia];
+        double sa = prev[3 * n + ia], ea = prev[4 * n + ia];
         const double Aa = f.fA[fi];
         const double ea_ = D3 ? d[0] - ya * d[1] + za * d[2] : d[0] - ya * d[1];
         double m1, d1, e1c;
