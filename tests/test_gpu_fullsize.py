@@ -1,4 +1,4 @@
-"""GPU parity at the full sizes of the two continuum workloads:
+"""GPU parity at the full sizes of the continuum and frame workloads:
 
 * the 64 x 16 plane-strain FourNodeQuad J2 strip (2,176 equations) through
   its whole 50-step DisplacementControl pushover to 0.05, against the oracle
@@ -8,7 +8,10 @@
   numbering and CSR pattern bit-exact, and after one update on a field that
   leaves most points plastic, every point's stress and tangent, the
   unbalance and all CSR values against the oracle; then a PCG solve checked
-  by its true residual in the GPU's own assembled matrix (scipy product)."""
+  by its true residual in the GPU's own assembled matrix (scipy product);
+* the 10 x 10-bay, 20-story 3D RC frame (4,092,000 fibers, 14,520
+  equations): two DisplacementControl steps and one update past first
+  cracking against the oracle."""
 import numpy as np
 import pytest
 
@@ -110,3 +113,62 @@ def test_cube_64_full_state_determination():
     assert np.linalg.norm(r) <= 1e-9 * np.linalg.norm(g["B"])
     assert gm.counters()["last_cg_relres"] <= 1e-10
     gm.close()
+
+
+def test_rc_frame_10x10x20_full_size():
+    """The 10x10-bay, 20-story 3D RC frame of dispBeamColumn3d elements at
+    its full size (6,820 elements, 34,100 sections, 4,092,000 fibers, 14,520
+    equations): numbering and CSR pattern bit-exact; two
+    DisplacementControl steps of the roof (GPU PCG preconditioned by the
+    tiled band factor, half bandwidth 731; the oracle's direct solve) with
+    the trajectory and load factor to 1e-9; then, from that committed state,
+    one update on a lateral field past first cracking: every section's
+    deformation, resultant and tangent (the fiber update in rows of 20
+    lanes), the unbalance and all CSR values against the oracle to
+    1e-9."""
+    print("case: rc frame 10x10x20 full size", flush=True)
+    fr = meshes.rc_frame_3d(10, 10, 20)
+    z = fr["crds"][:, 2]
+    h = z.max()
+    rng = np.random.default_rng(5)
+    U = np.zeros((len(z), 6))
+    U[:, 0] = 60.0 * (z / h) * (1.5 - 0.5 * z / h)
+    U[:, 1] = 15.0 * (z / h) ** 2
+    U[:, 2] = -1.2e-2 * z / h
+    U[:, 4] = 90.0 / h * (1 - z / h)
+    U += rng.normal(0, 0.5, U.shape) * (z[:, None] > 0) * np.array([1, 1, 1e-3, 1e-5, 1e-5, 1e-5])
+    nsec = 5 * len(meshes.frame_elements(fr)[0])
+    out = []
+    for kind in ("gpu", "oracle"):
+        m = Model(3, 6) if kind == "gpu" else _oracle.oracle_model(3, 6)
+        meshes.build_rc_frame_3d(m, fr)
+        m.system("GpuBlockPCG", 1e-12, 20000)
+        m.test("NormUnbalance", 1e-3, 40)
+        m.integrator("DisplacementControl", fr["roof"], 1, 0.03 * fr["height"] / 300.0)
+        m.setup()
+        rowptr, colind = m.csr_pattern()
+        res = dict(eq=m.node_eqns(), rowptr=rowptr, colind=colind, traj=[])
+        for step in range(2):
+            assert m.analyze(1) == 0, step
+            res["traj"].append((m.trial_disp().copy(), m.load_factor))
+        res["counters"] = m.counters()
+        # then one update from the committed state of the second step
+        m.set_trial_disp(U)
+        assert m.update() == 0
+        m.set_load_factor(1000.0)
+        res.update(B=m.form_unbalance(), vals=m.form_tangent(),
+                   **{w: m.point_data(w, nsec, 16) for w in ("strain", "stress", "tangent")})
+        out.append(res)
+        m.close()
+        print(f"  {kind} done", flush=True)
+    g, o = out
+    assert np.array_equal(g["eq"], o["eq"])
+    assert np.array_equal(g["rowptr"], o["rowptr"]) and np.array_equal(g["colind"], o["colind"])
+    assert len(g["rowptr"]) - 1 == 14520
+    for key in ("strain", "stress", "tangent", "B", "vals"):
+        assert relerr(g[key], o[key]) <= RTOL, key
+    for k, ((ug, lg), (uo, lo)) in enumerate(zip(g["traj"], o["traj"])):
+        assert relerr(ug, uo) <= RTOL, k
+        assert abs(lg - lo) <= RTOL * abs(lo), k
+    c = g["counters"]
+    assert c["band_bw"] > 63 and c["cg_iters"] <= 3 * c["solves"], c
