@@ -128,6 +128,69 @@ def test_frame_update_row_widths(dim, monkeypatch):
             assert e <= 1e-12, (width, k, e)
 
 
+def build_mixed_frame_3d(m, fr):
+    """The frame of rc_frame_3d with two section definitions of mixed fiber
+    kinds: columns of core (Concrete02, tag 3) and cover (Concrete02, tag 1)
+    concrete, 20 Steel02 bars and four elastic corner plates; beams of an
+    8 x 8 concrete patch, 20 bars and two elastic plates."""
+    m.nodes(fr["node_tags"], fr["crds"])
+    fx = fr["fixed_nodes"]
+    m.fix_many(fx, np.ones((len(fx), 6), np.int32))
+    m.uniaxialMaterial("Concrete02", 1, *meshes.CONCRETE02)
+    m.uniaxialMaterial("Steel02", 2, *meshes.STEEL02)
+    m.uniaxialMaterial("Concrete02", 3, -36.0, -0.0025, -25.0, -0.03, 0.1, 3.0, 1500.0)
+    m.uniaxialMaterial("Elastic", 4, 2.0e5)
+    col = []
+    for y, z, A, mat in meshes.rc_section_fibers():
+        core = mat == 1 and abs(y) < 150.0 and abs(z) < 150.0
+        col.append((y, z, A, 3 if core else mat))
+    col += [(sy * 240.0, sz * 240.0, 40.0, 4) for sy in (-1, 1) for sz in (-1, 1)]
+    beam = list(meshes.rc_section_fibers(width=400.0, depth=600.0, nc=8))
+    beam += [(-290.0, 0.0, 120.0, 4), (290.0, 0.0, 120.0, 4)]
+    m.section("Fiber", 1, col, GJ=1.1e14)
+    m.section("Fiber", 2, beam, GJ=1.1e14)
+    m.geomTransf("Linear", 1, 1.0, 0.0, 0.0)
+    m.geomTransf("Linear", 2, 0.0, 0.0, 1.0)
+    nc, nx_, ny_ = len(fr["columns"]), len(fr["beams_x"]), len(fr["beams_y"])
+    m.elements("dispBeamColumn", np.arange(1, nc + 1), fr["columns"], 1, 1, 5)
+    m.elements("dispBeamColumn", np.arange(nc + 1, nc + nx_ + 1), fr["beams_x"], 2, 2, 5)
+    m.elements("dispBeamColumn", np.arange(nc + nx_ + 1, nc + nx_ + ny_ + 1), fr["beams_y"], 2, 2, 5)
+
+
+@pytest.mark.parametrize("width", ["0", "16", "20"])
+def test_frame_rows_mixed_sections(width, monkeypatch):
+    """Two section definitions (60 and 70 sections: rows left empty at the
+    end of each definition's group) with runs of two Concrete02 materials,
+    Steel02 and elastic fibers, in each layout of the update: section
+    strains, resultants and tangents, the unbalance and the assembled
+    tangent against the oracle to 1e-9 over three loading steps, with a
+    commit after each."""
+    print(f"case: frame3d mixed sections width {width}", flush=True)
+    monkeypatch.setenv("XG_FRAME_ROWS", width)
+    fr = meshes.rc_frame_3d(2, 1, 2)
+    nsec = 5 * (len(fr["columns"]) + len(fr["beams_x"]) + len(fr["beams_y"]))
+    g, o = Model(3, 6), _oracle.oracle_model(3, 6)
+    for m in (g, o):
+        build_mixed_frame_3d(m, fr)
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        m.setup()
+    for step, amp in enumerate((25.0, 70.0, -40.0)):
+        U = lateral_field(fr, amp, 10 + step)
+        for m in (g, o):
+            m.set_trial_disp(U)
+            m.update()
+        for what in ("strain", "stress", "tangent"):
+            assert relerr(g.point_data(what, nsec, 16), o.point_data(what, nsec, 16)) <= RTOL, (what, step)
+        for m in (g, o):
+            m.set_load_factor(0.0)
+        assert relerr(g.form_unbalance(), o.form_unbalance()) <= RTOL, step
+        assert relerr(g.form_tangent(), o.form_tangent()) <= RTOL, step
+        for m in (g, o):
+            m.commit()
+    for m in (g, o):
+        m.close()
+
+
 def test_frame3d_revert_matches_reference_semantics():
     print("case: frame3d revert", flush=True)
     fr, (g, o) = frame3d_pair(1, 1, 2)
