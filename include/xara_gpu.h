@@ -80,6 +80,9 @@ typedef struct xg_counters {
   long long band_fallbacks;   /* matrices whose tiled band factor met a pivot
                                  that is not positive and were solved with
                                  block Jacobi instead                      */
+  long long band_reuses;      /* assembled matrices equal bit for bit to the
+                                 last one factored, whose band factor was
+                                 kept instead of formed again              */
 } xg_counters;
 
 /* ---- lifetime -------------------------------------------------------------- */

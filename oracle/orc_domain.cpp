@@ -1563,6 +1563,7 @@ int orc_get_counters(xg_domain* d, xg_counters* c) {
   *c = d->counters;
   c->band_bw = -1;  // direct solve, no preconditioner
   c->band_fallbacks = 0;
+  c->band_reuses = 0;
   return 0;
 }
 // Wall time of `reps` serial updates (the reference execution model,

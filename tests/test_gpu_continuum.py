@@ -559,3 +559,34 @@ def test_block_jacobi_with_partly_fixed_nodes(system):
     ref = _pcg_iterations(K, b, blocks, rtol)
     assert abs(its - ref) <= 2, (its, ref)
     assert relerr(x, o.solve(b)) <= 1e-9
+
+
+def test_band_factor_kept_for_an_unchanged_matrix(monkeypatch):
+    """The 64 x 16 FourNodeQuad strip stays elastic through a
+    DisplacementControl pushover, so every tangent after the first equals
+    the first bit for bit: its band factor is kept (counted in band_reuses)
+    instead of formed again, and the trajectory equals, bit for bit, the one
+    with every matrix factored (XG_PC_REUSE=0)."""
+    print("case: band factor reuse on the elastic quad strip", flush=True)
+    mesh = meshes.quad_strip(64, 16)
+    runs = []
+    for reuse in ("1", "0"):
+        monkeypatch.setenv("XG_PC_REUSE", reuse)
+        m = Model(2, 2)
+        meshes.build_quad_strip(m, mesh)
+        m.load(mesh["tip"], 0.0, 1.0)
+        m.system("GpuBlockPCG", 1e-14, 100000)
+        m.test("NormUnbalance", 1e-8, 25)
+        m.integrator("DisplacementControl", mesh["tip"], 2, 0.001)
+        m.setup()
+        traj = []
+        for k in range(8):
+            assert m.analyze(1) == 0, k
+            traj.append((m.trial_disp().copy(), m.load_factor))
+        c = m.counters()
+        runs.append((traj, c))
+        m.close()
+    (ta, ca), (tb, cb) = runs
+    assert ca["band_bw"] > 0 and ca["band_reuses"] > 0 and cb["band_reuses"] == 0, (ca, cb)
+    for (ua, la), (ub, lb) in zip(ta, tb):
+        assert np.array_equal(ua, ub) and la == lb

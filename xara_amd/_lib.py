@@ -31,6 +31,7 @@ class Counters(C.Structure):
         ("time_solve_ms", C.c_double),
         ("band_bw", C.c_longlong),
         ("band_fallbacks", C.c_longlong),
+        ("band_reuses", C.c_longlong),
     ]
 
 

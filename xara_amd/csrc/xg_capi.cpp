@@ -718,6 +718,7 @@ int xg_get_counters(xg_domain* h, xg_counters* c) {
     *c = d.counters;
     c->band_bw = d.band_active() ? d.band_bw : (d.wide_active() && !d.small_path()) ? d.wb_bw : -1;
     c->band_fallbacks = d.wb_fallbacks;
+    c->band_reuses = d.pc_reuses;
     return 0;
   });
 }

@@ -838,6 +838,27 @@ void Domain::precond_setup(const CgBufs& w) {
                     : sweep ? (const void*)w.binvs
                             : (const void*)w.binv;
   if (pc_done == key) return;
+  // the domain's own factor storage (the allocation-free solve's narrow band
+  // factor lives in its scratch instead)
+  const bool own = (band && w.bandL == band_L.p) || wide;
+  if (own && pc_cache && pc_cache_key == key) {
+    XG_HIP(hipMemsetAsync(pc_diff.p, 0, sizeof(int), stream));
+    launch_vals_differ(stream, (long long)num.colind.size(), vals.p, pc_vals.p, pc_diff.p);
+    int diff = 1;
+    XG_HIP(hipMemcpyAsync(&diff, pc_diff.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+    XG_HIP(hipStreamSynchronize(stream));
+    if (!diff) {  // the same matrix: its factor is in place (and had no bad pivot)
+      XG_HIP(hipMemsetAsync(w.pcbad, 0, sizeof(int), stream));
+      pc_reuses++;
+      pc_done = key;
+      return;
+    }
+  }
+  if (own) {
+    pc_cache = false;
+    pc_fresh = pc_reuse_on && pc_vals.n >= num.colind.size();
+    pc_fresh_key = key;
+  }
   XG_HIP(hipMemsetAsync(w.pcbad, 0, sizeof(int), stream));
   if (wide) {
     wb_factor(w.pcbad);
@@ -976,6 +997,9 @@ static int band_width(const Numbering& num, const std::vector<int>& pinv) {
 void Domain::band_order() {
   band_bw = -1;
   wb_bw = -1;
+  pc_cache = pc_fresh = false;
+  pc_vals.release();
+  pc_diff.release();
   band_pinv.release();
   band_A.release();
   band2.release();
@@ -1034,6 +1058,7 @@ void Domain::band_order() {
     wb_D.resize(wband_diag_doubles(n));
     wb_Y.resize(6 * np);
     wb_v.resize(4 * (size_t)n);
+    pc_buffers();
     return;
   }
   band_bw = best;
@@ -1043,6 +1068,18 @@ void Domain::band_order() {
   scal2.resize(cg_scal_bytes());
   band_L.resize((size_t)n * (best + 1));
   band_Lr.resize((size_t)n * (best + 1));
+  pc_buffers();
+}
+
+// The copy of the factored matrix's values (and the comparison flag), for
+// keeping the band factor of an unchanged matrix; XG_PC_REUSE=0 in the
+// environment at setup refactors every assembled matrix instead.
+void Domain::pc_buffers() {
+  const char* env = std::getenv("XG_PC_REUSE");
+  pc_reuse_on = !(env && env[0] == '0');
+  if (!pc_reuse_on) return;
+  pc_vals.resize(std::max<size_t>(num.colind.size(), 1));
+  pc_diff.resize(1);
 }
 
 // Destroys the captured chunks once the stream has drained: a replay still
@@ -1158,7 +1195,16 @@ bool Domain::precond_ready(const CgBufs& w) {
     int bad = 0;
     XG_HIP(hipMemcpyAsync(&bad, w.pcbad, sizeof(int), hipMemcpyDeviceToHost, stream));
     XG_HIP(hipStreamSynchronize(stream));
-    if (!bad) return true;
+    if (!bad) {
+      if (pc_fresh) {  // a factor just formed in the domain's own storage: remember its matrix
+        launch_copy_d(stream, (long long)num.colind.size(), vals.p, pc_vals.p);
+        pc_cache = true;
+        pc_cache_key = pc_fresh_key;
+      }
+      pc_fresh = false;
+      return true;
+    }
+    pc_fresh = false;
     if (!wide) return false;
     wb_failed = true;
     wb_fallbacks++;

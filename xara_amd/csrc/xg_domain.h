@@ -256,10 +256,21 @@ struct Domain {
   // the PCG without it until the next assembly
   bool wb_failed = false;
   long long wb_fallbacks = 0;
+  // the values of the matrix whose band factor (either kind) the domain's
+  // own factor storage holds: a matrix equal to them bit for bit (a model
+  // still elastic) keeps its factor instead of being factored again
+  // (XG_PC_REUSE=0 turns this off)
+  DVec<double> pc_vals;
+  DVec<int> pc_diff;
+  bool pc_reuse_on = true, pc_cache = false, pc_fresh = false;
+  const void* pc_cache_key = nullptr;
+  const void* pc_fresh_key = nullptr;
+  long long pc_reuses = 0;
   bool wide_active() const {
     return cg_small_on && cg_wide_on && !wb_failed && wb_bw >= 0 && band_bw < 0 && integ != 3;
   }
   void wb_factor(int* bad);
+  void pc_buffers();
   void wb_apply(int k, const double* const* r, double* const* z, bool graphs);
   int solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w, bool graphs);
   // the preconditioner's buffers last formed from the current matrix

@@ -71,6 +71,9 @@ struct SymSpmv {
 };
 // Rows of B doubles to the sweep order of a symmetric layout (dst[p*B + t] =
 // src[ord[p]*B + t]) or back (dst[ord[p]*B + t] = src[p*B + t]).
+// *flag = 1 if a[i] and b[i] differ in any bit for some i < n (the flag
+// cleared by the caller)
+void launch_vals_differ(hipStream_t s, long long n, const double* a, const double* b, int* flag);
 void launch_sweep_perm(hipStream_t s, int nbr, int B, const int* ord, const double* src, double* dst, bool back);
 // workgroups (= partial sums) of one SpMV launch
 int spmv_grid(int n, int lanes, const SymSpmv* sym);

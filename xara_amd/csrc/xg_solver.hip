@@ -1659,6 +1659,19 @@ void launch_sym_convert(hipStream_t s, const SymSpmv& S, int nup, const int* src
   else throw Error(XG_ERR_ARG, "symmetric SpMV block size must be 2 or 3");
 }
 
+__global__ __launch_bounds__(256) void k_vals_differ(long long n, const double* __restrict__ a,
+                                                     const double* __restrict__ b, int* __restrict__ flag) {
+  bool d = false;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    d |= __double_as_longlong(a[i]) != __double_as_longlong(b[i]);
+  if (d) *flag = 1;  // every writer stores the same value
+}
+void launch_vals_differ(hipStream_t s, long long n, const double* a, const double* b, int* flag) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_vals_differ, dim3((unsigned)std::min<long long>((n + 255) / 256, 2048)), dim3(256), 0, s, n, a,
+                     b, flag);
+}
+
 void launch_sweep_perm(hipStream_t s, int nbr, int B, const int* ord, const double* src, double* dst, bool back) {
   if (nbr <= 0 || B <= 0) return;
   const long long total = (long long)nbr * B;
