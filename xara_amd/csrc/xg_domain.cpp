@@ -1221,7 +1221,7 @@ int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs,
   double rr = 0, bb = 0;
   if (band_active()) {
     // one workgroup, one launch, preconditioned by the band's exact factor
-    const BandRhs R{b, x, w.r0, w.z, w.p, w.q, w.r1, (CGScal*)w.scal};
+    const BandRhs R{b, x, w.r0, w.z, w.p, w.q, w.r1, (CGScal*)w.scal, warm ? 1 : 0};
     cg_solve_band(stream, n, rowptr.p, colind.p, vals.p, band_bw, band_pinv.p, w.bandL, w.bandLr, rtol * rtol, maxit,
                   R);
     cg_read_scal(stream, w.scal, &conv, &it, &rr, &bb, &brk);

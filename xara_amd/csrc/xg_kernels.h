@@ -123,6 +123,7 @@ struct BandRhs {
   double* x;
   double *r, *z, *p, *q, *t;
   CGScal* S;
+  int warm = 0;  // x holds a guess: start from r = b - A x (stop test unchanged)
 };
 void cg_solve_band(hipStream_t s, int n, const int* rowptr, const int* colind, const double* vals, int bw,
                    const int* pinv, const double* L, const double* Lr, double tol2, int maxit, const BandRhs& r0,

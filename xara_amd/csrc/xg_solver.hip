@@ -1306,23 +1306,50 @@ __global__ __launch_bounds__(kSmallPcgThreads) void k_pcg_band(
     for (int i = tid; i < n; i += NT) z[i] = t[pinv[i]];
     __syncthreads();
   };
-  for (int i = tid; i < n; i += NT) {
-    x[i] = 0.0;
-    r[i] = b[i];
+  double l1 = 0.0, l2 = 0.0, rz = 0.0, bb = 0.0, rr = 0.0;
+  bool done;
+  if (!R.warm) {
+    for (int i = tid; i < n; i += NT) {
+      x[i] = 0.0;
+      r[i] = b[i];
+    }
+    __syncthreads();
+    precond();
+    for (int i = tid; i < n; i += NT) {
+      p[i] = z[i];
+      l1 += r[i] * z[i];
+      l2 += r[i] * r[i];
+    }
+    rz = block_sum_1024(l1, sh);
+    bb = block_sum_1024(l2, sh);
+    rr = bb;
+    done = bb == 0.0;
+  } else {
+    // from the guess in x: r = b - A x; a guess already within the
+    // tolerance ends the solve before the first iteration
+    double lb = 0.0;
+    for (int i = tid; i < n; i += NT) {
+      double s = 0.0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) s += vals[k] * x[colind[k]];
+      const double ri = b[i] - s;
+      r[i] = ri;
+      l2 += ri * ri;
+      lb += b[i] * b[i];
+    }
+    bb = block_sum_1024(lb, sh);
+    rr = block_sum_1024(l2, sh);
+    done = bb == 0.0 || rr <= tol2 * bb;
+    if (!done) {
+      precond();
+      for (int i = tid; i < n; i += NT) {
+        p[i] = z[i];
+        l1 += r[i] * z[i];
+      }
+      rz = block_sum_1024(l1, sh);
+    }
   }
-  __syncthreads();
-  precond();
-  double l1 = 0.0, l2 = 0.0;
-  for (int i = tid; i < n; i += NT) {
-    p[i] = z[i];
-    l1 += r[i] * z[i];
-    l2 += r[i] * r[i];
-  }
-  double rz = block_sum_1024(l1, sh);
-  const double bb = block_sum_1024(l2, sh);
-  double rr = bb, pq = 0.0, alpha = 0.0, beta = 0.0;
+  double pq = 0.0, alpha = 0.0, beta = 0.0;
   int it = 0, brk = 0;
-  bool done = bb == 0.0;
   while (!done) {
     double lpq = 0.0;
     for (int i = tid; i < n; i += NT) {
