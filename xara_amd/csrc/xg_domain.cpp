@@ -914,7 +914,8 @@ void Domain::wb_apply(int k, const double* const* r, double* const* z, bool grap
 // set of sweeps): from x = 0, stopping at |r| <= rtol |b| like the other
 // paths.  The factor makes the first iterate the direct solution up to
 // rounding, so a solve takes one or two iterations.
-int Domain::solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w, bool graphs) {
+int Domain::solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w, bool graphs,
+                       unsigned warm) {
   const int n = num.neq;
   const double tol2 = rtol * rtol;
   double* vr[2] = {w.r0, wb_v.p};
@@ -925,10 +926,19 @@ int Domain::solve_wide(int R, const double* const* b, double* const* x, const Cg
   bool conv[2] = {true, true}, fail[2] = {false, false};
   int its[2] = {0, 0};
   for (int s = 0; s < R; s++) {
-    wband_zero(stream, n, x[s]);
     launch_copy(stream, n, b[s], vr[s]);
     bb[s] = dot_dev(b[s], b[s], n);
     conv[s] = !(bb[s] > 0);
+    if (!((warm >> s) & 1u) || conv[s]) {
+      wband_zero(stream, n, x[s]);
+    } else {
+      // from the guess in x (bit s of warm): r = b - A x; a guess already
+      // within the tolerance ends the solve before the first iteration
+      spmv(x[s], vq[s], true);
+      launch_axpby(stream, n, -1.0, vq[s], 1.0, vr[s]);
+      rr[s] = dot_dev(vr[s], vr[s], n);
+      conv[s] = rr[s] <= tol2 * bb[s];
+    }
   }
   auto apply_open = [&] {
     const double* rs[2];
@@ -1228,7 +1238,7 @@ int Domain::solve_with(const double* b, double* x, const CgBufs& w, bool graphs,
   } else if (!small_path() && w.wide && wide_active()) {
     const double* bs[1] = {b};
     double* xs[1] = {x};
-    return solve_wide(1, bs, xs, w, graphs);
+    return solve_wide(1, bs, xs, w, graphs, warm ? 1u : 0u);
   } else if (small_path()) {
     // the whole solve in one workgroup, one launch
     cg_solve_small(stream, n, rowptr.p, colind.p, vals.p, b, x, w.r0, w.z, w.p, w.q, bw, w.binv, bfirst.p, bsize.p,

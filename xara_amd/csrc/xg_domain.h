@@ -272,7 +272,8 @@ struct Domain {
   void wb_factor(int* bad);
   void pc_buffers();
   void wb_apply(int k, const double* const* r, double* const* z, bool graphs);
-  int solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w, bool graphs);
+  // warm: bit s set when x[s] holds a guess to start right-hand side s from
+  int solve_wide(int R, const double* const* b, double* const* x, const CgBufs& w, bool graphs, unsigned warm = 0);
   // the preconditioner's buffers last formed from the current matrix
   // (nullptr: the matrix changed since), so repeated solves of one matrix
   // form it once
