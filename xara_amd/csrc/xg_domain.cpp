@@ -1159,7 +1159,9 @@ int Domain::solve_pair(const double* b1, double* x1, const double* b2, double* x
     if (wide_active()) {
       const double* bs[2] = {b1, b2};
       double* xs[2] = {x1, x2};
-      return solve_wide(2, bs, xs, w, cg_graph_on);
+      // the second right-hand side is DisplacementControl's reference load:
+      // its last solution in x2 is the start
+      return solve_wide(2, bs, xs, w, cg_graph_on, (cg_warm_on && dc_hat_valid) ? 2u : 0u);
     }
     // the factor gave way to block Jacobi: one solve after the other
     const int r = solve(b1, x1);
