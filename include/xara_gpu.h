@@ -198,8 +198,9 @@ int  xg_spmv_config(xg_domain* d, int format, int lanes);
 #define XG_CG_WARM  8  /* under LoadControl the first Newton iteration of a step
                           starts from the previous step's first solution,
                           DisplacementControl's reference-load solves from the
-                          previous one (the chunked PCG; the default; the stop
-                          test stays |r| <= rtol |b|)                          */
+                          previous one (the chunked PCG and the band-factor
+                          preconditioned solves; the default; the stop test
+                          stays |r| <= rtol |b|)                               */
 #define XG_CG_WIDE  16 /* with XG_CG_SMALL, under LoadControl and
                           DisplacementControl: systems of more than 1,024
                           equations whose half bandwidth in the band order is
